@@ -1,0 +1,80 @@
+// Internal declarations shared by the operator translation units.
+#pragma once
+
+#include "common.h"
+
+#include <vector>
+
+namespace imp {
+
+// Resident state of one Toeplitz normal operator A = sum_c S_c^H T S_c + lambda R.
+struct ToepPlan {
+    int N1 = 0, N2 = 0, N3 = 1;   // image [N1][N2][N3] = [Ny][Nx][Nz]
+    int M1 = 0, M2 = 0, M3 = 1;   // 2x grid (M3 = 1 in 2D)
+    long G = 0;                   // M1*M2*M3
+    long nvox = 0;                // N1*N2*N3
+    int nseg = 1;                 // L + 1 interpolators
+    int ncoils = 1;
+    int coils_per_group = 8;      // pass-E coil grouping
+    float tau = 0.f, t0 = 0.f;
+    float lambda = 0.f;
+    float2 *qf = nullptr;         // [nseg][G]  FFT(ifftshift(Q_l))
+    float2 *g1 = nullptr;         // [ncoils][N1][A][Ml] pass buffer
+    float2 *partial = nullptr;    // [ngroups][nvox]
+    float2 *sens = nullptr;       // [ncoils][nvox] or null (S = 1)
+    float *fcyc = nullptr;        // [nvox] field map in Hz (w / 2pi)
+    float *w = nullptr;           // [nvox] TV weight or null (W = I), not owned
+    // CG work vectors (interleaved)
+    float2 *vb = nullptr, *vx = nullptr, *vr = nullptr, *vp = nullptr,
+           *vq = nullptr, *vbest = nullptr;
+    double *scal = nullptr;       // device CG scalars
+};
+
+int twiddles(imp_ctx *ctx, int L, const float2 **out);
+int fft_grid(imp_ctx *ctx, float2 *data, int M1, int M2, int M3, bool inverse);
+int toep_prepare_qf(imp_ctx *ctx, ToepPlan &P, const float *q_r, const float *q_i);
+int toep_apply_normal(imp_ctx *ctx, ToepPlan &P, const float2 *p, float2 *out);
+bool toep_sizes_supported(int N1, int N2, int N3);
+
+// vector / stencil helpers (vec.hip)
+int split_to_c(imp_ctx *ctx, const float *re, const float *im, float2 *out, long n);
+int c_to_split(imp_ctx *ctx, const float2 *in, float *re, float *im, long n);
+int fd_reg(imp_ctx *ctx, int N1, int N2, int N3, float lambda, const float2 *p,
+           const float *w, float2 *out, bool accumulate);
+int tv_weight(imp_ctx *ctx, int N1, int N2, int N3, const float2 *x, float *w);
+// Re(a^H b) in FP64, result to host (synchronous)
+int dot_re(imp_ctx *ctx, const float2 *a, const float2 *b, long n, double *out);
+int axpy_c(imp_ctx *ctx, float alpha, const float2 *x, float2 *y, long n);  // y += alpha x
+int toep_cg(imp_ctx *ctx, ToepPlan &P, const float2 *b, float2 *x, int max_iter,
+            double tol, int flags, imp_cg_stats *stats);
+int toep_apply_full(imp_ctx *ctx, ToepPlan &P, const float2 *p, float2 *out);
+
+// NUDFT (nudft.hip)
+int nudft_adjoint(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *kx,
+                  const float *ky, const float *kz, const float *t, const float *x,
+                  const float *y, const float *z, const float *fm,
+                  const float *phi_r, const float *phi_i, const float *d_r,
+                  const float *d_i, const float *s_r, const float *s_i,
+                  float *out_r, float *out_i);
+int nudft_forward(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *kx,
+                  const float *ky, const float *kz, const float *t, const float *x,
+                  const float *y, const float *z, const float *fm,
+                  const float *rho_r, const float *rho_i, const float *s_r,
+                  const float *s_i, float *d_r, float *d_i);
+int toeplitz_q(imp_ctx *ctx, int num_k, const float *kx, const float *ky,
+               const float *kz, const float *t, const float *phi_r,
+               const float *phi_i, int Nx, int Ny, int Nz, int num_q,
+               const float *gx, const float *gy, const float *gz, int ntime_segs,
+               int flags, const float *t_host_first_last, float *q_r, float *q_i);
+
+// internal CG flag: track the best iterate in ToepPlan::vbest but leave the
+// last iterate in x
+constexpr int CG_KEEP_LAST = 0x100;
+
+}  // namespace imp
+
+struct imp_toep {
+    imp_ctx *ctx = nullptr;
+    imp::ToepPlan P;
+    std::vector<void *> owned;
+};

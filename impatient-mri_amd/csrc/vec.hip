@@ -1,0 +1,419 @@
+// Vector kernels, the periodic finite-difference regularizer and the
+// device-resident CG of the Toeplitz solver.
+//
+// The reference reduces dot products with a multi-pass in-place tree whose
+// blocks read and write the same buffer (recon.cu:585-700,
+// dotProductGpu.cu:200-278) and finishes the sum in FP32 on the host.  Here a
+// dot product is one pass of per-workgroup FP64 partials (wavefront shuffles,
+// then LDS) into a separate buffer and one single-workgroup FP64 finish; the
+// CG scalars alpha, beta and the best-iterate test never leave the device.
+#include "common.h"
+#include "toeplitz.h"
+
+#include <chrono>
+#include <cmath>
+
+namespace imp {
+
+namespace {
+
+constexpr int VT = 256;
+constexpr int RED_BLOCKS = 1024;  // max partials per reduction
+
+__global__ void k_split_to_c(const float *re, const float *im, float2 *out, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = make_float2(re[i], im ? im[i] : 0.f);
+}
+__global__ void k_c_to_split(const float2 *in, float *re, float *im, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const float2 v = in[i];
+        re[i] = v.x;
+        im[i] = v.y;
+    }
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// block-wide FP64 sum; result valid in thread 0
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double sm[VT / 64];
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) sm[wid] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < VT / 64; ++w) s += sm[w];
+    __syncthreads();
+    return s;
+}
+
+// partials[block] = sum Re(a^H b)
+__global__ __launch_bounds__(VT) void k_dot_re(const float2 *__restrict__ a,
+                                               const float2 *__restrict__ b, long n,
+                                               double *__restrict__ partials) {
+    double s = 0.0;
+    for (long i = (long)blockIdx.x * VT + threadIdx.x; i < n; i += (long)gridDim.x * VT) {
+        const float2 x = a[i], y = b[i];
+        s = fma((double)x.x, (double)y.x, s);
+        s = fma((double)x.y, (double)y.y, s);
+    }
+    s = block_sum(s);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(VT) void k_sum_partials(const double *__restrict__ partials,
+                                                     int np, double *__restrict__ out) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < np; i += VT) s += partials[i];
+    s = block_sum(s);
+    if (threadIdx.x == 0) *out = s;
+}
+
+__global__ void k_axpy_c(float alpha, const float2 *x, float2 *y, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const float2 a = x[i];
+        float2 b = y[i];
+        b.x += alpha * a.x;
+        b.y += alpha * a.y;
+        y[i] = b;
+    }
+}
+
+// s = lambda * sum_d D_d^T W D_d p (periodic), optionally added to out.
+// (D_d p)(i) = p(i) - p(i+e_d);  (D_d^T t)(i) = t(i) - t(i-e_d)
+__global__ void k_fd_reg(int N1, int N2, int N3, float lambda, const float2 *__restrict__ p,
+                         const float *__restrict__ w, float2 *__restrict__ out, int accumulate) {
+    const long n = (long)N1 * N2 * N3;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int z = (int)(i % N3);
+    const int x = (int)((i / N3) % N2);
+    const int y = (int)(i / ((long)N3 * N2));
+    const float2 pc = p[i];
+    const float wc = w ? w[i] : 1.f;
+    float sx = 0.f, sy = 0.f;
+    // axes: y (stride N2*N3), x (stride N3), z (stride 1, 3D only)
+    const int ndim = N3 > 1 ? 3 : 2;
+    for (int d = 0; d < ndim; ++d) {
+        long ip, im;
+        if (d == 0) {
+            ip = (long)((y + 1) % N1) * N2 * N3 + (long)x * N3 + z;
+            im = (long)((y + N1 - 1) % N1) * N2 * N3 + (long)x * N3 + z;
+        } else if (d == 1) {
+            ip = (long)y * N2 * N3 + (long)((x + 1) % N2) * N3 + z;
+            im = (long)y * N2 * N3 + (long)((x + N2 - 1) % N2) * N3 + z;
+        } else {
+            ip = (long)y * N2 * N3 + (long)x * N3 + (z + 1) % N3;
+            im = (long)y * N2 * N3 + (long)x * N3 + (z + N3 - 1) % N3;
+        }
+        const float2 pp = p[ip], pm = p[im];
+        const float wm = w ? w[im] : 1.f;
+        // w_i (p_i - p_{i+e}) - w_{i-e} (p_{i-e} - p_i)
+        sx += wc * (pc.x - pp.x) - wm * (pm.x - pc.x);
+        sy += wc * (pc.y - pp.y) - wm * (pm.y - pc.y);
+    }
+    float2 r = make_float2(lambda * sx, lambda * sy);
+    if (accumulate) {
+        const float2 o = out[i];
+        r.x += o.x;
+        r.y += o.y;
+    }
+    out[i] = r;
+}
+
+// w = 0.5 / sqrt(sum_d |D_d x|^2 + 1e-6)  (multiplyGpuKernel2d/3d)
+__global__ void k_tv_weight(int N1, int N2, int N3, const float2 *__restrict__ xv,
+                            float *__restrict__ w) {
+    const long n = (long)N1 * N2 * N3;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int z = (int)(i % N3);
+    const int x = (int)((i / N3) % N2);
+    const int y = (int)(i / ((long)N3 * N2));
+    const float2 c = xv[i];
+    const long ih = (long)y * N2 * N3 + (long)((x + 1) % N2) * N3 + z;   // Dhori
+    const long iv = (long)((y + 1) % N1) * N2 * N3 + (long)x * N3 + z;   // Dverti
+    float2 dh = make_float2(c.x - xv[ih].x, c.y - xv[ih].y);
+    float2 dv = make_float2(c.x - xv[iv].x, c.y - xv[iv].y);
+    float s = dh.x * dh.x + dh.y * dh.y + dv.x * dv.x + dv.y * dv.y;
+    if (N3 > 1) {
+        const long iz = (long)y * N2 * N3 + (long)x * N3 + (z + 1) % N3;  // Dzaxis
+        const float2 dz = make_float2(c.x - xv[iz].x, c.y - xv[iz].y);
+        s += dz.x * dz.x + dz.y * dz.y;
+    }
+    w[i] = 0.5f / sqrtf(s + 0.000001f);
+}
+
+// ------------------------------------------------------------------ CG
+// device scalar slots
+enum {
+    SC_BETA = 0,     // |r|^2 of the current residual
+    SC_BNORM = 1,    // |b|^2
+    SC_ALPHA = 2,
+    SC_RATIO = 3,    // beta2 / beta
+    SC_BEST = 4,     // best |r|^2 so far
+    SC_IMPROVED = 5, // 1 if this iteration improved the best residual
+    SC_BREAK = 6,    // breakdown flag
+    SC_PQ = 7,
+    SC_N = 8
+};
+
+__global__ void k_cg_init(double *sc, const double *partials, int np) {
+    // r = b (x0 = 0) or r = b - A x0: partials hold |r|^2; bnorm set separately
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        double s = 0.0;
+        for (int i = 0; i < np; ++i) s += partials[i];
+        sc[SC_BETA] = s;
+        sc[SC_BEST] = s;
+        sc[SC_IMPROVED] = 0.0;
+        sc[SC_BREAK] = 0.0;
+    }
+}
+
+__global__ __launch_bounds__(VT) void k_cg_alpha(const double *__restrict__ partials, int np,
+                                                 double *__restrict__ sc) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < np; i += VT) s += partials[i];
+    s = block_sum(s);
+    if (threadIdx.x == 0) {
+        sc[SC_PQ] = s;
+        if (!(s > 0.0) || !isfinite(s)) {
+            sc[SC_BREAK] = 1.0;
+            sc[SC_ALPHA] = 0.0;
+        } else {
+            sc[SC_ALPHA] = sc[SC_BETA] / s;
+        }
+    }
+}
+
+// x += alpha p; r -= alpha q; partial |r|^2
+__global__ __launch_bounds__(VT) void k_cg_update_xr(float2 *__restrict__ x, float2 *__restrict__ r,
+                                                     const float2 *__restrict__ p,
+                                                     const float2 *__restrict__ q, long n,
+                                                     const double *__restrict__ sc,
+                                                     double *__restrict__ partials) {
+    const float alpha = (float)sc[SC_ALPHA];
+    double s = 0.0;
+    for (long i = (long)blockIdx.x * VT + threadIdx.x; i < n; i += (long)gridDim.x * VT) {
+        const float2 pv = p[i], qv = q[i];
+        float2 xv = x[i], rv = r[i];
+        xv.x += alpha * pv.x;
+        xv.y += alpha * pv.y;
+        rv.x -= alpha * qv.x;
+        rv.y -= alpha * qv.y;
+        x[i] = xv;
+        r[i] = rv;
+        s = fma((double)rv.x, (double)rv.x, s);
+        s = fma((double)rv.y, (double)rv.y, s);
+    }
+    s = block_sum(s);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(VT) void k_cg_beta(const double *__restrict__ partials, int np,
+                                                double *__restrict__ sc) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < np; i += VT) s += partials[i];
+    s = block_sum(s);
+    if (threadIdx.x == 0) {
+        const double beta = sc[SC_BETA];
+        sc[SC_RATIO] = beta > 0.0 ? s / beta : 0.0;
+        sc[SC_BETA] = s;
+        if (s < sc[SC_BEST]) {
+            sc[SC_BEST] = s;
+            sc[SC_IMPROVED] = 1.0;
+        } else {
+            sc[SC_IMPROVED] = 0.0;
+        }
+    }
+}
+
+// p = r + ratio p; if improved: best = x
+__global__ void k_cg_update_p(float2 *__restrict__ p, const float2 *__restrict__ r,
+                              const float2 *__restrict__ x, float2 *__restrict__ best, long n,
+                              const double *__restrict__ sc, int track_best) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float ratio = (float)sc[SC_RATIO];
+    const float2 rv = r[i];
+    float2 pv = p[i];
+    pv.x = rv.x + ratio * pv.x;
+    pv.y = rv.y + ratio * pv.y;
+    p[i] = pv;
+    if (track_best && sc[SC_IMPROVED] != 0.0) best[i] = x[i];
+}
+
+__global__ void k_sub_c(const float2 *a, const float2 *b, float2 *out, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = make_float2(a[i].x - b[i].x, a[i].y - b[i].y);
+}
+
+int red_blocks(long n) {
+    long nb = (n + VT * 4 - 1) / (VT * 4);
+    if (nb < 1) nb = 1;
+    if (nb > RED_BLOCKS) nb = RED_BLOCKS;
+    return (int)nb;
+}
+
+}  // namespace
+
+int split_to_c(imp_ctx *ctx, const float *re, const float *im, float2 *out, long n) {
+    hipLaunchKernelGGL(k_split_to_c, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, re, im,
+                       out, n);
+    IMP_CHECK_LAUNCH();
+    return IMP_OK;
+}
+
+int c_to_split(imp_ctx *ctx, const float2 *in, float *re, float *im, long n) {
+    hipLaunchKernelGGL(k_c_to_split, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, in, re,
+                       im, n);
+    IMP_CHECK_LAUNCH();
+    return IMP_OK;
+}
+
+int fd_reg(imp_ctx *ctx, int N1, int N2, int N3, float lambda, const float2 *p, const float *w,
+           float2 *out, bool accumulate) {
+    const long n = (long)N1 * N2 * N3;
+    hipLaunchKernelGGL(k_fd_reg, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, N1, N2, N3,
+                       lambda, p, w, out, accumulate ? 1 : 0);
+    IMP_CHECK_LAUNCH();
+    return IMP_OK;
+}
+
+int tv_weight(imp_ctx *ctx, int N1, int N2, int N3, const float2 *x, float *w) {
+    const long n = (long)N1 * N2 * N3;
+    hipLaunchKernelGGL(k_tv_weight, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, N1, N2,
+                       N3, x, w);
+    IMP_CHECK_LAUNCH();
+    return IMP_OK;
+}
+
+int dot_re(imp_ctx *ctx, const float2 *a, const float2 *b, long n, double *out) {
+    IMP_TRY(ensure_dscratch(ctx, RED_BLOCKS + 16));
+    const int nb = red_blocks(n);
+    hipLaunchKernelGGL(k_dot_re, dim3(nb), dim3(VT), 0, ctx->stream, a, b, n, ctx->dscratch);
+    IMP_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(VT), 0, ctx->stream, ctx->dscratch, nb,
+                       ctx->dscratch + RED_BLOCKS);
+    IMP_CHECK_LAUNCH();
+    IMP_HIP(hipMemcpyAsync(ctx->host_scalars, ctx->dscratch + RED_BLOCKS, sizeof(double),
+                           hipMemcpyDeviceToHost, ctx->stream));
+    IMP_HIP(hipStreamSynchronize(ctx->stream));
+    *out = ctx->host_scalars[0];
+    return IMP_OK;
+}
+
+int axpy_c(imp_ctx *ctx, float alpha, const float2 *x, float2 *y, long n) {
+    hipLaunchKernelGGL(k_axpy_c, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, alpha, x, y,
+                       n);
+    IMP_CHECK_LAUNCH();
+    return IMP_OK;
+}
+
+// out = A p = normal(p) + lambda R p
+int toep_apply_full(imp_ctx *ctx, ToepPlan &P, const float2 *p, float2 *out) {
+    IMP_TRY(toep_apply_normal(ctx, P, p, out));
+    if (P.lambda != 0.f) IMP_TRY(fd_reg(ctx, P.N1, P.N2, P.N3, P.lambda, p, P.w, out, true));
+    return IMP_OK;
+}
+
+int toep_cg(imp_ctx *ctx, ToepPlan &P, const float2 *b, float2 *x, int max_iter, double tol,
+            int flags, imp_cg_stats *stats) {
+    const auto t_start = std::chrono::steady_clock::now();
+    const long n = P.nvox;
+    const int nb = red_blocks(n);
+    IMP_TRY(ensure_dscratch(ctx, RED_BLOCKS + 16));
+    double *partials = ctx->dscratch;
+    double *sc = P.scal;
+    const bool best = (flags & IMP_CG_BEST_ITERATE) != 0;
+    const dim3 g1(ceil_div(n, 256)), b1(256);
+
+    // |b|^2
+    double bnorm = 0.0;
+    IMP_TRY(dot_re(ctx, b, b, n, &bnorm));
+    if (stats) *stats = imp_cg_stats{};
+    if (!(bnorm > 0.0)) {
+        IMP_HIP(hipMemsetAsync(x, 0, n * sizeof(float2), ctx->stream));
+        IMP_HIP(hipStreamSynchronize(ctx->stream));
+        if (stats) stats->converged = 1;
+        return IMP_OK;
+    }
+    // r = b - A x0 (x0 = 0: r = b, the reference's wasted apply at
+    // recon.cu:2036 is skipped since A 0 = 0)
+    if (flags & IMP_CG_USE_X0) {
+        IMP_TRY(toep_apply_full(ctx, P, x, P.vq));
+        hipLaunchKernelGGL(k_sub_c, g1, b1, 0, ctx->stream, b, P.vq, P.vr, n);
+    } else {
+        IMP_HIP(hipMemsetAsync(x, 0, n * sizeof(float2), ctx->stream));
+        IMP_HIP(hipMemcpyAsync(P.vr, b, n * sizeof(float2), hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    IMP_CHECK_LAUNCH();
+    IMP_HIP(hipMemcpyAsync(P.vp, P.vr, n * sizeof(float2), hipMemcpyDeviceToDevice, ctx->stream));
+    IMP_HIP(hipMemcpyAsync(P.vbest, x, n * sizeof(float2), hipMemcpyDeviceToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_dot_re, dim3(nb), dim3(VT), 0, ctx->stream, P.vr, P.vr, n, partials);
+    IMP_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_cg_init, dim3(1), dim3(1), 0, ctx->stream, sc, partials, nb);
+    IMP_CHECK_LAUNCH();
+
+    int iter = 0, best_iter = 0;
+    bool converged = false;
+    double beta2 = 0.0;
+    while (iter < max_iter) {
+        IMP_TRY(toep_apply_full(ctx, P, P.vp, P.vq));
+        hipLaunchKernelGGL(k_dot_re, dim3(nb), dim3(VT), 0, ctx->stream, P.vp, P.vq, n, partials);
+        hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(VT), 0, ctx->stream, partials, nb, sc);
+        hipLaunchKernelGGL(k_cg_update_xr, dim3(nb), dim3(VT), 0, ctx->stream, x, P.vr, P.vp,
+                           P.vq, n, sc, partials);
+        hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(VT), 0, ctx->stream, partials, nb, sc);
+        hipLaunchKernelGGL(k_cg_update_p, g1, b1, 0, ctx->stream, P.vp, P.vr, x, P.vbest, n, sc,
+                           best ? 1 : 0);
+        IMP_CHECK_LAUNCH();
+        IMP_HIP(hipMemcpyAsync(ctx->host_scalars, sc, SC_N * sizeof(double),
+                               hipMemcpyDeviceToHost, ctx->stream));
+        IMP_HIP(hipStreamSynchronize(ctx->stream));
+        ++iter;
+        const double *h = ctx->host_scalars;
+        if (h[SC_BREAK] != 0.0)
+            return fail(IMP_EBREAKDOWN, "CG breakdown at iteration %d: p^H A p = %g", iter,
+                        h[SC_PQ]);
+        beta2 = h[SC_BETA];
+        if (h[SC_IMPROVED] != 0.0) best_iter = iter;
+        if (std::sqrt(beta2 / bnorm) < tol) {
+            converged = true;
+            break;
+        }
+    }
+    const double best_res = ctx->host_scalars[SC_BEST];
+    if (best && !(flags & CG_KEEP_LAST)) IMP_HIP(hipMemcpyAsync(x, P.vbest, n * sizeof(float2), hipMemcpyDeviceToDevice,
+                                     ctx->stream));
+    if (stats) {
+        stats->iterations = iter;
+        stats->converged = converged ? 1 : 0;
+        stats->best_iteration = best ? best_iter : iter;
+        stats->rel_residual = std::sqrt(beta2 / bnorm);
+        stats->best_rel_residual = std::sqrt(best_res / bnorm);
+        stats->true_rel_residual = -1.0;
+    }
+    if (flags & IMP_CG_TRUE_RESIDUAL) {
+        IMP_TRY(toep_apply_full(ctx, P, x, P.vq));
+        hipLaunchKernelGGL(k_sub_c, g1, b1, 0, ctx->stream, P.vq, b, P.vr, n);
+        IMP_CHECK_LAUNCH();
+        double rr = 0.0;
+        IMP_TRY(dot_re(ctx, P.vr, P.vr, n, &rr));
+        if (stats) stats->true_rel_residual = std::sqrt(rr / bnorm);
+    }
+    IMP_HIP(hipStreamSynchronize(ctx->stream));
+    if (stats)
+        stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start)
+                             .count();
+    return IMP_OK;
+}
+
+}  // namespace imp

@@ -1,0 +1,197 @@
+"""NumPy-facing wrappers of the device operators (upload, call, download).
+
+Every function here runs the HIP kernels of libimpatient.so through the C
+ABI; none has a host fallback.  Complex arrays are numpy complex64/128 and
+are split into the float32 real / imaginary arrays the C ABI takes.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import (CGStats, Context, DeviceArray, IMP_CG_BEST_ITERATE, call, ptr)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _split(ctx: Context, z):
+    z = np.asarray(z)
+    return ctx.to_device(_f32(z.real)), ctx.to_device(_f32(z.imag))
+
+
+def _opt(ctx: Context, a):
+    return None if a is None else ctx.to_device(_f32(a))
+
+
+def _pair(ctx: Context, n):
+    return ctx.alloc((n,)), ctx.alloc((n,))
+
+
+def _cplx(re: DeviceArray, im: DeviceArray, shape=None):
+    z = re.get().astype(np.float32) + 1j * im.get().astype(np.float32)
+    z = z.astype(np.complex64)
+    return z.reshape(shape) if shape is not None else z
+
+
+def nudft_adjoint(ctx: Context, kx, ky, kz, t, x, y, z, fm, d, sens=None, phi=None):
+    """sum_c conj(S_c) sum_k conj(phi) d_c(k) e^{+i(2pi k.x + fm t)}; d is [C, M]."""
+    d = np.atleast_2d(np.asarray(d))
+    ncoils, nk = d.shape
+    nx = len(x)
+    a = [ctx.to_device(_f32(v)) for v in (kx, ky, t, x, y)]
+    akz, az, afm = _opt(ctx, kz), _opt(ctx, z), _opt(ctx, fm)
+    dr, di = _split(ctx, d.reshape(-1))
+    sr = si = pr = pi = None
+    if sens is not None:
+        sr, si = _split(ctx, np.asarray(sens).reshape(-1))
+    if phi is not None:
+        pr, pi = _split(ctx, np.asarray(phi).reshape(-1))
+    orr, oi = _pair(ctx, nx)
+    call("imp_nudft_adjoint", ctx.h, C.c_int(nk), C.c_int(nx), C.c_int(ncoils),
+         ptr(a[0]), ptr(a[1]), ptr(akz), ptr(a[2]), ptr(a[3]), ptr(a[4]), ptr(az), ptr(afm),
+         ptr(pr), ptr(pi), ptr(dr), ptr(di), ptr(sr), ptr(si), ptr(orr), ptr(oi))
+    return _cplx(orr, oi)
+
+
+def nudft_forward(ctx: Context, kx, ky, kz, t, x, y, z, fm, rho, sens=None, ncoils=None):
+    """d_c(k) = sum_x S_c(x) rho(x) e^{-i(2pi k.x + fm t)}; returns [C, M]."""
+    nk, nx = len(kx), len(x)
+    if ncoils is None:
+        ncoils = 1 if sens is None else int(np.asarray(sens).size // nx)
+    a = [ctx.to_device(_f32(v)) for v in (kx, ky, t, x, y)]
+    akz, az, afm = _opt(ctx, kz), _opt(ctx, z), _opt(ctx, fm)
+    rr, ri = _split(ctx, np.asarray(rho).reshape(-1))
+    sr = si = None
+    if sens is not None:
+        sr, si = _split(ctx, np.asarray(sens).reshape(-1))
+    dr, di = _pair(ctx, ncoils * nk)
+    call("imp_nudft_forward", ctx.h, C.c_int(nk), C.c_int(nx), C.c_int(ncoils),
+         ptr(a[0]), ptr(a[1]), ptr(akz), ptr(a[2]), ptr(a[3]), ptr(a[4]), ptr(az), ptr(afm),
+         ptr(rr), ptr(ri), ptr(sr), ptr(si), ptr(dr), ptr(di))
+    return _cplx(dr, di, (ncoils, nk))
+
+
+def toeplitz_q(ctx: Context, kx, ky, kz, t, Nx, Ny, Nz, ntime_segs, phi=None, flags=0,
+               grid=None):
+    """Q_l on the standard 2x grid (or on `grid` = (gx, gy, gz)); returns [(L+1), G]."""
+    nk = len(kx)
+    a = [ctx.to_device(_f32(v)) for v in (kx, ky, t)]
+    akz = _opt(ctx, kz)
+    pr = pi = None
+    if phi is not None:
+        pr, pi = _split(ctx, np.asarray(phi).reshape(-1))
+    if grid is None:
+        nq = 4 * Nx * Ny * (2 * Nz if Nz > 1 else 1)
+        g = (None, None, None)
+    else:
+        nq = len(grid[0])
+        g = tuple(_opt(ctx, v) for v in grid)
+    nseg = ntime_segs + 1
+    qr, qi = _pair(ctx, nseg * nq)
+    call("imp_toeplitz_q", ctx.h, C.c_int(nk), ptr(a[0]), ptr(a[1]), ptr(akz), ptr(a[2]),
+         ptr(pr), ptr(pi), C.c_int(Nx), C.c_int(Ny), C.c_int(Nz), C.c_int(nq),
+         ptr(g[0]), ptr(g[1]), ptr(g[2]), C.c_int(ntime_segs), C.c_int(flags), ptr(qr), ptr(qi))
+    return _cplx(qr, qi, (nseg, nq))
+
+
+def fd_reg(ctx: Context, p, lam, w=None):
+    """lambda * sum_d D_d^T W D_d p for p of shape (N1, N2) or (N1, N2, N3)."""
+    p = np.asarray(p)
+    shp = p.shape + (1,) * (3 - p.ndim)
+    pr, pi = _split(ctx, p.reshape(-1))
+    aw = _opt(ctx, None if w is None else np.asarray(w).reshape(-1))
+    sr, si = _pair(ctx, p.size)
+    call("imp_fd_reg_apply", ctx.h, C.c_int(shp[0]), C.c_int(shp[1]), C.c_int(shp[2]),
+         C.c_float(lam), ptr(pr), ptr(pi), ptr(aw), ptr(sr), ptr(si))
+    return _cplx(sr, si, p.shape)
+
+
+def tv_weight(ctx: Context, x):
+    x = np.asarray(x)
+    shp = x.shape + (1,) * (3 - x.ndim)
+    xr, xi = _split(ctx, x.reshape(-1))
+    w = ctx.alloc((x.size,))
+    call("imp_tv_weight", ctx.h, C.c_int(shp[0]), C.c_int(shp[1]), C.c_int(shp[2]),
+         ptr(xr), ptr(xi), ptr(w))
+    return w.get().reshape(x.shape)
+
+
+class Toeplitz:
+    """Resident Toeplitz normal operator (imp_toep)."""
+
+    def __init__(self, ctx: Context, shape, q, tau, t0, fm=None, sens=None, lam=0.0):
+        shape = tuple(shape) + (1,) * (3 - len(shape))
+        self.ctx, self.shape = ctx, shape
+        self.nvox = int(np.prod(shape))
+        q = np.asarray(q)
+        nseg = q.shape[0]
+        self._qr, self._qi = _split(ctx, q.reshape(-1))
+        self._fm = _opt(ctx, None if fm is None else np.asarray(fm).reshape(-1))
+        ncoils = 1
+        self._sr = self._si = None
+        if sens is not None:
+            sens = np.asarray(sens).reshape(-1, self.nvox)
+            ncoils = sens.shape[0]
+            self._sr, self._si = _split(ctx, sens.reshape(-1))
+        h = C.c_void_p()
+        call("imp_toep_create", ctx.h, C.c_int(shape[0]), C.c_int(shape[1]), C.c_int(shape[2]),
+             C.c_int(ncoils), C.c_int(nseg - 1), C.c_float(tau), C.c_float(t0), ptr(self._qr),
+             ptr(self._qi), ptr(self._fm), ptr(self._sr), ptr(self._si), C.c_float(lam),
+             C.byref(h))
+        self.h = h
+        self._in = _pair(ctx, self.nvox)
+        self._out = _pair(ctx, self.nvox)
+
+    def close(self):
+        if getattr(self, "h", None):
+            call("imp_toep_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def apply(self, p):
+        p = np.asarray(p).reshape(-1)
+        self._in[0].upload(_f32(p.real))
+        self._in[1].upload(_f32(p.imag))
+        call("imp_toep_apply", self.h, ptr(self._in[0]), ptr(self._in[1]), ptr(self._out[0]),
+             ptr(self._out[1]))
+        return _cplx(*self._out)
+
+    def cg(self, b, max_iter, tol=0.0, flags=IMP_CG_BEST_ITERATE, x0=None):
+        b = np.asarray(b).reshape(-1)
+        self._in[0].upload(_f32(b.real))
+        self._in[1].upload(_f32(b.imag))
+        if x0 is not None:
+            x0 = np.asarray(x0).reshape(-1)
+            self._out[0].upload(_f32(x0.real))
+            self._out[1].upload(_f32(x0.imag))
+        st = CGStats()
+        call("imp_toep_cg", self.h, ptr(self._in[0]), ptr(self._in[1]), ptr(self._out[0]),
+             ptr(self._out[1]), C.c_int(max_iter), C.c_double(tol), C.c_int(flags),
+             C.byref(st))
+        return _cplx(*self._out), st.as_dict()
+
+
+def bruteforce_cg(ctx: Context, kx, ky, t, x, y, fm, d, sens, cg_num, lam, x0):
+    """bruteForceGpu's CG from x0 (cg_num - 1 iterations); d is [C, M]."""
+    d = np.atleast_2d(np.asarray(d))
+    ncoils, nk = d.shape
+    nx = len(x)
+    a = [ctx.to_device(_f32(v)) for v in (kx, ky, t, x, y)]
+    afm = _opt(ctx, fm)
+    dr, di = _split(ctx, d.reshape(-1))
+    sr = si = None
+    if sens is not None:
+        sr, si = _split(ctx, np.asarray(sens).reshape(-1))
+    xr, xi = _split(ctx, np.asarray(x0).reshape(-1))
+    call("imp_bruteforce_cg", ctx.h, C.c_int(nk), C.c_int(nx), C.c_int(ncoils), ptr(a[0]),
+         ptr(a[1]), ptr(a[2]), ptr(a[3]), ptr(a[4]), ptr(afm), ptr(dr), ptr(di), ptr(sr),
+         ptr(si), C.c_int(cg_num), C.c_float(lam), ptr(xr), ptr(xi))
+    return _cplx(xr, xi)

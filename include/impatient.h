@@ -1,0 +1,266 @@
+/*
+ * impatient.h -- C ABI of the MI355X (gfx950) IMPATIENT reconstruction engine.
+ *
+ * Every entry point returns an int status (IMP_OK == 0, negative on error);
+ * imp_last_error() returns the message of the calling thread's last failure.
+ * No entry point calls exit(): errors the reference handles with
+ * printf + exit(1) come back as status codes here.
+ *
+ * Memory: functions named imp_<op> take DEVICE pointers (HBM-resident
+ * operands, asynchronous on the context's HIP stream). The file-level
+ * wrappers (imp_toeplitz_computeQ_GPU, imp_toeplitz_computeFH_GPU,
+ * imp_toeplitz_recon, imp_bruteForceGpu) take HOST pointers exactly like the
+ * reference functions they replace and are synchronous.
+ *
+ * Complex vectors are passed as split real / imaginary float arrays, the
+ * layout of the reference's data files and host arrays (kdata_r/kdata_i,
+ * sensi_r/sensi_i, Fhd_R/Fhd_I, Qr[l]/Qi[l]).
+ *
+ * Phase convention (reference README "Data Format", kernels/ftGpu.cu:493):
+ *   forward  d_c(k) = sum_x S_c(x) rho(x) exp(-i(2*pi*k.x + w(x) t_k))
+ *   adjoint  rho(x) = sum_c conj(S_c(x)) sum_k d_c(k) exp(+i(2*pi*k.x + w(x) t_k))
+ * k in cycles/FOV, x in FOV units, w (field map) in rad/s, t in seconds.
+ */
+#ifndef IMPATIENT_H
+#define IMPATIENT_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define IMP_OK            0
+#define IMP_EINVAL       -1   /* bad argument / size                          */
+#define IMP_EHIP         -2   /* HIP runtime error                            */
+#define IMP_EIO          -3   /* file missing / short read / write failure    */
+#define IMP_ENOMEM       -4   /* allocation failed                            */
+#define IMP_EFORMAT      -5   /* malformed data file header                   */
+#define IMP_ENODEV       -6   /* no usable GPU                                */
+#define IMP_EBREAKDOWN   -7   /* CG breakdown: p^H A p <= 0 or not finite     */
+#define IMP_EUNSUPPORTED -8   /* size the kernels do not support (yet)        */
+
+const char *imp_version_string(void);
+const char *imp_status_string(int status);
+/* Message of the calling thread's last failed call ("" if none). */
+const char *imp_last_error(void);
+
+/* Process-wide reference-compatibility switches; 0 (default) = correct math.
+ * IMP_COMPAT_Q_CUMULATIVE: imp_toeplitz_computeQ_GPU returns the prefix sums
+ * over l that the reference direct path produces (its device accumulator is
+ * not cleared between segments, computeQ.cmem.cu:190-191, 951-963). */
+#define IMP_COMPAT_Q_CUMULATIVE 0x1
+int imp_set_compat(int flags);
+int imp_get_compat(void);
+
+/* --------------------------------------------------------------- context */
+typedef struct imp_ctx imp_ctx;
+
+/* One context per (host thread, GPU); owns one HIP stream and a workspace.
+ * Replaces the cudaSetDevice + implicit default stream of main_mri.cu:222-233. */
+int imp_ctx_create(int device, imp_ctx **out);
+int imp_ctx_destroy(imp_ctx *ctx);
+/* hipStream_t of the context (as void*). */
+int imp_ctx_stream(imp_ctx *ctx, void **hip_stream);
+int imp_ctx_synchronize(imp_ctx *ctx);
+
+/* Device-memory helpers for callers without an allocator of their own. */
+int imp_device_alloc(imp_ctx *ctx, size_t bytes, void **dptr);
+int imp_device_free(imp_ctx *ctx, void *dptr);
+int imp_copy_h2d(imp_ctx *ctx, void *dst, const void *src, size_t bytes);
+int imp_copy_d2h(imp_ctx *ctx, void *dst, const void *src, size_t bytes);
+int imp_memset_d(imp_ctx *ctx, void *dst, int value, size_t bytes);
+
+/* ------------------------------------------------------------- operators
+ * Direct, field-corrected NUDFT with SENSE (device pointers).
+ *
+ * imp_nudft_adjoint replaces ComputeFH_GPU_BF / computeFH_GPU_BF /
+ * parcomputeAH_GPU_BF (toeplitz/computeFH.cmem.cu:155-200, 805-869,
+ * 1146-1186) and iftGpu + parallelIftGpu (kernels/ftGpu.cu:653-1013,
+ * kernels/parImagingGpu.cu:119-152):
+ *   out(x) = sum_c conj(S_c(x)) sum_k conj(phi_c(k)) d_c(k) e^{+i(2pi k.x + fm(x) t_k)}
+ *   kx,ky,t [num_k]; kz [num_k] or NULL (2D); x,y [num_x]; z NULL (2D);
+ *   fm [num_x] or NULL (0); phi_r/phi_i [ncoils*num_k] or NULL (phi = 1);
+ *   d_r/d_i [ncoils*num_k] coil-major; s_r/s_i [ncoils*num_x] or NULL (S = 1);
+ *   out_r/out_i [num_x].
+ */
+int imp_nudft_adjoint(imp_ctx *ctx, int num_k, int num_x, int ncoils,
+                      const float *kx, const float *ky, const float *kz,
+                      const float *t,
+                      const float *x, const float *y, const float *z,
+                      const float *fm,
+                      const float *phi_r, const float *phi_i,
+                      const float *d_r, const float *d_i,
+                      const float *s_r, const float *s_i,
+                      float *out_r, float *out_i);
+
+/* Replaces ftGpu + parallelFtGpu (kernels/ftGpu.cu:198-604,
+ * kernels/parImagingGpu.cu:92-117):
+ *   d_c(k) = sum_x S_c(x) rho(x) e^{-i(2pi k.x + fm(x) t_k)}
+ *   d_r/d_i [ncoils*num_k] coil-major.  No power-of-two padding is needed. */
+int imp_nudft_forward(imp_ctx *ctx, int num_k, int num_x, int ncoils,
+                      const float *kx, const float *ky, const float *kz,
+                      const float *t,
+                      const float *x, const float *y, const float *z,
+                      const float *fm,
+                      const float *rho_r, const float *rho_i,
+                      const float *s_r, const float *s_i,
+                      float *d_r, float *d_i);
+
+/* Toeplitz kernels Q_l, l = 0..ntime_segs (ComputeQ_GPU / computeQ_GPU /
+ * toeplitz_computeQ_GPU, toeplitz/computeQ.cmem.cu:169-208, 756-822, 826-1028):
+ *   Q_l(r) = sum_k a_l(t_k) |phi(k)|^2 e^{+i 2pi k.r}
+ *   a_l(t) = 1/2 + 1/2 cos(pi (t - t0 - l tau)/tau) if |t - t0 - l tau| < tau, else 0
+ *   tau = (t[num_k-1] - t[0]) / ntime_segs (ntime_segs > 0), t0 = t[0].
+ * gx/gy/gz: coordinates of the num_q grid points (the ixQ/iyQ/izQ files of
+ * main_mri.cu:461-547), or all NULL for the standard grid of (Nx,Ny,Nz):
+ * index ((y+Ny)*2Nx + (x+Nx))*M3 + (z+Nz), r = (x/Nx, y/Ny, z/Nz), M3 = 2Nz
+ * (3D) or 1 (2D), num_q = 4NxNy*M3.  gz may be NULL in 2D.
+ * phi_r/phi_i [num_k] or NULL (phi = 1).  q_r/q_i [(ntime_segs+1)*num_q].
+ * flags: IMP_Q_CUMULATIVE reproduces the reference direct path, whose device
+ * accumulator is not cleared between segments (computeQ.cmem.cu:190-191,
+ * 951-963), i.e. returns prefix sums over l. */
+#define IMP_Q_CUMULATIVE 0x1
+int imp_toeplitz_q(imp_ctx *ctx, int num_k,
+                   const float *kx, const float *ky, const float *kz,
+                   const float *t, const float *phi_r, const float *phi_i,
+                   int Nx, int Ny, int Nz, int num_q,
+                   const float *gx, const float *gy, const float *gz,
+                   int ntime_segs, int flags,
+                   float *q_r, float *q_i);
+
+/* Periodic first-difference regularizer (DHWD2dGpu / DHWD3dGpu,
+ * toeplitz/DHWD2dGpu_toeplitz.cu:132-387, toeplitz/DHWD3dGpu_toeplitz.cu:170-580):
+ *   s = lambda * sum_d D_d^T W D_d p,  (D_d p)(i) = p(i) - p(i + e_d mod N_d)
+ * image [N1][N2][N3] row-major (N3 = 1 in 2D); w [N1*N2*N3] or NULL (W = I).
+ * Because the operator sums over all axes, the column-major brute-force
+ * variant (kernels/DHWD2dGpu.cu) gives identical results on square images. */
+int imp_fd_reg_apply(imp_ctx *ctx, int N1, int N2, int N3, float lambda,
+                     const float *p_r, const float *p_i, const float *w,
+                     float *s_r, float *s_i);
+
+/* TV weight w = 0.5 / sqrt(sum_d |D_d x|^2 + 1e-6)
+ * (multiply2dGpu / multiply3dGpu, toeplitz/multiplyGpu_toeplitz.cu:81-172). */
+int imp_tv_weight(imp_ctx *ctx, int N1, int N2, int N3,
+                  const float *x_r, const float *x_i, float *w);
+
+/* ------------------------------------------------ Toeplitz normal operator
+ * A p = sum_c conj(S_c) sum_l e^{+i w tau_l} T_l (e^{-i w tau_l} S_c p) + lambda R p
+ * with tau_l = t0 + l*tau and T_l the 2x-oversampled circulant embedding of
+ * Q_l (CudaFhF / CudaAhA / parCudaAhA / Computeq, toeplitz/recon.cu:1113-1697).
+ * Image layout [N1][N2][N3] = [Ny][Nx][Nz] row-major, N3 = 1 for 2D.
+ * q_r/q_i: DEVICE pointers to Q_l on the standard grid (imp_toeplitz_q with
+ * NULL coordinates, without IMP_Q_CUMULATIVE), (ntime_segs+1) planes.
+ * The plan keeps FFT(ifftshift(Q_l)) resident in HBM (the reference
+ * re-uploads it from the host for every coil and segment, recon.cu:1404-1411).
+ * fm [N], s_r/s_i [ncoils*N] (NULL: single coil with S = 1) are copied. */
+typedef struct imp_toep imp_toep;
+int imp_toep_create(imp_ctx *ctx, int N1, int N2, int N3, int ncoils,
+                    int ntime_segs, float tau, float t0,
+                    const float *q_r, const float *q_i,
+                    const float *fm, const float *s_r, const float *s_i,
+                    float lambda, imp_toep **out);
+int imp_toep_destroy(imp_toep *op);
+/* Replace the TV weight W of lambda R (device [N], NULL = identity). */
+int imp_toep_set_weight(imp_toep *op, const float *w);
+/* out = A p (device pointers, [N] each). */
+int imp_toep_apply(imp_toep *op, const float *p_r, const float *p_i,
+                   float *out_r, float *out_i);
+
+typedef struct {
+    int    iterations;        /* CG iterations performed                     */
+    int    converged;         /* recursive residual reached tol              */
+    int    best_iteration;    /* iteration of the returned best iterate      */
+    double rel_residual;      /* sqrt(beta2/|b|^2) at exit (recursive)       */
+    double best_rel_residual; /* same, for the returned iterate              */
+    double true_rel_residual; /* ||A x - b|| / ||b|| recomputed (flag)       */
+    double seconds;           /* wall time of the solve                      */
+} imp_cg_stats;
+
+#define IMP_CG_BEST_ITERATE   0x1  /* return argmin-residual iterate (recon.cu:2107-2110) */
+#define IMP_CG_USE_X0         0x2  /* start from x instead of 0                 */
+#define IMP_CG_TRUE_RESIDUAL  0x4  /* recompute ||Ax-b||/||b|| after the solve  */
+
+/* CG on A x = b (toeplitz_recon's loop, recon.cu:2011-2133).  All scalars
+ * (dot products, alpha, beta) are FP64 and stay on the device; tol is on
+ * the relative recursive residual sqrt(|r|^2/|b|^2). Device pointers. */
+int imp_toep_cg(imp_toep *op, const float *b_r, const float *b_i,
+                float *x_r, float *x_i, int max_iter, double tol, int flags,
+                imp_cg_stats *stats);
+
+/* ---------------------------------------------- brute-force CG (a11)
+ * bruteForceGpu semantics (bruteForce/bruteForceGpu.cu:142-813): Fessler CG
+ * on (F^H F + lambda R) x = F^H y starting from x, cg_num - 1 iterations,
+ * alpha = Re(d^H g) / (|F d|^2 + Re((lambda R d)^H d)). Device pointers;
+ * x_r/x_i [num_x] in/out. lambda <= 0 disables the regularizer.
+ * Image is Nimg x Nimg (square, as the reference assumes). */
+int imp_bruteforce_cg(imp_ctx *ctx, int num_k, int num_x, int ncoils,
+                      const float *kx, const float *ky, const float *t,
+                      const float *x, const float *y, const float *fm,
+                      const float *d_r, const float *d_i,
+                      const float *s_r, const float *s_i,
+                      int cg_num, float lambda,
+                      float *x_r, float *x_i);
+
+/* ------------------------------------------------------------ data files
+ * Reference file formats (kernels/structures.cpp:718-1110,
+ * toeplitz/utils.cpp:520-905): v0.2 = float version, int per_coil, int
+ * ncoils, int nslices, int total, float[total]; v1.0 = text "key = value"
+ * header, "//" comments, per-coil "Binary_Size = n" + "Binary:" sections of
+ * raw float32, then "EOF".  Host-only; no GPU needed. */
+typedef struct {
+    float version;      /* 0.2 or 1.0                                  */
+    int   per_coil;     /* elements per coil                           */
+    int   ncoils;       /* coil_number (1 if absent)                   */
+    int   nslices;
+    int   total;        /* file_size / total                           */
+    int   nx, ny, nz;   /* v1.0 xDimension/yDimension/zDimension (-1)  */
+} imp_file_info;
+
+/* Read header only (data == NULL) or header + data into caller buffer
+ * of capacity `cap` floats. */
+int imp_read_data_file(const char *path, imp_file_info *info,
+                       float *data, int cap);
+int imp_write_data_file_v02(const char *path, int per_coil, int ncoils,
+                            int nslices, const float *data, int total);
+/* ncoils = -1 writes no coil_number key, as writeDataFile_JGAI_10 does for
+ * coil-independent files; otherwise one Binary section per coil. */
+int imp_write_data_file_v10(const char *path, int nx, int ny, int nz,
+                            int ncoils, int nslices, const float *data,
+                            int total);
+
+/* ------------------------------------ reference-named file-level wrappers
+ * Same arguments and meaning as the reference functions; host pointers;
+ * the process-wide context on device `imp_set_default_device()` (0).
+ * They return IMP_* codes instead of exiting. */
+int imp_set_default_device(int device);
+
+/* toeplitz_computeQ_GPU (toeplitz/computeQ.cmem.cuh:55-61). Reads
+ * kx,ky,kz,t,ixQ,iyQ,izQ,phiR,phiI from dir; Qr[l], Qi[l] caller-allocated
+ * (l = 0..ntime_segs), each of the Q-grid size.  Direct evaluation only
+ * (enable_gridding is IMP_EUNSUPPORTED).  Writes Q_file when writeQ. */
+int imp_toeplitz_computeQ_GPU(const char *dir, float ntime_segs, int Nx,
+                              int Ny, int Nz, const char *Q_file,
+                              float **Qr, float **Qi, int enable_direct,
+                              int enable_gridding, float gridOS,
+                              int enable_writeQ);
+
+/* toeplitz_computeFH_GPU (toeplitz/computeFH.cmem.cuh:55-60). */
+int imp_toeplitz_computeFH_GPU(const char *dir, float ntime_segs, int Nx,
+                               int Ny, int Nz, const char *FhD_file,
+                               float *outR, float *outI, int enable_direct,
+                               int enable_gridding, float gridOS);
+
+/* toeplitz_recon (toeplitz/recon.cuh:55): argv[1..17] = N1 N2 N3
+ * numRestarts numIterMax symTraj lambda2 D Dp R W Qfile FhDfile dir tv_num
+ * ntime_segs outfile (main_mri.cu:578-598).  reg (sparse-matrix
+ * regularizer) is IMP_EUNSUPPORTED; coo must be NULL. */
+int imp_toeplitz_recon(int argc, char **argv, int enable_tv, int enable_reg,
+                       int enable_fd, int enable_reuseQ, const void *coo,
+                       float *Fhd_R, float *Fhd_I, float **Q_R, float **Q_I);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IMPATIENT_H */

@@ -1,0 +1,276 @@
+"""FP64 CPU oracle of the IMPATIENT hot path -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may use
+this module, and only as the checker.  The product (libimpatient.so) never
+calls it.
+
+Each function restates the reference algorithm it cites (paths are relative
+to the reference repository) in float64 numpy.  How each is pinned (closed
+forms, identities, independent restatements) is listed in
+tests/test_oracle.py; every function below has at least one such check.
+
+Conventions (README "Data Format"; kernels/ftGpu.cu:493-498):
+  forward  d_c(k) = sum_x S_c(x) rho(x) exp(-i(2 pi k.x + w(x) t_k))
+  adjoint  rho(x) = sum_c conj(S_c(x)) sum_k d_c(k) exp(+i(2 pi k.x + w(x) t_k))
+"""
+from __future__ import annotations
+
+import numpy as np
+
+_BLOCK = 1 << 22   # complex elements per phase block (memory bound of the oracle)
+
+
+def _phase(kx, ky, kz, t, x, y, z, fm, k_sl):
+    """2 pi k.x + w t for samples k_sl (rows) and all voxels (cols), FP64."""
+    ph = 2.0 * np.pi * (np.outer(kx[k_sl], x) + np.outer(ky[k_sl], y))
+    if kz is not None and z is not None:
+        ph += 2.0 * np.pi * np.outer(kz[k_sl], z)
+    if fm is not None:
+        ph += np.outer(t[k_sl], fm)
+    return ph
+
+
+def _as64(*arrs):
+    return [None if a is None else np.asarray(a, dtype=np.float64) for a in arrs]
+
+
+def nudft_forward(kx, ky, kz, t, x, y, z, fm, rho, sens=None):
+    """ftCpu (kernels/ftCpu.cpp:110-189) with the SENSE wrap of parallelFtCpu
+    (kernels/parImagingCpu.cpp:85-112): d_c(k) = sum_x S_c(x) rho(x) e^{-i phase}.
+    Returns [C, M] complex128."""
+    kx, ky, kz, t, x, y, z, fm = _as64(kx, ky, kz, t, x, y, z, fm)
+    rho = np.asarray(rho, dtype=np.complex128).reshape(-1)
+    S = np.ones((1, len(x))) if sens is None else np.asarray(sens, np.complex128).reshape(-1, len(x))
+    src = S * rho[None, :]                                   # [C, N]
+    nk = len(kx)
+    out = np.zeros((S.shape[0], nk), dtype=np.complex128)
+    step = max(1, _BLOCK // max(1, len(x)))
+    for k0 in range(0, nk, step):
+        sl = slice(k0, min(nk, k0 + step))
+        E = np.exp(-1j * _phase(kx, ky, kz, t, x, y, z, fm, sl))   # [m, N]
+        out[:, sl] = src @ E.T
+    return out
+
+
+def nudft_adjoint(kx, ky, kz, t, x, y, z, fm, d, sens=None, phi=None):
+    """iftCpu (kernels/ftCpu.cpp:199-277) + parallelIftCpu
+    (kernels/parImagingCpu.cpp:114-154) and the direct F^H d of
+    ComputeFH_GPU_BF / parcomputeAH_GPU_BF (toeplitz/computeFH.cmem.cu:155-200,
+    1146-1186), with the conj(phi) d weighting of ComputeRhoPhiGPU (:112-127).
+    d is [C, M]; returns [N] complex128."""
+    kx, ky, kz, t, x, y, z, fm = _as64(kx, ky, kz, t, x, y, z, fm)
+    d = np.atleast_2d(np.asarray(d, dtype=np.complex128))
+    if phi is not None:
+        d = np.conj(np.atleast_2d(np.asarray(phi, np.complex128)).reshape(d.shape)) * d
+    C = d.shape[0]
+    S = np.ones((C, len(x))) if sens is None else np.asarray(sens, np.complex128).reshape(C, len(x))
+    nk = len(kx)
+    acc = np.zeros((C, len(x)), dtype=np.complex128)
+    step = max(1, _BLOCK // max(1, len(x)))
+    for k0 in range(0, nk, step):
+        sl = slice(k0, min(nk, k0 + step))
+        E = np.exp(1j * _phase(kx, ky, kz, t, x, y, z, fm, sl))    # [m, N]
+        acc += d[:, sl] @ E
+    return np.sum(np.conj(S) * acc, axis=0)
+
+
+def hanning(tm, tau, l, t0):
+    """hanning_d (toeplitz/computeQ.cmem.cu:125-138):
+    1/2 + 1/2 cos(pi (t - tau l - t0)/tau) inside |t - tau l - t0| < tau, else 0."""
+    a = np.asarray(tm, np.float64) - tau * l - t0
+    return np.where(np.abs(a) < tau, 0.5 + 0.5 * np.cos(np.pi * a / tau), 0.0)
+
+
+def segment_timing(t, ntime_segs):
+    """tau, t0 as recon.cu:1785-1787 computes them (float32 arithmetic)."""
+    t = np.asarray(t, np.float32)
+    span = np.float32(t[-1] - t[0])
+    tau = np.float32(span / np.float32(ntime_segs)) if ntime_segs > 0 else span
+    return float(tau), float(t[0])
+
+
+def q_grid(Nx, Ny, Nz):
+    """Q-grid coordinates written by main_mri.cu:461-500: y outer, x, z
+    inner; r = (x/Nx, y/Ny, z/Nz) with x in [-Nx, Nx) etc. (z absent in 2D)."""
+    ys = np.arange(-Ny, Ny) / Ny
+    xs = np.arange(-Nx, Nx) / Nx
+    if Nz > 1:
+        zs = np.arange(-Nz, Nz) / Nz
+        Y, X, Z = np.meshgrid(ys, xs, zs, indexing="ij")
+        return X.reshape(-1), Y.reshape(-1), Z.reshape(-1)
+    Y, X = np.meshgrid(ys, xs, indexing="ij")
+    return X.reshape(-1), Y.reshape(-1), np.zeros(X.size)
+
+
+def toeplitz_q(kx, ky, kz, t, Nx, Ny, Nz, ntime_segs, phi=None, cumulative=False, grid=None):
+    """Q_l(r) = sum_k a_l(t_k) |phi_k|^2 e^{+i 2 pi k.r}, l = 0..L
+    (ComputeQ_GPU / toeplitz_computeQ_GPU, toeplitz/computeQ.cmem.cu:169-208,
+    826-1028).  cumulative=True restates the reference direct path, whose
+    accumulator is not cleared between segments (:190-191, 951-963).
+    Returns [(L+1), G] complex128."""
+    kx, ky, kz, t = _as64(kx, ky, kz, t)
+    gx, gy, gz = q_grid(Nx, Ny, Nz) if grid is None else _as64(*grid)
+    tau, t0 = segment_timing(t, ntime_segs)
+    mag = np.ones(len(kx)) if phi is None else np.abs(np.asarray(phi, np.complex128)) ** 2
+    W = np.stack([hanning(t, tau, l, t0) * mag for l in range(ntime_segs + 1)])  # [L+1, M]
+    use_z = kz is not None and Nz > 1
+    out = np.zeros((ntime_segs + 1, len(gx)), dtype=np.complex128)
+    step = max(1, _BLOCK // max(1, len(gx)))
+    for k0 in range(0, len(kx), step):
+        sl = slice(k0, min(len(kx), k0 + step))
+        ph = 2.0 * np.pi * (np.outer(kx[sl], gx) + np.outer(ky[sl], gy))
+        if use_z:
+            ph += 2.0 * np.pi * np.outer(kz[sl], gz)
+        out += W[:, sl] @ np.exp(1j * ph)
+    if cumulative:
+        out = np.cumsum(out, axis=0)
+    return out
+
+
+def _shift(a, inverse):
+    """cuda_fftshift / cuda_fft3shift on even grids (recon.cu:235-303,
+    fftshift.cu:5-104): a swap of half-spaces along every axis."""
+    return np.fft.ifftshift(a) if inverse else np.fft.fftshift(a)
+
+
+def toeplitz_apply(p, q, tau, t0, fm=None, sens=None, lam=0.0, w=None):
+    """q = sum_c conj(S_c) sum_l e^{+i w tau_l} T_l(e^{-i w tau_l} S_c p) + lam D^H W D p
+    with T_l v = crop(IFFT(FFT(ifftshift(pad_center(v))) Qf_l)) / G and
+    Qf_l = FFT(ifftshift(Q_l)): Computeq / parCudaAhA / CudaAhA / CudaFhF
+    (toeplitz/recon.cu:1113-1202, 1370-1465, 1467-1697, Qf at :1939-1964),
+    with the reference's centre padding (cuda_oversample :319-367) and
+    centre crop (cuda_undersample_scale :381-421).
+    p: image [N1, N2] or [N1, N2, N3]; q: [(L+1), G] on the standard grid."""
+    p = np.asarray(p, np.complex128)
+    shp = p.shape
+    d3 = p.ndim == 3 and shp[2] > 1
+    N = shp if d3 else shp[:2]
+    M = tuple(2 * n for n in N)
+    G = int(np.prod(M))
+    q = np.asarray(q, np.complex128).reshape(q.shape[0], *M)
+    Qf = [np.fft.fftn(_shift(q[l], True)) for l in range(q.shape[0])]
+    f = np.zeros(N) if fm is None else np.asarray(fm, np.float64).reshape(N)
+    S = [np.ones(N)] if sens is None else list(np.asarray(sens, np.complex128).reshape(-1, *N))
+    mid = tuple(n - n // 2 for n in N)
+    sl = tuple(slice(m, m + n) for m, n in zip(mid, N))
+    pv = p.reshape(N)
+    out = np.zeros(N, dtype=np.complex128)
+    for Sc in S:
+        acc = np.zeros(N, dtype=np.complex128)
+        for l in range(len(Qf)):
+            tl = tau * l + t0
+            v = np.exp(-1j * f * tl) * (Sc * pv)
+            big = np.zeros(M, dtype=np.complex128)
+            big[sl] = v
+            # unnormalised inverse FFT scaled by 1/G (cuda_undersample_scale)
+            r = np.fft.ifftn(np.fft.fftn(_shift(big, True)) * Qf[l])
+            r = _shift(r, False)[sl]
+            acc += np.exp(1j * f * tl) * r
+        out += np.conj(Sc) * acc
+    if lam:
+        out += fd_reg(pv, lam, w)
+    return out.reshape(shp)
+
+
+def fd_reg(p, lam, w=None):
+    """s = lam * sum_d D_d^T W D_d p, periodic: (D_d p)(i) = p(i) - p(i+e_d),
+    (D_d^T t)(i) = t(i) - t(i-e_d).  DHWD2dGpu / DHWD3dGpu
+    (toeplitz/DHWD2dGpu_toeplitz.cu:132-387, toeplitz/DHWD3dGpu_toeplitz.cu:170-580);
+    the same periodic D as matlab_utils/create_sparse_matrix/createDWithPeriodicBoundary.m."""
+    p = np.asarray(p, np.complex128)
+    W = np.ones(p.shape) if w is None else np.asarray(w, np.float64).reshape(p.shape)
+    s = np.zeros_like(p)
+    for ax in range(p.ndim):
+        if p.shape[ax] == 1:
+            continue
+        t1 = W * (p - np.roll(p, -1, axis=ax))
+        s += t1 - np.roll(t1, 1, axis=ax)
+    return lam * s
+
+
+def tv_weight(x):
+    """w = 0.5 / sqrt(sum_d |D_d x|^2 + 1e-6): multiply2dGpu / multiply3dGpu
+    (toeplitz/multiplyGpu_toeplitz.cu:134-172) on Dhori / Dverti / Dzaxis."""
+    x = np.asarray(x, np.complex128)
+    s = np.zeros(x.shape)
+    for ax in range(x.ndim):
+        if x.shape[ax] == 1:
+            continue
+        s += np.abs(x - np.roll(x, -1, axis=ax)) ** 2
+    return 0.5 / np.sqrt(s + 1e-6)
+
+
+def _vec_ops(vec_dtype):
+    """Storage rounding of CG vectors: complex128 (exact oracle) or complex64,
+    the FP32 vectors of the reference (FLOAT_T = float, kernels/tools.h:118-120;
+    cufftComplex matrices in toeplitz/recon.cu) with FP64 dot products."""
+    vd = np.dtype(vec_dtype)
+    sd = np.float32 if vd == np.complex64 else np.float64
+    st = lambda v: np.asarray(v).astype(vd)
+    dot = lambda a, b: np.vdot(np.asarray(a, np.complex128), np.asarray(b, np.complex128)).real
+    return st, dot, sd
+
+
+def toeplitz_cg(apply, b, max_iter, tol=1e-16, best_iterate=True, vec_dtype=np.complex128):
+    """The CG loop of toeplitz_recon (toeplitz/recon.cu:2011-2133):
+    x0 = 0, r = b, p = r; alpha = beta/Re(p^H q); x += alpha p;
+    r -= alpha q; beta2 = |r|^2; p = r + (beta2/beta) p; keep the iterate
+    with the smallest |r| (reconBest); stop when sqrt(beta2/|b|^2) < tol.
+    vec_dtype=complex64 stores the vectors in FP32 (scalars stay FP64).
+    Returns (x, iterations, residual history)."""
+    st, dot, sd = _vec_ops(vec_dtype)
+    b = st(b)
+    x = st(np.zeros_like(b))
+    r = b.copy()
+    p = r.copy()
+    beta2 = dot(r, r)
+    normb = beta2
+    best, xbest = beta2, x.copy()
+    hist = []
+    it = 0
+    while it < max_iter:
+        q = st(apply(p))
+        beta = beta2
+        alpha = sd(beta / dot(p, q))
+        x = st(x + alpha * p)
+        r = st(r - alpha * q)
+        beta2 = dot(r, r)
+        p = st(r + sd(beta2 / beta) * p)
+        hist.append(np.sqrt(beta2 / normb))
+        it += 1
+        if beta2 < best:
+            best, xbest = beta2, x.copy()
+        if np.sqrt(beta2 / normb) < tol:
+            break
+    return (xbest if best_iterate else x), it, hist
+
+
+def bruteforce_cg(F, FH, R, y, x0, cg_num, vec_dtype=np.complex128):
+    """bruteForceGpu's CG (bruteForce/bruteForceGpu.cu:470-654):
+    cg_num - 1 iterations from x0 with grad = F^H(y - F x) - R x,
+    d = grad + gamma d, alpha = Re(d^H grad) / (|F d|^2 + Re((R d)^H d)),
+    Af += alpha F d, x += alpha d.  F, FH, R are callables (R may be None).
+    vec_dtype=complex64 stores the vectors in FP32 (scalars stay FP64)."""
+    st, dot, sd = _vec_ops(vec_dtype)
+    y = st(y)
+    x = st(x0)
+    Af = st(F(x))
+    d = None
+    old = 0.0
+    for cg_i in range(1, cg_num):
+        g = st(FH(st(y - Af)))
+        if R is not None:
+            g = st(g - st(R(x)))
+        new = dot(g, g)
+        if cg_i == 1:
+            d = g.copy()
+        else:
+            gamma = 0.0 if old == 0.0 else new / old
+            d = st(g + sd(gamma) * d)
+        old = new
+        q = st(F(d))
+        a_ctcd = dot(st(R(d)), d) if R is not None else 0.0
+        alpha = sd(dot(d, g) / (dot(q, q) + a_ctcd))
+        Af = st(Af + alpha * q)
+        x = st(x + alpha * d)
+    return x

@@ -1,0 +1,146 @@
+/*
+ * C restatement of the reference CPU path -- TEST / BASELINE INFRASTRUCTURE
+ * ONLY.  Used by tests/ as an independent FP32 implementation and by
+ * bench.py's cpu_baseline leg as the timed reference CPU path.  The product
+ * library never links or calls it.
+ *
+ * Follows kernels/ftCpu.cpp:110-277 (ftCpu / iftCpu): the same loop order
+ * (samples outer for ft, voxels outer for ift), FP32 arithmetic, FP32
+ * sinf/cosf, and OpenMP on the inner reduction (ftCpu.cpp:162-167,
+ * 248-253).  The SENSE wrap follows kernels/parImagingCpu.cpp:85-154, the
+ * direct F^H d of toeplitz/computeFH.cmem.cu:155-200 (3D phase, conj(phi) d)
+ * and the direct Q of toeplitz/computeQ.cmem.cu:169-208 are restated in the
+ * same FP32 style (the reference has no CPU version of those two).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define TPI 6.28318530717958647692f
+
+/* kdata = F idata (one coil): ftCpu, kernels/ftCpu.cpp:110-189 */
+void ft_cpu(float *kdata_r, float *kdata_i, const float *idata_r, const float *idata_i,
+            const float *kx, const float *ky, const float *ix, const float *iy,
+            const float *fm, const float *t, int num_k, int num_i) {
+    for (int i = 0; i < num_k; i++) {
+        float sumr = 0.0f, sumi = 0.0f;
+        const float kxtpi = kx[i] * TPI, kytpi = ky[i] * TPI, ti = t[i];
+#pragma omp parallel for reduction(+ : sumr, sumi)
+        for (int j = 0; j < num_i; j++) {
+            const float expr = (kxtpi * ix[j] + kytpi * iy[j]) + (fm[j] * ti);
+            const float c = cosf(expr), s = sinf(expr);
+            sumr += (c * idata_r[j]) + (s * idata_i[j]);
+            sumi += (-s * idata_r[j]) + (c * idata_i[j]);
+        }
+        kdata_r[i] = sumr;
+        kdata_i[i] = sumi;
+    }
+}
+
+/* idata = F^H kdata (one coil): iftCpu, kernels/ftCpu.cpp:199-277 */
+void ift_cpu(float *idata_r, float *idata_i, const float *kdata_r, const float *kdata_i,
+             const float *kx, const float *ky, const float *ix, const float *iy,
+             const float *fm, const float *t, int num_k, int num_i) {
+    for (int j = 0; j < num_i; j++) {
+        float sumr = 0.0f, sumi = 0.0f;
+        const float xt = ix[j] * TPI, yt = iy[j] * TPI, fj = fm[j];
+#pragma omp parallel for reduction(+ : sumr, sumi)
+        for (int i = 0; i < num_k; i++) {
+            const float expr = (kx[i] * xt + ky[i] * yt) + (fj * t[i]);
+            const float c = cosf(expr), s = sinf(expr);
+            sumr += (c * kdata_r[i]) - (s * kdata_i[i]);
+            sumi += (s * kdata_r[i]) + (c * kdata_i[i]);
+        }
+        idata_r[j] = sumr;
+        idata_i[j] = sumi;
+    }
+}
+
+/* parallelIftCpu (kernels/parImagingCpu.cpp:114-154): sum_c conj(S_c) F^H d_c */
+void parallel_ift_cpu(float *out_r, float *out_i, const float *kdata_r, const float *kdata_i,
+                      const float *kx, const float *ky, const float *ix, const float *iy,
+                      const float *fm, const float *t, int num_k, int num_i,
+                      const float *s_r, const float *s_i, int ncoils) {
+    float *tr = (float *)malloc(sizeof(float) * num_i);
+    float *ti = (float *)malloc(sizeof(float) * num_i);
+    memset(out_r, 0, sizeof(float) * num_i);
+    memset(out_i, 0, sizeof(float) * num_i);
+    for (int c = 0; c < ncoils; c++) {
+        ift_cpu(tr, ti, kdata_r + (size_t)c * num_k, kdata_i + (size_t)c * num_k, kx, ky, ix, iy,
+                fm, t, num_k, num_i);
+        for (int j = 0; j < num_i; j++) {
+            const float sr = s_r ? s_r[(size_t)c * num_i + j] : 1.0f;
+            const float si = s_i ? s_i[(size_t)c * num_i + j] : 0.0f;
+            out_r[j] += sr * tr[j] + si * ti[j];
+            out_i[j] += sr * ti[j] - si * tr[j];
+        }
+    }
+    free(tr);
+    free(ti);
+}
+
+/* parallelFtCpu (kernels/parImagingCpu.cpp:85-112): d_c = F (S_c x) */
+void parallel_ft_cpu(float *kdata_r, float *kdata_i, const float *x_r, const float *x_i,
+                     const float *kx, const float *ky, const float *ix, const float *iy,
+                     const float *fm, const float *t, int num_k, int num_i,
+                     const float *s_r, const float *s_i, int ncoils) {
+    float *vr = (float *)malloc(sizeof(float) * num_i);
+    float *vi = (float *)malloc(sizeof(float) * num_i);
+    for (int c = 0; c < ncoils; c++) {
+        for (int j = 0; j < num_i; j++) {
+            const float sr = s_r ? s_r[(size_t)c * num_i + j] : 1.0f;
+            const float si = s_i ? s_i[(size_t)c * num_i + j] : 0.0f;
+            vr[j] = sr * x_r[j] - si * x_i[j];
+            vi[j] = sr * x_i[j] + si * x_r[j];
+        }
+        ft_cpu(kdata_r + (size_t)c * num_k, kdata_i + (size_t)c * num_k, vr, vi, kx, ky, ix, iy,
+               fm, t, num_k, num_i);
+    }
+    free(vr);
+    free(vi);
+}
+
+/* Direct F^H d with 3D phase and conj(phi) d weighting, one coil, FP32:
+ * the arithmetic of ComputeFH_GPU_BF (toeplitz/computeFH.cmem.cu:155-200)
+ * in the loop order of iftCpu. */
+void fhd_cpu(float *out_r, float *out_i, const float *rhophi_r, const float *rhophi_i,
+             const float *kx, const float *ky, const float *kz, const float *t, const float *ix,
+             const float *iy, const float *iz, const float *fm, int num_k, int num_i) {
+    for (int j = 0; j < num_i; j++) {
+        float sr = 0.0f, si = 0.0f;
+        const float xj = ix[j], yj = iy[j], zj = iz ? iz[j] : 0.0f, fj = fm[j];
+#pragma omp parallel for reduction(+ : sr, si)
+        for (int k = 0; k < num_k; k++) {
+            const float arg = TPI * kx[k] * xj + TPI * ky[k] * yj +
+                              (kz ? TPI * kz[k] * zj : 0.0f) + t[k] * fj;
+            const float c = cosf(arg), s = sinf(arg);
+            sr += rhophi_r[k] * c - rhophi_i[k] * s;
+            si += rhophi_i[k] * c + rhophi_r[k] * s;
+        }
+        out_r[j] = sr;
+        out_i[j] = si;
+    }
+}
+
+/* Direct Q_l for one l, FP32: ComputeQ_GPU (toeplitz/computeQ.cmem.cu:169-208)
+ * with hanning_d (:125-138). */
+void q_cpu(float *q_r, float *q_i, const float *kx, const float *ky, const float *kz,
+           const float *t, const float *phimag, const float *gx, const float *gy,
+           const float *gz, int num_k, int num_q, float l, float tau, float t0) {
+    for (int j = 0; j < num_q; j++) {
+        float sr = 0.0f, si = 0.0f;
+        const float xj = gx[j], yj = gy[j], zj = gz ? gz[j] : 0.0f;
+#pragma omp parallel for reduction(+ : sr, si)
+        for (int k = 0; k < num_k; k++) {
+            const float a = t[k] - tau * l - t0;
+            float atm = 0.0f;
+            if (fabsf(a) < tau) atm = 0.5f + 0.5f * cosf(3.14159265358979f * a / tau);
+            const float arg = TPI * kx[k] * xj + TPI * ky[k] * yj + (kz ? TPI * kz[k] * zj : 0.0f);
+            const float w = atm * (phimag ? phimag[k] : 1.0f);
+            sr += w * cosf(arg);
+            si += w * sinf(arg);
+        }
+        q_r[j] = sr;
+        q_i[j] = si;
+    }
+}

@@ -1,0 +1,111 @@
+"""GPU parity of the direct NUDFT operators against the FP64 oracle.
+
+Tolerances (relative L2 against FP64): the north-star bound for the
+reconstructed image is 1e-4; the operators themselves are held to 2e-6
+(split-precision phase, compensated sample sums).
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+OP_TOL = 2e-6
+
+
+def _prob(**kw):
+    from impatient import synth
+    return synth.tiny(**kw)
+
+
+@pytest.mark.parametrize("N,Nz,ncoils", [(16, 1, 1), (16, 1, 3), (12, 1, 5), (8, 4, 2)])
+def test_adjoint_matches_oracle(ctx, N, Nz, ncoils):
+    from impatient import ops
+    from oracle import oracle as orc
+    p = _prob(N=N, Nz=Nz, ncoils=ncoils)
+    rng = np.random.default_rng(1)
+    d = (rng.standard_normal((ncoils, p.nk)) + 1j * rng.standard_normal((ncoils, p.nk))).astype(np.complex64)
+    phi = (1 + 0.3 * rng.standard_normal((ncoils, p.nk)) + 0.2j * rng.standard_normal((ncoils, p.nk)))
+    got = ops.nudft_adjoint(ctx, p.kx, p.ky, p.kz, p.t, p.x, p.y, p.z, p.fm, d, p.sens, phi)
+    ref = orc.nudft_adjoint(p.kx, p.ky, p.kz, p.t, p.x, p.y, p.z, p.fm, d, p.sens, phi)
+    err = rel_l2(got, ref)
+    print(f"adjoint N={N} Nz={Nz} C={ncoils}: rel_l2={err:.2e}")
+    assert err < OP_TOL
+
+
+@pytest.mark.parametrize("N,Nz,ncoils", [(16, 1, 1), (16, 1, 4), (8, 4, 3)])
+def test_forward_matches_oracle(ctx, N, Nz, ncoils):
+    from impatient import ops
+    from oracle import oracle as orc
+    p = _prob(N=N, Nz=Nz, ncoils=ncoils)
+    rng = np.random.default_rng(2)
+    rho = (rng.standard_normal(p.nvox) + 1j * rng.standard_normal(p.nvox)).astype(np.complex64)
+    got = ops.nudft_forward(ctx, p.kx, p.ky, p.kz, p.t, p.x, p.y, p.z, p.fm, rho, p.sens, ncoils)
+    ref = orc.nudft_forward(p.kx, p.ky, p.kz, p.t, p.x, p.y, p.z, p.fm, rho, p.sens)
+    err = rel_l2(got, ref)
+    print(f"forward N={N} Nz={Nz} C={ncoils}: rel_l2={err:.2e}")
+    assert got.shape == (ncoils, p.nk)
+    assert err < OP_TOL
+
+
+def test_adjoint_identity_large(ctx):
+    """Re<F a, b> == Re<a, F^H b> at a size the oracle does not run
+    (64x64 image, 8 coils, 16384 samples)."""
+    from impatient import ops, synth
+    p = synth.make_problem("adj64", 64, 1, 8, 16, 1024, 8e-3, 150.0, 3, 4, 99)
+    rng = np.random.default_rng(3)
+    a = (rng.standard_normal(p.nvox) + 1j * rng.standard_normal(p.nvox)).astype(np.complex64)
+    b = (rng.standard_normal((8, p.nk)) + 1j * rng.standard_normal((8, p.nk))).astype(np.complex64)
+    Fa = ops.nudft_forward(ctx, p.kx, p.ky, None, p.t, p.x, p.y, None, p.fm, a, p.sens, 8)
+    FHb = ops.nudft_adjoint(ctx, p.kx, p.ky, None, p.t, p.x, p.y, None, p.fm, b, p.sens)
+    lhs = np.vdot(b.astype(np.complex128), Fa.astype(np.complex128))
+    rhs = np.vdot(FHb.astype(np.complex128), a.astype(np.complex128))
+    print(f"<b,Fa>={lhs:.6e} <F^H b,a>={rhs:.6e}")
+    assert abs(lhs - rhs) / abs(lhs) < 1e-5
+
+
+def test_adjoint_linearity_and_coil_sum(ctx):
+    """F^H(2 d1 + d2) = 2 F^H d1 + F^H d2; with S = 1 the C-coil adjoint of
+    identical coils is C times the single-coil adjoint."""
+    from impatient import ops
+    p = _prob(N=16, ncoils=1)
+    rng = np.random.default_rng(4)
+    d1 = (rng.standard_normal(p.nk) + 1j * rng.standard_normal(p.nk)).astype(np.complex64)
+    d2 = (rng.standard_normal(p.nk) + 1j * rng.standard_normal(p.nk)).astype(np.complex64)
+    f = lambda d: ops.nudft_adjoint(ctx, p.kx, p.ky, None, p.t, p.x, p.y, None, p.fm, d)
+    assert rel_l2(f((2 * d1 + d2).astype(np.complex64)), 2 * f(d1) + f(d2)) < 1e-5
+    d3 = np.stack([d1, d1, d1])
+    assert rel_l2(f(d3), 3 * f(d1)) < 1e-6
+
+
+@pytest.mark.parametrize("N,Nz,L", [(8, 1, 0), (8, 1, 3), (8, 1, 12), (4, 4, 2)])
+def test_q_matches_oracle(ctx, N, Nz, L):
+    from impatient import ops
+    from oracle import oracle as orc
+    p = _prob(N=N, Nz=Nz, ncoils=1, ntime_segs=L)
+    got = ops.toeplitz_q(ctx, p.kx, p.ky, p.kz, p.t, p.Nx, p.Ny, p.Nz, L)
+    ref = orc.toeplitz_q(p.kx, p.ky, p.kz, p.t, p.Nx, p.Ny, p.Nz, L)
+    err = rel_l2(got, ref)
+    print(f"Q N={N} Nz={Nz} L={L}: rel_l2={err:.2e}")
+    assert got.shape == ref.shape
+    assert err < OP_TOL
+
+
+def test_q_cumulative_compat(ctx):
+    """IMP_Q_CUMULATIVE returns the prefix sums over l (reference direct path)."""
+    from impatient import ops, IMP_Q_CUMULATIVE
+    p = _prob(N=8, ncoils=1, ntime_segs=4)
+    q = ops.toeplitz_q(ctx, p.kx, p.ky, None, p.t, 8, 8, 1, 4)
+    qc = ops.toeplitz_q(ctx, p.kx, p.ky, None, p.t, 8, 8, 1, 4, flags=IMP_Q_CUMULATIVE)
+    assert rel_l2(qc, np.cumsum(q.astype(np.complex128), axis=0)) < 1e-6
+
+
+def test_q_explicit_grid_matches_standard(ctx):
+    from impatient import ops
+    from oracle import oracle as orc
+    p = _prob(N=8, ncoils=1, ntime_segs=2)
+    g = orc.q_grid(8, 8, 1)
+    a = ops.toeplitz_q(ctx, p.kx, p.ky, None, p.t, 8, 8, 1, 2)
+    b = ops.toeplitz_q(ctx, p.kx, p.ky, None, p.t, 8, 8, 1, 2, grid=(g[0], g[1], None))
+    assert rel_l2(a, b) < 1e-6
