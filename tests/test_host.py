@@ -1,0 +1,181 @@
+"""Host-side checks (CPU only): the C-ABI library loads and exports every
+declared symbol, the reference data formats round-trip, malformed inputs
+return error codes, and the synthetic workloads have the stated layout."""
+import ctypes as C
+import re
+import struct
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, REPO
+
+
+def test_library_exports_every_declared_symbol():
+    from impatient import _lib
+    lib = _lib.load()
+    header = (REPO / "include" / "impatient.h").read_text()
+    names = sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(imp_\w+)\s*\(", header, re.M)))
+    assert len(names) >= 30
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_status_and_version_strings():
+    from impatient import _lib
+    lib = _lib.load()
+    assert b"gfx950" in lib.imp_version_string()
+    assert lib.imp_status_string(-5) == b"malformed data file"
+    assert lib.imp_status_string(0) == b"ok"
+
+
+def _parse_v02(path):
+    b = path.read_bytes()
+    v, = struct.unpack("<f", b[:4])
+    hdr = struct.unpack("<4i", b[4:20])
+    return v, hdr, np.frombuffer(b[20:], dtype="<f4")
+
+
+def test_read_reference_v02_files():
+    from impatient import files
+    d = GOLDEN / "ref_64x64x1"
+    for name in ["kx", "ky", "t", "ix", "iy", "fm", "kdata_r", "kdata_i", "ixQ", "sensi_r"]:
+        p = d / f"{name}.dat"
+        data, info = files.read(p, with_info=True)
+        v, hdr, raw = _parse_v02(p)
+        assert abs(info.version - 0.2) < 1e-6
+        assert (info.per_coil, info.ncoils, info.nslices, info.total) == hdr
+        assert np.array_equal(data, raw)
+    kx = files.read(d / "kx.dat")
+    assert kx.size == 3770 and abs(kx[0] - 31.906708) < 1e-5
+    ix = files.read(d / "ix.dat")
+    # the shipped 2D image coordinates vary along y first (x constant over 64 entries)
+    assert np.all(ix[:64] == ix[0]) and ix[64] != ix[0]
+
+
+def _parse_v10(path):
+    """Independent parser of the text-header format (structures.cpp:867-1110)."""
+    b = path.read_bytes()
+    pos, keys, data = 0, {}, []
+    while True:
+        end = b.index(b"\n", pos)
+        line = b[pos:end].decode().strip()
+        pos = end + 1
+        if not line or line.startswith("//"):
+            continue
+        if line.startswith("EOF"):
+            break
+        if line == "Binary:":
+            n = keys["Binary_Size"]
+            data.append(np.frombuffer(b[pos:pos + 4 * n], dtype="<f4"))
+            pos += 4 * n
+            continue
+        k, v = [s.strip() for s in line.split("=")]
+        keys[k] = float(v) if k == "version" else int(v)
+    return keys, np.concatenate(data)
+
+
+@pytest.mark.parametrize("name,ncoils", [("t.dat", 1), ("sensi_i.dat", 4)])
+def test_read_reference_v10_files(name, ncoils):
+    from impatient import files
+    p = GOLDEN / "ref_64x64x16_4coils" / name
+    data, info = files.read(p, with_info=True)
+    keys, raw = _parse_v10(p)
+    assert info.version == 1.0 and info.ncoils == ncoils
+    assert (info.nx, info.ny, info.nz) == (64, 64, 16)
+    assert info.total == keys["file_size"] and info.per_coil == keys["file_size"] // ncoils
+    assert np.array_equal(data, raw)
+
+
+@pytest.mark.parametrize("ncoils", [-1, 1, 3])
+def test_v10_round_trip(tmp_path, ncoils):
+    from impatient import files
+    rng = np.random.default_rng(0)
+    a = rng.standard_normal(3 * 40).astype(np.float32)
+    p = tmp_path / "a.dat"
+    files.write_v10(p, a, 4, 5, 2, ncoils)
+    b, info = files.read(p, with_info=True)
+    assert np.array_equal(a, b)
+    assert info.ncoils == (1 if ncoils == -1 else ncoils)
+    keys, raw = _parse_v10(p)
+    assert np.array_equal(raw, a)
+    assert ("coil_number" in keys) == (ncoils != -1)
+
+
+def test_v02_round_trip(tmp_path):
+    from impatient import files
+    a = np.arange(60, dtype=np.float32)
+    p = tmp_path / "b.dat"
+    files.write_v02(p, a, per_coil=20, ncoils=3)
+    b, info = files.read(p, with_info=True)
+    assert np.array_equal(a, b) and (info.per_coil, info.ncoils, info.total) == (20, 3, 60)
+    v, hdr, raw = _parse_v02(p)
+    assert abs(v - 0.2) < 1e-7 and hdr == (20, 3, 1, 60)
+
+
+def _status(path):
+    from impatient import _lib
+    fi = _lib.FileInfo()
+    buf = np.zeros(1024, np.float32)
+    return _lib.load().imp_read_data_file(str(path).encode(), C.byref(fi),
+                                          buf.ctypes.data_as(C.POINTER(C.c_float)), 1024)
+
+
+def test_malformed_files_return_error_codes(tmp_path):
+    assert _status(tmp_path / "missing.dat") == -3                 # IMP_EIO
+    p = tmp_path / "short.dat"
+    p.write_bytes(struct.pack("<f", 0.2) + struct.pack("<2i", 5, 1))
+    assert _status(p) == -5                                          # truncated header
+    p.write_bytes(struct.pack("<f", 0.2) + struct.pack("<4i", 5, 1, 1, 5) + b"\0" * 8)
+    assert _status(p) == -3                                          # short data
+    q = tmp_path / "v10.dat"
+    q.write_bytes(b"version = 1.0\nfile_size = 2\nBinary_Size = 2\nBinary:\n" + b"\0" * 8)
+    assert _status(q) == -5                                          # no EOF marker
+    q.write_bytes(b"version = 1.0\nfile_size = 3\nBinary_Size = 2\nBinary:\n" + b"\0" * 8 + b"EOF\n")
+    assert _status(q) == -5                                          # sections != file_size
+    q.write_bytes(b"version = 1.0\nbogus = 2\nEOF\n")
+    assert _status(q) == -5                                          # unknown key
+    q.write_bytes(b"version = 1.0\nfile_size = 2\nBinary_Size = 2\nBinary:\n" + b"\0" * 8 + b"EOF\n")
+    assert _status(q) == 0
+
+
+def test_file_level_wrappers_reject_bad_arguments():
+    """Argument errors are reported before any device work (no GPU needed)."""
+    from impatient import _lib
+    lib = _lib.load()
+    argv = (C.c_char_p * 3)(b"toeplitz_recon", b"1", b"1")
+    assert lib.imp_toeplitz_recon(3, argv, 0, 0, 1, 0, None, None, None, None, None) == -1
+    argv18 = (C.c_char_p * 18)(*([b"1"] * 18))
+    x = np.zeros(1, np.float32)
+    P = x.ctypes.data_as(C.c_void_p)
+    assert lib.imp_toeplitz_recon(18, argv18, 0, 1, 0, 0, None, P, P, None, None) == -8
+    assert lib.imp_toeplitz_computeQ_GPU(b"/nonexistent", C.c_float(2), 8, 8, 1, None,
+                                         None, None, 1, 0, C.c_float(1.0), 0) == -1
+    assert lib.imp_toeplitz_computeFH_GPU(b"/nonexistent", C.c_float(2), 8, 8, 1, None,
+                                          P, P, 0, 1, C.c_float(1.0)) == -8
+
+
+def test_context_without_gpu_fails_cleanly():
+    import os
+    from impatient import _lib
+    if os.path.exists("/dev/kfd") and os.environ.get("IMP_ASSUME_NO_GPU") is None:
+        pytest.skip("a GPU may be present")
+    h = C.c_void_p()
+    st = _lib.load().imp_ctx_create(0, C.byref(h))
+    assert st == -6 and not h.value
+    assert "no HIP device" in _lib.last_error()
+
+
+def test_synthetic_workload_layout():
+    from impatient import synth
+    p = synth.config(0)
+    assert (p.Nx, p.Ny, p.Nz, p.ncoils, p.nk) == (64, 64, 1, 1, 8192)
+    assert p.kx.min() >= -32 and p.kx.max() < 32
+    # x fastest: consecutive voxels step in x
+    assert p.x[1] - p.x[0] == pytest.approx(1 / 64) and p.y[1] == p.y[0]
+    assert np.abs(p.fm).max() == pytest.approx(2 * np.pi * 100, rel=1e-5)
+    p1 = synth.make_problem("s", 16, 4, 4, 4, 64, 5e-3, 100.0, 3, 2, 1)
+    ss = np.sum(np.abs(p1.sens) ** 2, axis=0)
+    assert np.allclose(ss, 1.0, atol=1e-5)
+    assert p1.z[1] - p1.z[0] == pytest.approx(1 / 4)        # z fastest in 3D
+    assert sorted(set(p1.kz.tolist())) == [-2, -1, 0, 1]

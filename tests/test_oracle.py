@@ -1,0 +1,329 @@
+"""Pins of the FP64 oracle (CPU only).
+
+Every oracle function is checked against something that does not share its
+code: closed forms, textbook library routines (numpy.fft, numpy.linalg,
+an explicit sparse difference matrix), identities the reference itself tests
+(the structured-vector adjoint test of bruteForceGpu.cu:240-303), and the
+independent FP32 C restatement of the reference CPU loops (oracle/ref_cpu.c).
+A plausible slip (sign of the phase, a dropped field-map or z term, a
+transposed grid axis, a wrong Hanning centre, an off-by-one shift) fails at
+least one test below.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import REPO, rel_l2
+
+
+def _orc():
+    from oracle import oracle
+    return oracle
+
+
+def _tiny(**kw):
+    from impatient import synth
+    return synth.tiny(**kw)
+
+
+# ------------------------------------------------------------------ NUDFT
+def test_single_sample_closed_form():
+    orc = _orc()
+    rng = np.random.default_rng(0)
+    x, y, z, fm = (rng.uniform(-.5, .5, 37) for _ in range(4))
+    fm = fm * 900.0
+    kx, ky, kz, t = np.array([3.25]), np.array([-7.5]), np.array([2.0]), np.array([4e-3])
+    d = np.array([[0.3 - 1.2j]])
+    got = orc.nudft_adjoint(kx, ky, kz, t, x, y, z, fm, d)
+    ph = 2 * np.pi * (3.25 * x - 7.5 * y + 2.0 * z) + fm * 4e-3
+    assert np.allclose(got, d[0, 0] * np.exp(1j * ph), rtol=0, atol=1e-12)
+    rho = rng.standard_normal(37) + 1j * rng.standard_normal(37)
+    fwd = orc.nudft_forward(kx, ky, kz, t, x, y, z, fm, rho)
+    assert np.allclose(fwd[0, 0], np.sum(rho * np.exp(-1j * ph)), atol=1e-10)
+
+
+def test_cartesian_grid_is_the_dft():
+    """Full Cartesian k-grid, w = 0: the forward NUDFT is numpy's 2D DFT with
+    centred index conventions."""
+    orc = _orc()
+    N = 8
+    idx = np.arange(N) - N // 2
+    ky, kx = np.meshgrid(idx, idx, indexing="ij")
+    iy, ix = np.meshgrid(idx / N, idx / N, indexing="ij")
+    rng = np.random.default_rng(1)
+    img = rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N))
+    d = orc.nudft_forward(kx.ravel(), ky.ravel(), None, np.zeros(N * N), ix.ravel(), iy.ravel(),
+                          None, None, img.ravel())[0].reshape(N, N)
+    ref = np.fft.fftshift(np.fft.fft2(np.fft.ifftshift(img)))
+    assert rel_l2(d, ref) < 1e-12
+    back = orc.nudft_adjoint(kx.ravel(), ky.ravel(), None, np.zeros(N * N), ix.ravel(),
+                             iy.ravel(), None, None, d.ravel()[None, :]).reshape(N, N)
+    assert rel_l2(back, N * N * img) < 1e-12
+
+
+def test_reference_structured_adjoint_test():
+    """The adjoint test vectors of bruteForceGpu.cu:240-303 (a: (1, M) with
+    every 10th entry (N, 1); b: (N+1, 1) with every 10th (1, M+2))."""
+    orc = _orc()
+    p = _tiny(N=16, ncoils=1)
+    n, m = p.nvox, p.nk
+    i = np.arange(n)
+    a = np.where(i % 10 != 0, 1 + 1j * m, n + 1j)
+    k = np.arange(m)
+    b = np.where(k % 10 != 0, (n + 1) + 1j, 1 + 1j * (m + 2))
+    o1 = np.vdot(orc.nudft_forward(p.kx, p.ky, None, p.t, p.x, p.y, None, p.fm, a)[0], b).real
+    o2 = np.vdot(a, orc.nudft_adjoint(p.kx, p.ky, None, p.t, p.x, p.y, None, p.fm, b[None])).real
+    assert abs(o1 - o2) / abs(o1) < 1e-12
+
+
+def test_sense_adjoint_identity_3d():
+    orc = _orc()
+    p = _tiny(N=8, Nz=4, ncoils=3)
+    rng = np.random.default_rng(2)
+    a = rng.standard_normal(p.nvox) + 1j * rng.standard_normal(p.nvox)
+    b = rng.standard_normal((3, p.nk)) + 1j * rng.standard_normal((3, p.nk))
+    Fa = orc.nudft_forward(p.kx, p.ky, p.kz, p.t, p.x, p.y, p.z, p.fm, a, p.sens)
+    FHb = orc.nudft_adjoint(p.kx, p.ky, p.kz, p.t, p.x, p.y, p.z, p.fm, b, p.sens)
+    assert abs(np.vdot(b, Fa) - np.vdot(FHb, a)) / abs(np.vdot(b, Fa)) < 1e-12
+
+
+def _cpu_lib():
+    so = REPO / "oracle" / "liboracle_cpu.so"
+    if not so.exists():
+        pytest.skip("oracle/liboracle_cpu.so not built (make -C oracle)")
+    return C.CDLL(str(so))
+
+
+def _fp(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a, a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def test_c_restatement_agrees_with_fp64():
+    """ftCpu / iftCpu restated in FP32 C (the reference loop order and
+    sinf/cosf) agree with the FP64 oracle to FP32 accuracy."""
+    orc = _orc()
+    lib = _cpu_lib()
+    p = _tiny(N=16, ncoils=1)
+    rng = np.random.default_rng(3)
+    rho = (rng.standard_normal(p.nvox) + 1j * rng.standard_normal(p.nvox)).astype(np.complex64)
+    keep = [_fp(v) for v in (p.kx, p.ky, p.x, p.y, p.fm, p.t, rho.real, rho.imag)]
+    kr = np.zeros(p.nk, np.float32)
+    ki = np.zeros(p.nk, np.float32)
+    P = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    lib.ft_cpu(P(kr), P(ki), keep[6][1], keep[7][1], keep[0][1], keep[1][1], keep[2][1],
+               keep[3][1], keep[4][1], keep[5][1], C.c_int(p.nk), C.c_int(p.nvox))
+    ref = orc.nudft_forward(p.kx, p.ky, None, p.t, p.x, p.y, None, p.fm, rho)[0]
+    assert rel_l2(kr + 1j * kr * 0 + 1j * ki, ref) < 1e-4
+    ir = np.zeros(p.nvox, np.float32)
+    ii = np.zeros(p.nvox, np.float32)
+    d = ref.astype(np.complex64)
+    dd = [_fp(d.real), _fp(d.imag)]
+    lib.ift_cpu(P(ir), P(ii), dd[0][1], dd[1][1], keep[0][1], keep[1][1], keep[2][1],
+                keep[3][1], keep[4][1], keep[5][1], C.c_int(p.nk), C.c_int(p.nvox))
+    ref2 = orc.nudft_adjoint(p.kx, p.ky, None, p.t, p.x, p.y, None, p.fm, d[None])
+    assert rel_l2(ir + 1j * ii, ref2) < 1e-4
+
+
+# ---------------------------------------------------------------------- Q
+def test_hanning_partition_of_unity():
+    orc = _orc()
+    t = np.linspace(1e-3, 9e-3, 777)
+    for L in (1, 3, 8):
+        tau, t0 = (t[-1] - t[0]) / L, t[0]
+        s = sum(orc.hanning(t, tau, l, t0) for l in range(L + 1))
+        assert np.allclose(s, 1.0, atol=1e-12)
+        # centre of segment l sits at t0 + l tau
+        assert orc.hanning(t0 + 2 * tau if L >= 2 else t0, tau, 2 if L >= 2 else 0, t0) == 1.0
+
+
+def test_q_properties():
+    orc = _orc()
+    p = _tiny(N=8, ncoils=1, ntime_segs=3)
+    q = orc.toeplitz_q(p.kx, p.ky, None, p.t, 8, 8, 1, 3)
+    G = q.reshape(4, 16, 16)
+    # Hermitian: Q(-r) = conj(Q(r)) on the grid r in [-N, N) (index i <-> r = i - N)
+    inner = G[:, 1:, 1:]
+    flipped = G[:, 1:, 1:][:, ::-1, ::-1]
+    assert np.allclose(inner, np.conj(flipped), atol=1e-9)
+    # sum over segments = the unsegmented kernel sum_k e^{i 2pi k.r}
+    gx, gy, _ = orc.q_grid(8, 8, 1)
+    full = np.exp(1j * 2 * np.pi * (np.outer(p.kx.astype(float), gx) +
+                                   np.outer(p.ky.astype(float), gy))).sum(0)
+    tau, t0 = orc.segment_timing(p.t, 3)
+    inside = np.ones(p.nk, bool)
+    assert rel_l2(q.sum(0), full) < 1e-6
+    # r = 0 holds the total segment weight
+    w0 = [orc.hanning(p.t, tau, l, t0).sum() for l in range(4)]
+    assert np.allclose(G[:, 8, 8].real, w0) and np.allclose(G[:, 8, 8].imag, 0, atol=1e-9)
+    assert inside.all()
+    # cumulative (reference direct path) = prefix sums
+    qc = orc.toeplitz_q(p.kx, p.ky, None, p.t, 8, 8, 1, 3, cumulative=True)
+    assert np.allclose(qc, np.cumsum(q, axis=0))
+
+
+def test_q_grid_layout():
+    """main_mri.cu:461-500: y outer, x inner (x fastest), r = x / Nx."""
+    orc = _orc()
+    gx, gy, gz = orc.q_grid(4, 3, 1)
+    assert gx[:8].tolist() == [(-4 + i) / 4 for i in range(8)]
+    assert np.all(gy[:8] == -1.0) and gy[8] == -2 / 3
+    gx3, gy3, gz3 = orc.q_grid(2, 2, 2)
+    assert gz3[:4].tolist() == [-1.0, -0.5, 0.0, 0.5] and np.all(gx3[:4] == -1.0)
+
+
+def test_c_q_restatement_agrees_with_fp64():
+    orc = _orc()
+    lib = _cpu_lib()
+    p = _tiny(N=8, ncoils=1, ntime_segs=2)
+    gx, gy, _ = orc.q_grid(8, 8, 1)
+    tau, t0 = orc.segment_timing(p.t, 2)
+    ref = orc.toeplitz_q(p.kx, p.ky, None, p.t, 8, 8, 1, 2)
+    keep = [_fp(v) for v in (p.kx, p.ky, p.t, gx, gy)]
+    P = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    for l in range(3):
+        qr = np.zeros(len(gx), np.float32)
+        qi = np.zeros(len(gx), np.float32)
+        lib.q_cpu(P(qr), P(qi), keep[0][1], keep[1][1], None, keep[2][1], None, keep[3][1],
+                  keep[4][1], None, C.c_int(p.nk), C.c_int(len(gx)), C.c_float(l),
+                  C.c_float(tau), C.c_float(t0))
+        assert rel_l2(qr + 1j * qi, ref[l]) < 1e-4
+
+
+# --------------------------------------------------------------- Toeplitz
+@pytest.mark.parametrize("Nz", [1, 4])
+def test_toeplitz_without_fieldmap_is_normal_operator(Nz):
+    """w = 0: sum_l T_l = F^H F exactly (circulant embedding, partition of unity)."""
+    orc = _orc()
+    p = _tiny(N=8, Nz=Nz, ncoils=2, ntime_segs=2, max_hz=0.0)
+    fm0 = np.zeros(p.nvox)
+    q = orc.toeplitz_q(p.kx, p.ky, p.kz, p.t, 8, 8, Nz, 2)
+    tau, t0 = orc.segment_timing(p.t, 2)
+    rng = np.random.default_rng(4)
+    v = rng.standard_normal(p.nvox) + 1j * rng.standard_normal(p.nvox)
+    shape = (8, 8, Nz) if Nz > 1 else (8, 8)
+    T = orc.toeplitz_apply(v.reshape(shape), q, tau, t0, fm0, p.sens).reshape(-1)
+    F = orc.nudft_forward(p.kx, p.ky, p.kz, p.t, p.x, p.y, p.z, fm0, v, p.sens)
+    FHF = orc.nudft_adjoint(p.kx, p.ky, p.kz, p.t, p.x, p.y, p.z, fm0, F, p.sens)
+    assert rel_l2(T, FHF) < 1e-12
+
+
+def test_toeplitz_matches_segmented_direct_formula():
+    """w != 0: the apply equals sum_c S_c^H sum_l e^{+iw tau_l} F^H diag(a_l) F
+    e^{-iw tau_l} S_c, evaluated by direct sums."""
+    orc = _orc()
+    L = 3
+    p = _tiny(N=8, ncoils=2, ntime_segs=L, max_hz=80.0)
+    q = orc.toeplitz_q(p.kx, p.ky, None, p.t, 8, 8, 1, L)
+    tau, t0 = orc.segment_timing(p.t, L)
+    rng = np.random.default_rng(5)
+    v = rng.standard_normal(p.nvox) + 1j * rng.standard_normal(p.nvox)
+    T = orc.toeplitz_apply(v.reshape(8, 8), q, tau, t0, p.fm, p.sens).reshape(-1)
+    fm = p.fm.astype(np.float64)
+    acc = np.zeros(p.nvox, complex)
+    for c in range(2):
+        for l in range(L + 1):
+            a = orc.hanning(p.t, tau, l, t0)
+            ph = np.exp(-1j * fm * (tau * l + t0))
+            u = ph * p.sens[c] * v
+            Fu = orc.nudft_forward(p.kx, p.ky, None, p.t, p.x, p.y, None, None, u)
+            acc += np.conj(p.sens[c]) * np.conj(ph) * orc.nudft_adjoint(
+                p.kx, p.ky, None, p.t, p.x, p.y, None, None, a * Fu)
+    assert rel_l2(T, acc) < 1e-10
+
+
+# ----------------------------------------------------------- regularizer
+def _periodic_D(n):
+    """First difference with periodic boundary, (Dp)(i) = p(i) - p(i+1 mod n),
+    as createDWithPeriodicBoundary.m builds it."""
+    D = np.eye(n)
+    D[np.arange(n), (np.arange(n) + 1) % n] -= 1
+    return D
+
+
+@pytest.mark.parametrize("shape", [(6, 5), (4, 3, 5)])
+def test_fd_reg_equals_explicit_matrix(shape):
+    orc = _orc()
+    n = int(np.prod(shape))
+    rng = np.random.default_rng(6)
+    w = rng.uniform(0.2, 2.0, n)
+    blocks = []
+    for ax, na in enumerate(shape):
+        mats = [np.eye(s) for s in shape]
+        mats[ax] = _periodic_D(na)
+        K = mats[0]
+        for m_ in mats[1:]:
+            K = np.kron(K, m_)
+        blocks.append(K)
+    R = sum(B.T @ np.diag(w) @ B for B in blocks)
+    p = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    assert rel_l2(orc.fd_reg(p.reshape(shape), 0.7, w.reshape(shape)).reshape(-1),
+                  0.7 * R @ p) < 1e-13
+
+
+def test_fd_reg_plane_wave_eigenvalue():
+    orc = _orc()
+    N1, N2 = 8, 12
+    f1, f2 = 3, 5
+    y, x = np.meshgrid(np.arange(N1), np.arange(N2), indexing="ij")
+    v = np.exp(2j * np.pi * (f1 * y / N1 + f2 * x / N2))
+    lam = 0.3
+    eig = lam * ((2 - 2 * np.cos(2 * np.pi * f1 / N1)) + (2 - 2 * np.cos(2 * np.pi * f2 / N2)))
+    assert rel_l2(orc.fd_reg(v, lam), eig * v) < 1e-13
+
+
+def test_tv_weight_closed_form():
+    orc = _orc()
+    x = np.zeros((4, 4), complex)
+    x[1, 2] = 3.0
+    w = orc.tv_weight(x)
+    # at (1,2): |x - right|^2 + |x - below|^2 = 9 + 9
+    assert np.isclose(w[1, 2], 0.5 / np.sqrt(18 + 1e-6))
+    assert np.isclose(w[0, 0], 0.5 / np.sqrt(1e-6))
+
+
+# --------------------------------------------------------------------- CG
+def _spd(n, seed):
+    rng = np.random.default_rng(seed)
+    B = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    return B @ B.conj().T + n * np.eye(n)
+
+
+def test_toeplitz_cg_solves_spd_system():
+    orc = _orc()
+    A = _spd(20, 7)
+    b = np.arange(20) + 1j
+    x, it, hist = orc.toeplitz_cg(lambda v: A @ v, b, 200, tol=1e-13)
+    assert rel_l2(x, np.linalg.solve(A, b)) < 1e-10
+    assert it <= 40
+
+
+def test_toeplitz_cg_best_iterate():
+    """reconBest: the returned iterate has the smallest recursive residual."""
+    orc = _orc()
+    A = _spd(30, 8)
+    b = np.ones(30, complex)
+    xb, it, hist = orc.toeplitz_cg(lambda v: A @ v, b, 6, best_iterate=True)
+    xl, _, _ = orc.toeplitz_cg(lambda v: A @ v, b, 6, best_iterate=False)
+    k = int(np.argmin(hist)) + 1
+    xk, _, _ = orc.toeplitz_cg(lambda v: A @ v, b, k, best_iterate=False)
+    assert np.allclose(xb, xk)
+    if k == 6:
+        assert np.allclose(xb, xl)
+
+
+def test_bruteforce_cg_converges_to_normal_equations():
+    orc = _orc()
+    rng = np.random.default_rng(9)
+    Fm = rng.standard_normal((40, 12)) + 1j * rng.standard_normal((40, 12))
+    Rm = _spd(12, 10) * 0.01
+    y = rng.standard_normal(40) + 1j * rng.standard_normal(40)
+    x = orc.bruteforce_cg(lambda v: Fm @ v, lambda d: Fm.conj().T @ d, lambda v: Rm @ v, y,
+                          np.zeros(12, complex), 40)
+    xs = np.linalg.solve(Fm.conj().T @ Fm + Rm, Fm.conj().T @ y)
+    assert rel_l2(x, xs) < 1e-8
+    # FP32 vector storage stays close on this well-conditioned system
+    x32 = orc.bruteforce_cg(lambda v: Fm @ v, lambda d: Fm.conj().T @ d, lambda v: Rm @ v, y,
+                            np.zeros(12, complex), 40, vec_dtype=np.complex64)
+    assert rel_l2(x32, xs) < 1e-4
