@@ -100,7 +100,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-recon", action="store_true", help="skip Q / CG (profiling runs)")
     ap.add_argument("--cg-max-iter", type=int, default=400)
+    ap.add_argument("--pmc-mode", action="store_true",
+                    help="counter runs: one F^H d, one Toeplitz apply and one 256 MiB "
+                         "device copy (the calibration kernel), nothing else")
     args = ap.parse_args()
+    if args.pmc_mode:
+        args.steps, args.warmup, args.no_cpu_baseline = 1, 0, True
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
               "(the path shards over coils with one image all-reduce per CG iteration; "
@@ -221,6 +226,19 @@ def main():
         e1.sync()
         t_apply = e1.ms_since(e0) / 3
         log(f"Toeplitz plan {t_plan:.3f} s, apply {t_apply:.2f} ms")
+        if args.pmc_mode:
+            # calibration kernel for the byte counters: a 256 MiB device copy
+            hip = C.CDLL("libamdhip64.so")
+            nbytes = 256 << 20
+            src, dst = ctx.alloc((nbytes // 4,)), ctx.alloc((nbytes // 4,))
+            ctx.synchronize()
+            assert hip.hipMemcpyAsync(C.c_void_p(dst.ptr), C.c_void_p(src.ptr),
+                                      C.c_size_t(nbytes), C.c_int(3), C.c_void_p(stream)) == 0
+            ctx.synchronize()
+            call("imp_toep_destroy", h)
+            log("pmc mode: F^H d, Q, 4 applies, one 256 MiB copy done")
+            print(json.dumps(result), flush=True)
+            return
         st = _lib.CGStats()
         tc0 = time.perf_counter()
         call("imp_toep_cg", h, ptr(dbr), ptr(dbi), ptr(dxr), ptr(dxi), C.c_int(args.cg_max_iter),

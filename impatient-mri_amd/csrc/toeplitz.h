@@ -67,6 +67,11 @@ int toeplitz_q(imp_ctx *ctx, int num_k, const float *kx, const float *ky,
                const float *gx, const float *gy, const float *gz, int ntime_segs,
                int flags, const float *t_host_first_last, float *q_r, float *q_i);
 
+int bruteforce_cg(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *kx,
+                  const float *ky, const float *t, const float *x, const float *y,
+                  const float *fm, const float *d_r, const float *d_i, const float *s_r,
+                  const float *s_i, int cg_num, float lambda, float *x_r, float *x_i);
+
 // internal CG flag: track the best iterate in ToepPlan::vbest but leave the
 // last iterate in x
 constexpr int CG_KEEP_LAST = 0x100;

@@ -389,3 +389,37 @@ extern "C" int imp_toeplitz_recon(int argc, char **argv, int enable_tv, int enab
     // out.file: Re(reconBest) then Im(reconBest), float32 (recon.cu:2150-2165)
     return write_raw(outfile, {best.data(), best.data() + nvox}, nvox);
 }
+
+extern "C" int imp_bruteForceGpu(float *x_r, float *x_i, const float *y_r, const float *y_i,
+                                 const float *kx, const float *ky, const float *ix,
+                                 const float *iy, const float *fm, const float *t, int cg_num,
+                                 int num_k, int num_i, float fd_penalizer, const float *s_r,
+                                 const float *s_i, int ncoils) {
+    if (!x_r || !x_i || !y_r || !y_i || !kx || !ky || !ix || !iy || !t || num_k <= 0 ||
+        num_i <= 0 || ncoils <= 0 || (s_r == nullptr) != (s_i == nullptr))
+        return fail(IMP_EINVAL, "imp_bruteForceGpu: bad arguments");
+    imp_ctx *ctx = nullptr;
+    IMP_TRY(default_ctx(&ctx));
+    const size_t nk = (size_t)num_k * ncoils;
+    Dev a[12];
+    IMP_TRY(a[0].up(kx, num_k));
+    IMP_TRY(a[1].up(ky, num_k));
+    IMP_TRY(a[2].up(t, num_k));
+    IMP_TRY(a[3].up(ix, num_i));
+    IMP_TRY(a[4].up(iy, num_i));
+    if (fm) IMP_TRY(a[5].up(fm, num_i));
+    IMP_TRY(a[6].up(y_r, nk));
+    IMP_TRY(a[7].up(y_i, nk));
+    if (s_r) {
+        IMP_TRY(a[8].up(s_r, (size_t)num_i * ncoils));
+        IMP_TRY(a[9].up(s_i, (size_t)num_i * ncoils));
+    }
+    IMP_TRY(a[10].up(x_r, num_i));
+    IMP_TRY(a[11].up(x_i, num_i));
+    IMP_TRY(bruteforce_cg(ctx, num_k, num_i, ncoils, a[0].p, a[1].p, a[2].p, a[3].p, a[4].p,
+                          a[5].p, a[6].p, a[7].p, a[8].p, a[9].p, cg_num, fd_penalizer,
+                          a[10].p, a[11].p));
+    IMP_HIP(hipMemcpy(x_r, a[10].p, num_i * sizeof(float), hipMemcpyDeviceToHost));
+    IMP_HIP(hipMemcpy(x_i, a[11].p, num_i * sizeof(float), hipMemcpyDeviceToHost));
+    return IMP_OK;
+}

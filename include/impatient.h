@@ -260,6 +260,18 @@ int imp_toeplitz_recon(int argc, char **argv, int enable_tv, int enable_reg,
                        int enable_fd, int enable_reuseQ, const void *coo,
                        float *Fhd_R, float *Fhd_I, float **Q_R, float **Q_I);
 
+/* bruteForceGpu for one slice (bruteForce/gpuPrototypes.cuh:90-105, def
+ * bruteForceGpu.cu:143), host pointers: x_r/x_i [num_i] in/out (initial
+ * estimate in, image out); y_r/y_i [ncoils*num_k]; kx,ky,t [num_k];
+ * ix,iy,fm [num_i]; s_r/s_i [ncoils*num_i] or NULL (S = 1).  fd_penalizer
+ * > 0 enables the finite-difference term (the -fd/-fdp options); the
+ * sparse-matrix and TV variants are not offered.  No power-of-two padding. */
+int imp_bruteForceGpu(float *x_r, float *x_i, const float *y_r, const float *y_i,
+                      const float *kx, const float *ky, const float *ix,
+                      const float *iy, const float *fm, const float *t,
+                      int cg_num, int num_k, int num_i, float fd_penalizer,
+                      const float *s_r, const float *s_i, int ncoils);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,81 @@
+"""mriSolver CLI on the reference's 64x64x1 input files: the Toeplitz and
+brute-force drivers produce the reference's output files, with the values
+the C-ABI entry points give for the same inputs."""
+import ctypes as C
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, REPO, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+CLI = REPO / "impatient-mri_amd" / "bin" / "mriSolver"
+
+
+@pytest.fixture()
+def data_dir(tmp_path):
+    if not CLI.exists():
+        pytest.fail(f"{CLI} not built")
+    d = tmp_path / "64x64x1"
+    shutil.copytree(GOLDEN / "ref_64x64x1", d)
+    for f in d.iterdir():
+        f.chmod(0o644)
+    return d
+
+
+def test_cli_toeplitz_direct(data_dir):
+    from impatient import files
+    from oracle import oracle as orc
+    r = subprocess.run([str(CLI), "-idir", str(data_dir), "-toeplitzDirect", "-ntime_segs", "3",
+                        "-cg_num", "10", "-fd", "-fdp", "0.001", "-writeQ"],
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    n = 64 * 64
+    img = np.fromfile(data_dir / "out.file", dtype=np.float32)
+    assert img.size == 2 * n and np.isfinite(img).all()
+    # side files the reference driver writes into the input directory
+    for name in ["phiR.dat", "phiI.dat", "ixQ.dat", "iyQ.dat", "izQ.dat", "FhD.file",
+                 "Q_stone.file"]:
+        assert (data_dir / name).exists(), name
+    # the generated Q grid follows the image layout of the coordinate files
+    # (these 2D files vary fastest along y, so the grid does too)
+    gx, gy = files.read(data_dir / "ixQ.dat"), files.read(data_dir / "iyQ.dat")
+    ref_gx = files.read(GOLDEN / "ref_64x64x1" / "ixQ.dat")
+    ref_gy = files.read(GOLDEN / "ref_64x64x1" / "iyQ.dat")
+    assert np.array_equal(gx, ref_gx) and np.array_equal(gy, ref_gy)
+    # the image solves the normal equations as well as the FP32 oracle CG does
+    q = np.fromfile(data_dir / "Q_stone.file", dtype=np.float32).reshape(4, 2, -1)
+    q = q[:, 0] + 1j * q[:, 1]
+    fhd = np.fromfile(data_dir / "FhD.file", dtype=np.float32).reshape(2, n)
+    b = (fhd[0] + 1j * fhd[1]).astype(np.complex128)
+    tau, t0 = orc.segment_timing(files.read(data_dir / "t.dat"), 3)
+    fm = files.read(data_dir / "fm.dat")
+    A = lambda v: orc.toeplitz_apply(v.reshape(64, 64), q, tau, t0, fm, None, 1e-3).reshape(-1)
+    x32, _, _ = orc.toeplitz_cg(A, b, 10, vec_dtype=np.complex64)
+    x = img[:n] + 1j * img[n:]
+    err = rel_l2(x, x32)
+    print(f"CLI toeplitz vs FP32-storage oracle CG: {err:.2e}")
+    assert err < 5e-3
+
+
+def test_cli_bruteforce(data_dir, ctx):
+    from impatient import files, ops
+    out = data_dir / "out"
+    r = subprocess.run([str(CLI), "-idir", str(data_dir), "-odir", str(out), "-cg_num", "6",
+                        "-fd", "-fdp", "0.01"], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    xr = np.loadtxt(out / "output_gpu_r.dat", dtype=np.float64)
+    xi = np.loadtxt(out / "output_gpu_i.dat", dtype=np.float64)
+    rd = lambda n: files.read(data_dir / f"{n}.dat")
+    d = (rd("kdata_r") + 1j * rd("kdata_i"))[None, :]
+    x0 = rd("idata_r") + 1j * rd("idata_i")
+    ref = ops.bruteforce_cg(ctx, rd("kx"), rd("ky"), rd("t"), rd("ix"), rd("iy"), rd("fm"), d,
+                            None, 6, 0.01, x0)
+    # text output keeps 6 decimals ("%f", structures.cpp:1170-1173)
+    assert np.max(np.abs(xr - ref.real)) <= 5e-7 + 1e-6 * np.abs(ref.real).max()
+    assert np.max(np.abs(xi - ref.imag)) <= 5e-7 + 1e-6 * np.abs(ref.imag).max()

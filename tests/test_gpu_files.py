@@ -23,6 +23,8 @@ def _read(name, d=REF):
 def data_dir(tmp_path):
     d = tmp_path / "64x64x1"
     shutil.copytree(REF, d)
+    for f in d.iterdir():
+        f.chmod(0o644)
     return d
 
 
