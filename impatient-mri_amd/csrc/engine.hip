@@ -235,7 +235,7 @@ extern "C" int imp_fd_reg_apply(imp_ctx *ctx, int N1, int N2, int N3, float lamb
         return fail(IMP_EINVAL, "imp_fd_reg_apply: bad arguments");
     const long n = (long)N1 * N2 * N3;
     void *buf = nullptr;
-    IMP_TRY(workspace(ctx, 3, 2 * n * sizeof(float2), &buf));
+    IMP_TRY(workspace(ctx, 7, 2 * n * sizeof(float2), &buf));
     float2 *pc = (float2 *)buf, *sc = pc + n;
     IMP_TRY(split_to_c(ctx, p_r, p_i, pc, n));
     IMP_TRY(fd_reg(ctx, N1, N2, N3, lambda, pc, w, sc, false));
@@ -249,7 +249,7 @@ extern "C" int imp_tv_weight(imp_ctx *ctx, int N1, int N2, int N3, const float *
         return fail(IMP_EINVAL, "imp_tv_weight: bad arguments");
     const long n = (long)N1 * N2 * N3;
     void *buf = nullptr;
-    IMP_TRY(workspace(ctx, 3, n * sizeof(float2), &buf));
+    IMP_TRY(workspace(ctx, 7, n * sizeof(float2), &buf));
     IMP_TRY(split_to_c(ctx, x_r, x_i, (float2 *)buf, n));
     IMP_TRY(tv_weight(ctx, N1, N2, N3, (const float2 *)buf, w));
     return IMP_OK;

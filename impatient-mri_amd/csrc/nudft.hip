@@ -2,27 +2,34 @@
 // SENSE) and the Toeplitz kernels Q_l.
 //
 // All three are one computation,
-//     out[a][c] = sum_b w[b][c] * exp(i * sign * 2pi * (P_a . B_b)),
+//     out[a][c] = sum_b W[c][b] * exp(i * sign * 2pi * (P_a . B_b)),
 // with 4-vectors P_a (owned by a lane) and B_b (streamed), because the phase
 // 2pi(k.x) + w(x) t is the dot product of (x, y, z, w/2pi) with (kx, ky, kz, t):
-//   adjoint  a = voxel  (x,y,z,fm/2pi), b = sample (kx,ky,kz,t), w = conj(phi) d_c
-//   forward  a = sample (kx,ky,kz,t),   b = voxel  (x,y,z,fm/2pi), w = S_c rho
-//   Q_l      a = Q-grid point (r,0),    b = sample (kx,ky,kz,0),   w = a_l(t) |phi|^2
+//   adjoint  a = voxel  (x,y,z,fm/2pi), b = sample (kx,ky,kz,t), W = conj(phi) d_c
+//   forward  a = sample (kx,ky,kz,t),   b = voxel  (x,y,z,fm/2pi), W = S_c rho
+//   Q_l      a = Q-grid point (r,0),    b = sample (kx,ky,kz,0),   W = a_l(t) |phi|^2
 // The reference evaluates one coil (or one segment l) per pass and recomputes
 // the sincos every time (computeFH.cmem.cu:1166-1178, computeQ.cmem.cu:951-963).
-// Here one sincos feeds G right-hand sides held in registers, so for 32 coils
-// the complex multiply-accumulates dominate instead of the transcendentals.
+// Here one phase evaluation feeds every right-hand side (coil or segment).
 //
-// Streamed operands are wave-uniform, so they are read with scalar loads
-// through the constant cache (the role CUDA __constant__ tiles play in the
-// reference, ftGpu.cu:103, computeFH.cmem.cu:108-109) without tile relaunches.
+// Phase: P . B is formed in FP64 (on gfx950 a non-packed FP32 FMA and an
+// FP64 FMA issue at the same rate), reduced with fract to [0, 1) revolutions
+// and passed to the hardware sin/cos, which take revolutions (max abs error
+// 1.2e-7).  The FP64 phase is exact to ~1e-12 revolutions for |k.x| < 10^3.
 //
-// Precision: the phase is formed in revolutions with split-precision
-// products (two-product via FMA, exact fractional reduction of each product)
-// and reduced to [-1/2, 1/2] before the hardware sin/cos, which take
-// revolutions.  Sample sums accumulate in FP32 over blocks of BLK samples and
-// are folded into a compensated (two-sum) total; chunk partials are added in
-// FP64.
+// Two kernels:
+//   k_nudft_valu  few right-hand sides: per-lane complex MACs, streamed
+//                 operands read with scalar loads (wave-uniform).
+//   k_nudft_mfma  many right-hand sides: the MACs are a complex GEMM
+//                 E(points x samples) * W(samples x rhs).  E is generated per
+//                 lane directly in the MFMA A-operand layout of
+//                 v_mfma_f32_32x32x16_f16; W is pre-packed as B operands.
+//                 FP32 accuracy from fp16 MFMAs: E and W are split into
+//                 hi + lo fp16 parts (scaled by powers of two to stay in the
+//                 normal range) and E W = Eh Wh + Eh Wl + El Wh (the dropped
+//                 El Wl term is below 2^-22 relative).
+// Sample sums are blocked and folded into compensated (two-sum) totals;
+// chunk partials are reduced in FP64.
 #include "common.h"
 
 #include <cmath>
@@ -33,34 +40,45 @@ namespace imp {
 
 namespace {
 
-constexpr int NU_THREADS = 256;
-constexpr int NU_BLK = 32;  // samples per FP32 block before compensated fold
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
 
+constexpr int NU_THREADS = 256;
+constexpr int NU_BLK = 32;        // VALU: samples per FP32 block before the compensated fold
+constexpr int MF_K = 8;           // MFMA: samples per 32x32x16 step (8 Re + 8 Im rows of K)
+constexpr int MF_FOLD = 32;       // MFMA: steps per compensated fold (256 samples)
+constexpr int MF_COLS = 16;       // MFMA: right-hand sides per 32-column tile (Re + Im)
+constexpr float E_SCALE = 4096.0f;  // 2^12: keeps the lo parts of E in fp16 normal range
+constexpr int W_EXP = 12;           // W is scaled so max |W| < 2^12
+
+struct alignas(16) BPair {
+    half8 hi;
+    half8 lo;
+};
+
+__device__ __forceinline__ float rev_fract(double u) { return (float)__builtin_amdgcn_fract(u); }
+
+// ---------------------------------------------------------------- VALU kernel
+// Weights stream: [group][n_stream][2*G] floats (Re, Im per rhs); positions
+// double4 [n_stream].  EPI_COMBINE sums conj(S_c) * acc_c over the rhs.
 enum { EPI_COMBINE = 0, EPI_PERCOIL = 1 };
 
-// Stream stride in floats for G right-hand sides; multiple of 4 so every
-// element starts on a 16-byte boundary (one s_load_dwordx4 for B).
-__host__ __device__ constexpr int stream_stride(int G, bool realw) {
-    return realw ? 4 + ((G + 3) & ~3) : 4 + 2 * ((G + 1) & ~1);
-}
-
-template <int G, bool D3, bool REALW, int EPI>
-__global__ __launch_bounds__(NU_THREADS) void k_nudft(
-    const float4 *__restrict__ lanes, int n_lane,
-    const float *__restrict__ stream, int n_stream, int chunk_len,
-    float sign, int ncoils,
+template <int G, bool D3, int EPI>
+__global__ __launch_bounds__(NU_THREADS) void k_nudft_valu(
+    const float4 *__restrict__ lanes, int n_lane, const double4 *__restrict__ spos,
+    const float *__restrict__ wstream, int n_stream, int chunk_len, float sign, int nrhs,
     const float *__restrict__ s_r, const float *__restrict__ s_i,
     float2 *__restrict__ partial) {
-    constexpr int S = stream_stride(G, REALW);
     const int a = blockIdx.x * NU_THREADS + threadIdx.x;
     const int chunk = blockIdx.y;
     const int group = blockIdx.z;
     const int nchunks = gridDim.y;
     const int b0 = chunk * chunk_len;
     const int b1 = min(n_stream, b0 + chunk_len);
-    const float *__restrict__ gstream = stream + (size_t)group * n_stream * S;
+    const float *__restrict__ gw = wstream + (size_t)group * n_stream * (2 * G);
 
-    const float4 P = lanes[min(a, n_lane - 1)];
+    const float4 Pf = lanes[min(a, n_lane - 1)];
+    const double px = Pf.x, py = Pf.y, pz = Pf.z, pw = Pf.w;
 
     float hr[G], hi[G], lr[G], li[G];
 #pragma unroll
@@ -72,37 +90,20 @@ __global__ __launch_bounds__(NU_THREADS) void k_nudft(
 #pragma unroll
         for (int g = 0; g < G; ++g) { ar[g] = ai[g] = 0.f; }
         for (int j = b; j < be; ++j) {
-            const float *__restrict__ e = gstream + (size_t)j * S;
-            const float4 B = *reinterpret_cast<const float4 *>(e);
-            // phase in revolutions, split precision
-            const float p1 = P.x * B.x;
-            const float e1 = __builtin_fmaf(P.x, B.x, -p1);
-            const float p2 = P.y * B.y;
-            const float e2 = __builtin_fmaf(P.y, B.y, -p2);
-            float h = frac_near(p1) + frac_near(p2);
-            float lo = e1 + e2;
-            if constexpr (D3) {
-                const float p3 = P.z * B.z;
-                const float e3 = __builtin_fmaf(P.z, B.z, -p3);
-                h += frac_near(p3);
-                lo += e3;
-            }
-            const float u = frac_near(h + __builtin_fmaf(P.w, B.w, lo));
-            float sn, cs;
-            sincos_rev(u, &sn, &cs);
-            sn *= sign;
+            const double4 B = spos[j];
+            double u = B.w * pw;
+            if constexpr (D3) u = __builtin_fma(B.z, pz, u);
+            u = __builtin_fma(B.y, py, u);
+            u = __builtin_fma(B.x, px, u);
+            const float r = rev_fract(u);
+            const float cs = __builtin_amdgcn_cosf(r);
+            const float sn = sign * __builtin_amdgcn_sinf(r);
+            const float *__restrict__ e = gw + (size_t)j * (2 * G);
 #pragma unroll
             for (int g = 0; g < G; ++g) {
-                if constexpr (REALW) {
-                    const float w = e[4 + g];
-                    ar[g] = __builtin_fmaf(w, cs, ar[g]);
-                    ai[g] = __builtin_fmaf(w, sn, ai[g]);
-                } else {
-                    const float wr = e[4 + 2 * g];
-                    const float wi = e[5 + 2 * g];
-                    ar[g] = __builtin_fmaf(wr, cs, __builtin_fmaf(-wi, sn, ar[g]));
-                    ai[g] = __builtin_fmaf(wr, sn, __builtin_fmaf(wi, cs, ai[g]));
-                }
+                const float wr = e[2 * g], wi = e[2 * g + 1];
+                ar[g] = __builtin_fmaf(wr, cs, __builtin_fmaf(-wi, sn, ar[g]));
+                ai[g] = __builtin_fmaf(wr, sn, __builtin_fmaf(wi, cs, ai[g]));
             }
         }
 #pragma unroll
@@ -117,16 +118,13 @@ __global__ __launch_bounds__(NU_THREADS) void k_nudft(
         }
     }
     if (a >= n_lane) return;
-
     if constexpr (EPI == EPI_COMBINE) {
-        // sum_g conj(S_g(a)) * acc_g
         float orr = 0.f, oi = 0.f;
 #pragma unroll
         for (int g = 0; g < G; ++g) {
             const int c = group * G + g;
-            if (c < ncoils) {
-                const float vr = hr[g] + lr[g];
-                const float vi = hi[g] + li[g];
+            if (c < nrhs) {
+                const float vr = hr[g] + lr[g], vi = hi[g] + li[g];
                 float sr = 1.f, si = 0.f;
                 if (s_r) {
                     sr = s_r[(size_t)c * n_lane + a];
@@ -141,120 +139,120 @@ __global__ __launch_bounds__(NU_THREADS) void k_nudft(
 #pragma unroll
         for (int g = 0; g < G; ++g) {
             const int c = group * G + g;
-            if (c < ncoils)
-                partial[((size_t)chunk * ncoils + c) * n_lane + a] =
+            if (c < nrhs)
+                partial[((size_t)chunk * nrhs + c) * n_lane + a] =
                     make_float2(hr[g] + lr[g], hi[g] + li[g]);
         }
     }
 }
 
-// ---- stream / lane packing -------------------------------------------------
+// ---------------------------------------------------------------- MFMA kernel
+// One wave owns 32 lane points (one M tile) and TILES 32-column N tiles
+// (16 rhs x {Re, Im} each).  Per step of 8 samples each lane evaluates the
+// 8 phases of its point and keeps cos (lanes 0-31: the Re rows of K) or
+// sin (lanes 32-63: the Im rows, via cos(u - 1/4)); B rows hold the matching
+// W coefficients (packed by k_pack_mfma).  Output planes:
+// partial[chunk][TILES*32][n_lane] (plane = tile*32 + part*16 + rhs%16).
+template <int TILES, bool D3>
+__global__ __launch_bounds__(NU_THREADS) __attribute__((amdgpu_waves_per_eu(TILES == 1 ? 4 : 3)))
+void k_nudft_mfma(
+    const float4 *__restrict__ lanes, int n_lane, const double4 *__restrict__ spos,
+    const BPair *__restrict__ bpack, int n_steps, int steps_per_chunk,
+    float *__restrict__ partial) {
+    const int l = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int mt = blockIdx.x * (NU_THREADS / 64) + wave;
+    if (mt * 32 >= n_lane) return;
+    const int h = l >> 5;
+    const int a = mt * 32 + (l & 31);
+    const float4 Pf = lanes[min(a, n_lane - 1)];
+    const double px = Pf.x, py = Pf.y, pz = Pf.z, pw = Pf.w;
+    const double quarter = h ? -0.25 : 0.0;   // sin(2 pi u) = cos(2 pi (u - 1/4))
 
-// adjoint stream: {kx,ky,kz,t, conj(phi_c) d_c ...} per sample, per coil group
-__global__ void k_pack_adjoint(int num_k, int ncoils, int G, int S,
-                               const float *kx, const float *ky, const float *kz,
-                               const float *t, const float *phi_r,
-                               const float *phi_i, const float *d_r,
-                               const float *d_i, float *stream) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int group = blockIdx.y;
-    if (k >= num_k) return;
-    float *e = stream + ((size_t)group * num_k + k) * S;
-    e[0] = kx[k];
-    e[1] = ky[k];
-    e[2] = kz ? kz[k] : 0.f;
-    e[3] = t[k];
-    for (int j = 4 + 2 * G; j < S; ++j) e[j] = 0.f;
-    for (int g = 0; g < G; ++g) {
-        const int c = group * G + g;
-        float wr = 0.f, wi = 0.f;
-        if (c < ncoils) {
-            const size_t i = (size_t)c * num_k + k;
-            const float dr = d_r[i], di = d_i[i];
-            if (phi_r) {
-                // conj(phi) * d (ComputeRhoPhiGPU, computeFH.cmem.cu:112-127)
-                const float pr = phi_r[i], pi = phi_i[i];
-                wr = pr * dr + pi * di;
-                wi = pr * di - pi * dr;
-            } else {
-                wr = dr;
-                wi = di;
+    const int chunk = blockIdx.y;
+    const int s0 = chunk * steps_per_chunk;
+    const int s1 = min(n_steps, s0 + steps_per_chunk);
+
+    floatx16 acc[TILES];
+    float tot[TILES][16], cmp[TILES][16];
+#pragma unroll
+    for (int t = 0; t < TILES; ++t) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) { acc[t][i] = 0.f; tot[t][i] = 0.f; cmp[t][i] = 0.f; }
+    }
+
+    int since_fold = 0;
+    for (int st = s0; st < s1; ++st) {
+        half8 ahi, alo;
+#pragma unroll
+        for (int j = 0; j < MF_K; ++j) {
+            const double4 B = spos[st * MF_K + j];
+            double u = __builtin_fma(B.w, pw, quarter);
+            if constexpr (D3) u = __builtin_fma(B.z, pz, u);
+            u = __builtin_fma(B.y, py, u);
+            u = __builtin_fma(B.x, px, u);
+            const float v = __builtin_amdgcn_cosf(rev_fract(u)) * E_SCALE;
+            const _Float16 vh = (_Float16)v;
+            ahi[j] = vh;
+            alo[j] = (_Float16)(v - (float)vh);
+        }
+#pragma unroll
+        for (int t = 0; t < TILES; ++t) {
+            const BPair bp = bpack[((size_t)st * TILES + t) * 64 + l];
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, bp.hi, acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, bp.lo, acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(alo, bp.hi, acc[t], 0, 0, 0);
+        }
+        if (++since_fold == MF_FOLD || st + 1 == s1) {
+            since_fold = 0;
+#pragma unroll
+            for (int t = 0; t < TILES; ++t) {
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    float s, e;
+                    two_sum(tot[t][i], acc[t][i], s, e);
+                    tot[t][i] = s;
+                    cmp[t][i] += e;
+                    acc[t][i] = 0.f;
+                }
             }
         }
-        e[4 + 2 * g] = wr;
-        e[5 + 2 * g] = wi;
     }
-}
-
-// forward stream: {x,y,z,fm/2pi, S_c rho ...} per voxel, per coil group
-__global__ void k_pack_forward(int num_x, int ncoils, int G, int S,
-                               const float *x, const float *y, const float *z,
-                               const float *fm, const float *rho_r,
-                               const float *rho_i, const float *s_r,
-                               const float *s_i, float *stream) {
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    const int group = blockIdx.y;
-    if (v >= num_x) return;
-    float *e = stream + ((size_t)group * num_x + v) * S;
-    e[0] = x[v];
-    e[1] = y[v];
-    e[2] = z ? z[v] : 0.f;
-    e[3] = fm ? (float)((double)fm[v] * (1.0 / (2.0 * M_PI))) : 0.f;
-    for (int j = 4 + 2 * G; j < S; ++j) e[j] = 0.f;
-    const float rr = rho_r[v], ri = rho_i[v];
-    for (int g = 0; g < G; ++g) {
-        const int c = group * G + g;
-        float wr = 0.f, wi = 0.f;
-        if (c < ncoils) {
-            float sr = 1.f, si = 0.f;
-            if (s_r) {
-                sr = s_r[(size_t)c * num_x + v];
-                si = s_i[(size_t)c * num_x + v];
-            }
-            wr = sr * rr - si * ri;
-            wi = sr * ri + si * rr;
+    // D layout of 32x32 MFMA: lane l, register i -> column l%32,
+    // row 8*(i/4) + 4*(l/32) + i%4
+    const int n = l & 31;
+    const int nplanes = TILES * 32;
+#pragma unroll
+    for (int t = 0; t < TILES; ++t) {
+        float *__restrict__ plane = partial + ((size_t)chunk * nplanes + t * 32 + n) * n_lane;
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int row = 8 * (i >> 2) + 4 * h + (i & 3);
+            const int ap = mt * 32 + row;
+            if (ap < n_lane) plane[ap] = tot[t][i] + cmp[t][i];
         }
-        e[4 + 2 * g] = wr;
-        e[5 + 2 * g] = wi;
     }
 }
 
-// Q stream: {kx,ky,kz,0, a_l(t_k)|phi_k|^2 ...} per sample, per segment group.
-// a_l is the Hanning interpolator of hanning_d (computeQ.cmem.cu:125-138),
-// evaluated in FP64.
-__global__ void k_pack_q(int num_k, int nseg, int G, int S, const float *kx,
-                         const float *ky, const float *kz, const float *t,
-                         const float *phi_r, const float *phi_i, float tau,
-                         float t0, float *stream) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    const int group = blockIdx.y;
-    if (k >= num_k) return;
-    float *e = stream + ((size_t)group * num_k + k) * S;
-    e[0] = kx[k];
-    e[1] = ky[k];
-    e[2] = kz ? kz[k] : 0.f;
-    e[3] = 0.f;
-    for (int j = 4 + G; j < S; ++j) e[j] = 0.f;
-    double mag = 1.0;
-    if (phi_r) {
-        const double pr = phi_r[k], pi = phi_i[k];
-        mag = pr * pr + pi * pi;
+// ---- packing ---------------------------------------------------------------
+
+// stream positions: {x, y, z, w} as FP64, zero-padded to n_pad
+__global__ void k_spos(int n, int n_pad, const float *x, const float *y, const float *z,
+                       const float *w, double wscale, double4 *out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_pad) return;
+    double4 v = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (i < n) {
+        v.x = x[i];
+        v.y = y[i];
+        v.z = z ? (double)z[i] : 0.0;
+        v.w = w ? (double)w[i] * wscale : 0.0;
     }
-    const double tk = t[k];
-    for (int g = 0; g < G; ++g) {
-        const int l = group * G + g;
-        double w = 0.0;
-        if (l < nseg) {
-            const double arg = tk - (double)tau * l - (double)t0;
-            if (fabs(arg) < (double)tau) w = 0.5 + 0.5 * cos(M_PI * arg / (double)tau);
-        }
-        e[4 + g] = (float)(w * mag);
-    }
+    out[i] = v;
 }
 
-__global__ void k_lanes_from_arrays(int n, const float *x, const float *y,
-                                    const float *z, const float *w, double wscale,
-                                    float4 *lanes) {
+__global__ void k_lanes_from_arrays(int n, const float *x, const float *y, const float *z,
+                                    const float *w, double wscale, float4 *lanes) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     lanes[i] = make_float4(x[i], y[i], z ? z[i] : 0.f,
@@ -270,16 +268,131 @@ __global__ void k_lanes_qgrid(int Nx, int Ny, int Nz, float4 *lanes) {
     const int iz = i % M3;
     const int ix = (i / M3) % M2;
     const int iy = i / (M3 * M2);
-    const float gx = (float)(ix - Nx) / (float)Nx;
-    const float gy = (float)(iy - Ny) / (float)Ny;
-    const float gz = Nz > 1 ? (float)(iz - Nz) / (float)Nz : 0.f;
-    lanes[i] = make_float4(gx, gy, gz, 0.f);
+    lanes[i] = make_float4((float)(ix - Nx) / (float)Nx, (float)(iy - Ny) / (float)Ny,
+                           Nz > 1 ? (float)(iz - Nz) / (float)Nz : 0.f, 0.f);
+}
+
+// Weights W[c][b] (complex, rhs-major) for the three operators.
+// adjoint: conj(phi) d (ComputeRhoPhiGPU, computeFH.cmem.cu:112-127)
+__global__ void k_w_adjoint(int num_k, int ncoils, const float *phi_r, const float *phi_i,
+                            const float *d_r, const float *d_i, float2 *W) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)num_k * ncoils) return;
+    const float dr = d_r[i], di = d_i[i];
+    if (phi_r) {
+        const float pr = phi_r[i], pi = phi_i[i];
+        W[i] = make_float2(pr * dr + pi * di, pr * di - pi * dr);
+    } else {
+        W[i] = make_float2(dr, di);
+    }
+}
+
+// forward: S_c rho
+__global__ void k_w_forward(int num_x, int ncoils, const float *rho_r, const float *rho_i,
+                            const float *s_r, const float *s_i, float2 *W) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)num_x * ncoils) return;
+    const int v = (int)(i % num_x);
+    const float rr = rho_r[v], ri = rho_i[v];
+    float sr = 1.f, si = 0.f;
+    if (s_r) {
+        sr = s_r[i];
+        si = s_i[i];
+    }
+    W[i] = make_float2(sr * rr - si * ri, sr * ri + si * rr);
+}
+
+// Q: a_l(t_k) |phi_k|^2 with the Hanning interpolator of hanning_d
+// (computeQ.cmem.cu:125-138), evaluated in FP64.
+__global__ void k_w_q(int num_k, int nseg, const float *t, const float *phi_r,
+                      const float *phi_i, float tau, float t0, float2 *W) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)num_k * nseg) return;
+    const int l = (int)(i / num_k), k = (int)(i % num_k);
+    double mag = 1.0;
+    if (phi_r) {
+        const double pr = phi_r[k], pi = phi_i[k];
+        mag = pr * pr + pi * pi;
+    }
+    const double arg = (double)t[k] - (double)tau * l - (double)t0;
+    double w = 0.0;
+    if (fabs(arg) < (double)tau) w = 0.5 + 0.5 * cos(M_PI * arg / (double)tau);
+    W[i] = make_float2((float)(w * mag), 0.f);
+}
+
+// VALU stream: [group][n][2G] from W[c][n]
+__global__ void k_pack_valu(int n, int nrhs, int G, const float2 *W, float *out) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int group = blockIdx.y;
+    if (b >= n) return;
+    float *e = out + ((size_t)group * n + b) * (2 * G);
+    for (int g = 0; g < G; ++g) {
+        const int c = group * G + g;
+        const float2 w = c < nrhs ? W[(size_t)c * n + b] : make_float2(0.f, 0.f);
+        e[2 * g] = w.x;
+        e[2 * g + 1] = w.y;
+    }
+}
+
+// max |Re|, |Im| over W (non-negative floats compare as unsigned ints)
+__global__ void k_absmax(const float2 *W, size_t n, unsigned int *out) {
+    float m = 0.f;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const float2 w = W[i];
+        m = fmaxf(m, fmaxf(fabsf(w.x), fabsf(w.y)));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
+}
+
+__device__ __forceinline__ int w_shift(unsigned int maxbits) {
+    // exponent s with max|W| * 2^s < 2^W_EXP
+    const float m = __uint_as_float(maxbits);
+    if (!(m > 0.f)) return 0;
+    int e;
+    frexpf(m, &e);  // m = f * 2^e, f in [0.5, 1)
+    return W_EXP - e;
+}
+
+// B operands for k_nudft_mfma: step st (8 samples), tile t, lane l:
+// K row (h, j) = (Re|Im of E, sample 8*st + j); column n = l%32 -> part n/16,
+// rhs t*16 + n%16.  Coefficients: out_r = Er Wr - sgn Ei Wi,
+// out_i = Er Wi + sgn Ei Wr.
+__global__ void k_pack_mfma(int n, int nrhs, int tiles, int n_steps, float sign,
+                            const float2 *W, const unsigned int *maxbits, BPair *out) {
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)n_steps * tiles * 64) return;
+    const int l = (int)(idx & 63);
+    const int t = (int)((idx >> 6) % tiles);
+    const int st = (int)((idx >> 6) / tiles);
+    const int h = l >> 5, nn = l & 31;
+    const int part = nn >> 4;
+    const int c = t * MF_COLS + (nn & 15);
+    const float scale = ldexpf(1.f, w_shift(*maxbits));
+    BPair bp;
+#pragma unroll
+    for (int j = 0; j < MF_K; ++j) {
+        const int b = st * MF_K + j;
+        float v = 0.f;
+        if (b < n && c < nrhs) {
+            const float2 w = W[(size_t)c * n + b];
+            if (h == 0) v = part == 0 ? w.x : w.y;
+            else v = part == 0 ? -sign * w.y : sign * w.x;
+        }
+        v *= scale;
+        const _Float16 vh = (_Float16)v;
+        bp.hi[j] = vh;
+        bp.lo[j] = (_Float16)(v - (float)vh);
+    }
+    out[idx] = bp;
 }
 
 // ---- reductions over chunk partials (FP64) ---------------------------------
 
-__global__ void k_reduce_combine(const float2 *partial, int nparts, int n,
-                                 float *out_r, float *out_i) {
+__global__ void k_reduce_combine(const float2 *partial, int nparts, int n, float *out_r,
+                                 float *out_i) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= n) return;
     double sr = 0.0, si = 0.0;
@@ -292,17 +405,16 @@ __global__ void k_reduce_combine(const float2 *partial, int nparts, int n,
     out_i[a] = (float)si;
 }
 
-// out[c][a] = sum_chunks partial[chunk][c][a]; cumulative: prefix sum over c
-__global__ void k_reduce_percoil(const float2 *partial, int nchunks, int ncoils,
-                                 int n, int cumulative, float *out_r,
-                                 float *out_i) {
+// VALU per-rhs partials [chunk][c][n] float2 -> out[c][n]; cumulative: prefix over c
+__global__ void k_reduce_percoil(const float2 *partial, int nchunks, int nrhs, int n,
+                                 int cumulative, float *out_r, float *out_i) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= n) return;
     double cr = 0.0, ci = 0.0;
-    for (int c = 0; c < ncoils; ++c) {
+    for (int c = 0; c < nrhs; ++c) {
         double sr = 0.0, si = 0.0;
         for (int p = 0; p < nchunks; ++p) {
-            const float2 v = partial[((size_t)p * ncoils + c) * n + a];
+            const float2 v = partial[((size_t)p * nrhs + c) * n + a];
             sr += v.x;
             si += v.y;
         }
@@ -317,78 +429,186 @@ __global__ void k_reduce_percoil(const float2 *partial, int nchunks, int ncoils,
     }
 }
 
-// ---- dispatch ----------------------------------------------------------------
-
-using KFn = void (*)(const float4 *, int, const float *, int, int, float, int,
-                     const float *, const float *, float2 *);
-
-template <int G, bool REALW, int EPI>
-KFn pick_d3(bool d3) {
-    return d3 ? (KFn)k_nudft<G, true, REALW, EPI> : (KFn)k_nudft<G, false, REALW, EPI>;
-}
-
-template <bool REALW, int EPI>
-KFn pick(int G, bool d3) {
-    switch (G) {
-        case 1: return pick_d3<1, REALW, EPI>(d3);
-        case 2: return pick_d3<2, REALW, EPI>(d3);
-        case 4: return pick_d3<4, REALW, EPI>(d3);
-        case 8: return pick_d3<8, REALW, EPI>(d3);
-        case 12: return pick_d3<12, REALW, EPI>(d3);
-        default: return pick_d3<16, REALW, EPI>(d3);
+// MFMA planes [chunk][tiles*32][n] -> combine with conj(S_c) (mode 0),
+// per-rhs output (mode 1), per-rhs prefix sums (mode 2); undoes the
+// E and W scaling.
+__global__ void k_reduce_mfma(const float *partial, int nchunks, int tiles, int nrhs, int n,
+                              int mode, const unsigned int *maxbits, const float *s_r,
+                              const float *s_i, float *out_r, float *out_i) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    const double unscale = ldexp(1.0, -w_shift(*maxbits)) / (double)E_SCALE;
+    const int nplanes = tiles * 32;
+    double orr = 0.0, oi = 0.0, cr = 0.0, ci = 0.0;
+    for (int c = 0; c < nrhs; ++c) {
+        const int t = c / MF_COLS, cc = c % MF_COLS;
+        double vr = 0.0, vi = 0.0;
+        for (int p = 0; p < nchunks; ++p) {
+            const size_t base = (size_t)p * nplanes + t * 32;
+            vr += partial[(base + cc) * n + a];
+            vi += partial[(base + MF_COLS + cc) * n + a];
+        }
+        vr *= unscale;
+        vi *= unscale;
+        if (mode == 0) {
+            double sr = 1.0, si = 0.0;
+            if (s_r) {
+                sr = s_r[(size_t)c * n + a];
+                si = s_i[(size_t)c * n + a];
+            }
+            orr += sr * vr + si * vi;
+            oi += sr * vi - si * vr;
+        } else {
+            if (mode == 2) {
+                cr += vr;
+                ci += vi;
+                vr = cr;
+                vi = ci;
+            }
+            out_r[(size_t)c * n + a] = (float)vr;
+            out_i[(size_t)c * n + a] = (float)vi;
+        }
+    }
+    if (mode == 0) {
+        out_r[a] = (float)orr;
+        out_i[a] = (float)oi;
     }
 }
 
-int group_size(int nrhs) {
-    if (nrhs <= 1) return 1;
-    if (nrhs <= 2) return 2;
-    if (nrhs <= 4) return 4;
-    if (nrhs <= 8) return 8;
-    if (nrhs <= 12) return 12;
-    return 16;
+// ---- dispatch ----------------------------------------------------------------
+
+using VFn = void (*)(const float4 *, int, const double4 *, const float *, int, int, float, int,
+                     const float *, const float *, float2 *);
+
+template <int G, int EPI>
+VFn pick_valu_d3(bool d3) {
+    return d3 ? (VFn)k_nudft_valu<G, true, EPI> : (VFn)k_nudft_valu<G, false, EPI>;
 }
 
-// Number of stream chunks so the launch holds enough waves to fill the GPU.
-int pick_chunks(imp_ctx *ctx, int lane_blocks, int groups, int n_stream) {
-    const long target = 8L * ctx->num_cus;  // workgroups (4 waves each)
-    long have = (long)lane_blocks * groups;
+template <int EPI>
+VFn pick_valu(int G, bool d3) {
+    switch (G) {
+        case 1: return pick_valu_d3<1, EPI>(d3);
+        case 2: return pick_valu_d3<2, EPI>(d3);
+        default: return pick_valu_d3<4, EPI>(d3);
+    }
+}
+
+using MFn = void (*)(const float4 *, int, const double4 *, const BPair *, int, int, float *);
+
+MFn pick_mfma(int tiles, bool d3) {
+    if (tiles == 1) return d3 ? (MFn)k_nudft_mfma<1, true> : (MFn)k_nudft_mfma<1, false>;
+    return d3 ? (MFn)k_nudft_mfma<2, true> : (MFn)k_nudft_mfma<2, false>;
+}
+
+// right-hand sides at or above which the MFMA kernel is used
+constexpr int MFMA_MIN_RHS = 3;
+
+// Chunks so the launch holds enough workgroups to fill the GPU.
+int pick_chunks(imp_ctx *ctx, long wgs, int n_units, int min_units_per_chunk) {
+    const long target = 8L * ctx->num_cus;
     int chunks = 1;
-    while (have * chunks < target && n_stream / (chunks * 2) >= 4 * NU_BLK) chunks *= 2;
+    while (wgs * chunks < target && n_units / (chunks * 2) >= min_units_per_chunk) chunks *= 2;
     return chunks;
 }
 
-struct NudftLaunch {
+enum class Out { Combine, PerRhs, PerRhsCumulative };
+
+// Shared driver: lanes + stream positions + W are built by the caller.
+struct Job {
     const float4 *lanes;
     int n_lane;
-    const float *stream;
+    const double4 *spos;     // padded to a multiple of MF_K
     int n_stream;
-    int G, groups, nrhs;
-    bool d3, realw;
-    int epi;
+    const float2 *W;         // [nrhs][n_stream]
+    int nrhs;
+    bool d3;
     float sign;
-    const float *s_r, *s_i;
+    Out out;
+    const float *s_r, *s_i;  // Combine only
+    float *out_r, *out_i;
 };
 
-int run_nudft(imp_ctx *ctx, const NudftLaunch &L, float2 **partial_out, int *nchunks_out) {
-    const int lane_blocks = ceil_div(L.n_lane, NU_THREADS);
-    const int chunks = pick_chunks(ctx, lane_blocks, L.groups, L.n_stream);
-    int chunk_len = ceil_div(L.n_stream, chunks);
-    chunk_len = ceil_div(chunk_len, NU_BLK) * NU_BLK;
-    const size_t nparts = L.epi == EPI_COMBINE ? (size_t)L.groups * chunks
-                                               : (size_t)chunks * L.nrhs;
-    void *part = nullptr;
-    IMP_TRY(workspace(ctx, 2, nparts * L.n_lane * sizeof(float2), &part));
-    KFn fn = L.realw ? (L.epi == EPI_COMBINE ? pick<true, EPI_COMBINE>(L.G, L.d3)
-                                             : pick<true, EPI_PERCOIL>(L.G, L.d3))
-                     : (L.epi == EPI_COMBINE ? pick<false, EPI_COMBINE>(L.G, L.d3)
-                                             : pick<false, EPI_PERCOIL>(L.G, L.d3));
-    dim3 grid(lane_blocks, chunks, L.groups);
-    hipLaunchKernelGGL(fn, grid, dim3(NU_THREADS), 0, ctx->stream, L.lanes, L.n_lane,
-                       L.stream, L.n_stream, chunk_len, L.sign, L.nrhs, L.s_r, L.s_i,
-                       (float2 *)part);
+int run_job(imp_ctx *ctx, const Job &J) {
+    const hipStream_t st = ctx->stream;
+    const int n = J.n_lane;
+    if (J.nrhs >= MFMA_MIN_RHS) {
+        int total_tiles = ceil_div(J.nrhs, MF_COLS);
+        if (total_tiles > 2)
+            return fail(IMP_EUNSUPPORTED, "NUDFT: %d right-hand sides (at most 32)", J.nrhs);
+        const int tiles = total_tiles;
+        const int n_steps = ceil_div(J.n_stream, MF_K);
+        // max |W| for the power-of-two scale
+        unsigned int *maxbits = nullptr;
+        void *mb = nullptr, *bp = nullptr, *part = nullptr;
+        IMP_TRY(workspace(ctx, 5, 256, &mb));
+        maxbits = (unsigned int *)mb;
+        IMP_HIP(hipMemsetAsync(maxbits, 0, sizeof(unsigned int), st));
+        hipLaunchKernelGGL(k_absmax, dim3(512), dim3(256), 0, st, J.W,
+                           (size_t)J.nrhs * J.n_stream, maxbits);
+        IMP_CHECK_LAUNCH();
+        const size_t nb = (size_t)n_steps * tiles * 64;
+        IMP_TRY(workspace(ctx, 3, nb * sizeof(BPair), &bp));
+        hipLaunchKernelGGL(k_pack_mfma, dim3(ceil_div((long)nb, 256)), dim3(256), 0, st,
+                           J.n_stream, J.nrhs, tiles, n_steps, J.sign, J.W, maxbits,
+                           (BPair *)bp);
+        IMP_CHECK_LAUNCH();
+        const int mtiles = ceil_div(n, 32);
+        const long wgs = ceil_div(mtiles, NU_THREADS / 64);
+        const int chunks = pick_chunks(ctx, wgs, n_steps, 4 * MF_FOLD);
+        const int spc = ceil_div(n_steps, chunks);
+        IMP_TRY(workspace(ctx, 2, (size_t)chunks * tiles * 32 * n * sizeof(float), &part));
+        hipLaunchKernelGGL(pick_mfma(tiles, J.d3), dim3((unsigned)wgs, chunks), dim3(NU_THREADS),
+                           0, st, J.lanes, n, J.spos, (const BPair *)bp, n_steps, spc,
+                           (float *)part);
+        IMP_CHECK_LAUNCH();
+        const int mode = J.out == Out::Combine ? 0 : (J.out == Out::PerRhs ? 1 : 2);
+        hipLaunchKernelGGL(k_reduce_mfma, dim3(ceil_div(n, 256)), dim3(256), 0, st,
+                           (const float *)part, chunks, tiles, J.nrhs, n, mode, maxbits, J.s_r,
+                           J.s_i, J.out_r, J.out_i);
+        IMP_CHECK_LAUNCH();
+        return IMP_OK;
+    }
+    // VALU path
+    const int G = J.nrhs <= 1 ? 1 : (J.nrhs <= 2 ? 2 : 4);
+    const int groups = ceil_div(J.nrhs, G);
+    void *ws = nullptr, *part = nullptr;
+    IMP_TRY(workspace(ctx, 3, (size_t)groups * J.n_stream * 2 * G * sizeof(float), &ws));
+    hipLaunchKernelGGL(k_pack_valu, dim3(ceil_div(J.n_stream, 256), groups), dim3(256), 0, st,
+                       J.n_stream, J.nrhs, G, J.W, (float *)ws);
     IMP_CHECK_LAUNCH();
-    *partial_out = (float2 *)part;
-    *nchunks_out = chunks;
+    const int lane_blocks = ceil_div(n, NU_THREADS);
+    const int chunks = pick_chunks(ctx, (long)lane_blocks * groups, J.n_stream, 4 * NU_BLK);
+    int chunk_len = ceil_div(ceil_div(J.n_stream, chunks), NU_BLK) * NU_BLK;
+    const bool combine = J.out == Out::Combine;
+    const size_t nparts = combine ? (size_t)groups * chunks : (size_t)chunks * J.nrhs;
+    IMP_TRY(workspace(ctx, 2, nparts * n * sizeof(float2), &part));
+    VFn fn = combine ? pick_valu<EPI_COMBINE>(G, J.d3) : pick_valu<EPI_PERCOIL>(G, J.d3);
+    hipLaunchKernelGGL(fn, dim3(lane_blocks, chunks, groups), dim3(NU_THREADS), 0, st, J.lanes,
+                       n, J.spos, (const float *)ws, J.n_stream, chunk_len, J.sign, J.nrhs,
+                       J.s_r, J.s_i, (float2 *)part);
+    IMP_CHECK_LAUNCH();
+    if (combine) {
+        hipLaunchKernelGGL(k_reduce_combine, dim3(ceil_div(n, 256)), dim3(256), 0, st,
+                           (const float2 *)part, groups * chunks, n, J.out_r, J.out_i);
+    } else {
+        hipLaunchKernelGGL(k_reduce_percoil, dim3(ceil_div(n, 256)), dim3(256), 0, st,
+                           (const float2 *)part, chunks, J.nrhs, n,
+                           J.out == Out::PerRhsCumulative ? 1 : 0, J.out_r, J.out_i);
+    }
+    IMP_CHECK_LAUNCH();
+    return IMP_OK;
+}
+
+int build_spos(imp_ctx *ctx, int n, const float *x, const float *y, const float *z,
+               const float *w, double wscale, const double4 **out) {
+    void *p = nullptr;
+    const int n_pad = ceil_div(n, MF_K) * MF_K;
+    IMP_TRY(workspace(ctx, 0, (size_t)n_pad * sizeof(double4), &p));
+    hipLaunchKernelGGL(k_spos, dim3(ceil_div(n_pad, 256)), dim3(256), 0, ctx->stream, n, n_pad,
+                       x, y, z, w, wscale, (double4 *)p);
+    IMP_CHECK_LAUNCH();
+    *out = (const double4 *)p;
     return IMP_OK;
 }
 
@@ -407,29 +627,21 @@ int nudft_adjoint(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *k
         (phi_r && !phi_i))
         return fail(IMP_EINVAL, "nudft_adjoint: bad arguments");
     const bool d3 = kz && z;
-    const int G = group_size(ncoils);
-    const int groups = ceil_div(ncoils, G);
-    const int S = stream_stride(G, false);
-    void *stream = nullptr, *lanes = nullptr;
-    IMP_TRY(workspace(ctx, 0, (size_t)groups * num_k * S * sizeof(float), &stream));
+    const double4 *spos = nullptr;
+    IMP_TRY(build_spos(ctx, num_k, kx, ky, d3 ? kz : nullptr, t, 1.0, &spos));
+    void *lanes = nullptr, *W = nullptr;
     IMP_TRY(workspace(ctx, 1, (size_t)num_x * sizeof(float4), &lanes));
-    hipLaunchKernelGGL(k_pack_adjoint, dim3(ceil_div(num_k, 256), groups), dim3(256), 0,
-                       ctx->stream, num_k, ncoils, G, S, kx, ky, d3 ? kz : nullptr, t,
-                       phi_r, phi_i, d_r, d_i, (float *)stream);
-    IMP_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_lanes_from_arrays, dim3(ceil_div(num_x, 256)), dim3(256), 0,
-                       ctx->stream, num_x, x, y, d3 ? z : nullptr, fm,
-                       1.0 / (2.0 * M_PI), (float4 *)lanes);
+                       ctx->stream, num_x, x, y, d3 ? z : nullptr, fm, 1.0 / (2.0 * M_PI),
+                       (float4 *)lanes);
     IMP_CHECK_LAUNCH();
-    NudftLaunch L{(const float4 *)lanes, num_x, (const float *)stream, num_k, G, groups,
-                  ncoils, d3, false, EPI_COMBINE, +1.f, s_r, s_i};
-    float2 *partial = nullptr;
-    int chunks = 0;
-    IMP_TRY(run_nudft(ctx, L, &partial, &chunks));
-    hipLaunchKernelGGL(k_reduce_combine, dim3(ceil_div(num_x, 256)), dim3(256), 0,
-                       ctx->stream, partial, groups * chunks, num_x, out_r, out_i);
+    IMP_TRY(workspace(ctx, 4, (size_t)num_k * ncoils * sizeof(float2), &W));
+    hipLaunchKernelGGL(k_w_adjoint, dim3(ceil_div((long)num_k * ncoils, 256)), dim3(256), 0,
+                       ctx->stream, num_k, ncoils, phi_r, phi_i, d_r, d_i, (float2 *)W);
     IMP_CHECK_LAUNCH();
-    return IMP_OK;
+    Job J{(const float4 *)lanes, num_x, spos, num_k, (const float2 *)W, ncoils, d3, +1.f,
+          Out::Combine, s_r, s_i, out_r, out_i};
+    return run_job(ctx, J);
 }
 
 int nudft_forward(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *kx,
@@ -441,29 +653,20 @@ int nudft_forward(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *k
         !x || !y || !rho_r || !rho_i || !d_r || !d_i || (s_r && !s_i))
         return fail(IMP_EINVAL, "nudft_forward: bad arguments");
     const bool d3 = kz && z;
-    const int G = group_size(ncoils);
-    const int groups = ceil_div(ncoils, G);
-    const int S = stream_stride(G, false);
-    void *stream = nullptr, *lanes = nullptr;
-    IMP_TRY(workspace(ctx, 0, (size_t)groups * num_x * S * sizeof(float), &stream));
+    const double4 *spos = nullptr;
+    IMP_TRY(build_spos(ctx, num_x, x, y, d3 ? z : nullptr, fm, 1.0 / (2.0 * M_PI), &spos));
+    void *lanes = nullptr, *W = nullptr;
     IMP_TRY(workspace(ctx, 1, (size_t)num_k * sizeof(float4), &lanes));
-    hipLaunchKernelGGL(k_pack_forward, dim3(ceil_div(num_x, 256), groups), dim3(256), 0,
-                       ctx->stream, num_x, ncoils, G, S, x, y, d3 ? z : nullptr, fm,
-                       rho_r, rho_i, s_r, s_i, (float *)stream);
-    IMP_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_lanes_from_arrays, dim3(ceil_div(num_k, 256)), dim3(256), 0,
-                       ctx->stream, num_k, kx, ky, d3 ? kz : nullptr, t, 1.0,
-                       (float4 *)lanes);
+                       ctx->stream, num_k, kx, ky, d3 ? kz : nullptr, t, 1.0, (float4 *)lanes);
     IMP_CHECK_LAUNCH();
-    NudftLaunch L{(const float4 *)lanes, num_k, (const float *)stream, num_x, G, groups,
-                  ncoils, d3, false, EPI_PERCOIL, -1.f, nullptr, nullptr};
-    float2 *partial = nullptr;
-    int chunks = 0;
-    IMP_TRY(run_nudft(ctx, L, &partial, &chunks));
-    hipLaunchKernelGGL(k_reduce_percoil, dim3(ceil_div(num_k, 256)), dim3(256), 0,
-                       ctx->stream, partial, chunks, ncoils, num_k, 0, d_r, d_i);
+    IMP_TRY(workspace(ctx, 4, (size_t)num_x * ncoils * sizeof(float2), &W));
+    hipLaunchKernelGGL(k_w_forward, dim3(ceil_div((long)num_x * ncoils, 256)), dim3(256), 0,
+                       ctx->stream, num_x, ncoils, rho_r, rho_i, s_r, s_i, (float2 *)W);
     IMP_CHECK_LAUNCH();
-    return IMP_OK;
+    Job J{(const float4 *)lanes, num_k, spos, num_x, (const float2 *)W, ncoils, d3, -1.f,
+          Out::PerRhs, nullptr, nullptr, d_r, d_i};
+    return run_job(ctx, J);
 }
 
 int toeplitz_q(imp_ctx *ctx, int num_k, const float *kx, const float *ky,
@@ -487,40 +690,33 @@ int toeplitz_q(imp_ctx *ctx, int num_k, const float *kx, const float *ky,
     }
     const bool d3 = kz && (std_grid ? Nz > 1 : gz != nullptr);
     const int nseg = ntime_segs + 1;
+    if (nseg > 32) return fail(IMP_EUNSUPPORTED, "toeplitz_q: at most 31 time segments");
     // tau and t0 as the reference computes them, in FP32 on the host
     // (computeQ.cmem.cu:870-871)
     const float t0 = t_host_first_last[0];
     const float tlast = t_host_first_last[1];
     const float tau = ntime_segs > 0 ? (tlast - t0) / (float)ntime_segs : (tlast - t0);
-    const int G = group_size(nseg);
-    const int groups = ceil_div(nseg, G);
-    const int S = stream_stride(G, true);
-    void *stream = nullptr, *lanes = nullptr;
-    IMP_TRY(workspace(ctx, 0, (size_t)groups * num_k * S * sizeof(float), &stream));
+    const double4 *spos = nullptr;
+    IMP_TRY(build_spos(ctx, num_k, kx, ky, d3 ? kz : nullptr, nullptr, 0.0, &spos));
+    void *lanes = nullptr, *W = nullptr;
     IMP_TRY(workspace(ctx, 1, (size_t)num_q * sizeof(float4), &lanes));
-    hipLaunchKernelGGL(k_pack_q, dim3(ceil_div(num_k, 256), groups), dim3(256), 0,
-                       ctx->stream, num_k, nseg, G, S, kx, ky, d3 ? kz : nullptr, t,
-                       phi_r, phi_i, tau, t0, (float *)stream);
-    IMP_CHECK_LAUNCH();
     if (std_grid) {
-        hipLaunchKernelGGL(k_lanes_qgrid, dim3(ceil_div(num_q, 256)), dim3(256), 0,
-                           ctx->stream, Nx, Ny, d3 ? Nz : 1, (float4 *)lanes);
+        hipLaunchKernelGGL(k_lanes_qgrid, dim3(ceil_div(num_q, 256)), dim3(256), 0, ctx->stream,
+                           Nx, Ny, d3 ? Nz : 1, (float4 *)lanes);
     } else {
         hipLaunchKernelGGL(k_lanes_from_arrays, dim3(ceil_div(num_q, 256)), dim3(256), 0,
                            ctx->stream, num_q, gx, gy, d3 ? gz : nullptr, nullptr, 0.0,
                            (float4 *)lanes);
     }
     IMP_CHECK_LAUNCH();
-    NudftLaunch L{(const float4 *)lanes, num_q, (const float *)stream, num_k, G, groups,
-                  nseg, d3, true, EPI_PERCOIL, +1.f, nullptr, nullptr};
-    float2 *partial = nullptr;
-    int chunks = 0;
-    IMP_TRY(run_nudft(ctx, L, &partial, &chunks));
-    hipLaunchKernelGGL(k_reduce_percoil, dim3(ceil_div(num_q, 256)), dim3(256), 0,
-                       ctx->stream, partial, chunks, nseg, num_q,
-                       (flags & IMP_Q_CUMULATIVE) ? 1 : 0, q_r, q_i);
+    IMP_TRY(workspace(ctx, 4, (size_t)num_k * nseg * sizeof(float2), &W));
+    hipLaunchKernelGGL(k_w_q, dim3(ceil_div((long)num_k * nseg, 256)), dim3(256), 0,
+                       ctx->stream, num_k, nseg, t, phi_r, phi_i, tau, t0, (float2 *)W);
     IMP_CHECK_LAUNCH();
-    return IMP_OK;
+    Job J{(const float4 *)lanes, num_q, spos, num_k, (const float2 *)W, nseg, d3, +1.f,
+          (flags & IMP_Q_CUMULATIVE) ? Out::PerRhsCumulative : Out::PerRhs, nullptr, nullptr,
+          q_r, q_i};
+    return run_job(ctx, J);
 }
 
 }  // namespace imp
