@@ -12,8 +12,10 @@ inputs resident in HBM.  value = pairs / time in GSample*vox/s.
 The same run also reports, once each: the Toeplitz kernels Q_l, the
 resident FFT(Q_l) plan, and plain CG to relative residual 1e-5.
 
-The roofline entry is for the dominant kernel (the NUDFT, k_nudft), from
-HIP events on the library's stream around every step.
+The roofline entry is for the dominant kernel (k_nudft_mfma), from HIP
+events on the library's stream around every step: 24 fp16 MFMA flops per
+(coil, sample, voxel) pair (a complex MAC as three fp16 products) against the
+dense fp16 MFMA peak.
 The cpu_baseline leg times the C restatement of the reference CPU path
 (iftCpu loop order, FP32 sinf/cosf, OpenMP inner reduction) on a bounded
 sample of the same adjoint and converts it to the same unit.
@@ -31,8 +33,11 @@ import numpy as np
 REPO = Path(__file__).resolve().parent
 sys.path.insert(0, str(REPO / "impatient-mri_amd"))
 
-PEAK_FP32_TFLOPS = 157.3     # MI355X FP32 (vector and dense matrix)
+PEAK_FP16_MFMA_TFLOPS = 2500.0   # MI355X dense fp16 MFMA
 PEAK_HBM_GBS = 8000.0
+# fp16 MFMA flops per (coil, sample, voxel) pair: one complex MAC is 8 real
+# flops, done as 3 fp16 products (Eh Wh + Eh Wl + El Wh) for FP32 accuracy
+MFMA_FLOPS_PER_PAIR = 24.0
 
 
 def hip_events():
@@ -73,17 +78,21 @@ def cpu_baseline(prob, d_host, seconds_target=15.0):
     kx, ky, t = f32(prob.kx), f32(prob.ky), f32(prob.t)
     x, y, fm = f32(prob.x), f32(prob.y), f32(prob.fm)
     dr, di = f32(d_host[0].real), f32(d_host[0].imag)
-    # calibrate on a small sample, then size the timed sample
-    m = 256
-    t0 = time.perf_counter()
-    lib.ift_cpu(P(out_r), P(out_i), P(dr), P(di), P(kx), P(ky), P(x), P(y), P(fm), P(t),
-                C.c_int(m), C.c_int(nv))
-    dt = time.perf_counter() - t0
-    m = int(min(prob.nk, max(256, m * seconds_target / max(dt, 1e-6))))
-    t0 = time.perf_counter()
-    lib.ift_cpu(P(out_r), P(out_i), P(dr), P(di), P(kx), P(ky), P(x), P(y), P(fm), P(t),
-                C.c_int(m), C.c_int(nv))
-    dt = time.perf_counter() - t0
+    # The restated loop has a per-voxel parallel region over samples, so its
+    # time is a + b*m rather than b*m: fit both from two samples, then size
+    # the timed sample to about seconds_target.
+    def run(m):
+        t0 = time.perf_counter()
+        lib.ift_cpu(P(out_r), P(out_i), P(dr), P(di), P(kx), P(ky), P(x), P(y), P(fm), P(t),
+                    C.c_int(m), C.c_int(nv))
+        return time.perf_counter() - t0
+    run(16)  # thread pool start-up
+    m1, m2 = 128, 512
+    t1, t2 = run(m1), run(m2)
+    b = max((t2 - t1) / (m2 - m1), 1e-9)
+    a = max(t1 - b * m1, 0.0)
+    m = int(min(prob.nk, max(m2, (seconds_target - a) / b)))
+    dt = run(m)
     pairs = float(m) * nv
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": pairs / dt / 1e9, "unit": "GSample*vox/s", "cores": cores,
@@ -153,9 +162,18 @@ def main():
              ptr(dky), None, ptr(dt_), ptr(dx), ptr(dy), None, ptr(dfm), None, None, ptr(ddr),
              ptr(ddi), ptr(dsr), ptr(dsi), ptr(dbr), ptr(dbi))
 
+    def kernel_ms(name):
+        tot, n = C.c_double(), C.c_int()
+        call("imp_profile_read", ctx.h, name.encode(), C.byref(tot), C.byref(n))
+        return tot.value, n.value
+
     for _ in range(args.warmup):
         fhd()
     ctx.synchronize()
+    # library kernel timer: HIP events on the library stream around each
+    # NUDFT / Toeplitz-pass launch inside the timed region
+    call("imp_profile_enable", ctx.h, C.c_int(1))
+    call("imp_profile_reset", ctx.h)
     evs = [Ev() for _ in range(args.steps + 1)]
     evs[0].record(stream)
     t0 = time.perf_counter()
@@ -171,13 +189,17 @@ def main():
     log(f"F^H d: {ms:.2f} ms/step (events), wall {wall / args.steps * 1e3:.2f} ms/step, "
         f"{value:.1f} GSample*vox/s")
 
-    # algorithmic flops per launch: 8 per (coil, sample, voxel) complex MAC +
-    # 8 per (sample, voxel) for the phase (3 FMA) and the two transcendentals
-    flops = 8.0 * pairs + 8.0 * nk * nv
-    achieved = flops / (ms * 1e-3) / 1e12
-    roofline = {"bound": "valu", "achieved": round(achieved, 2), "peak": PEAK_FP32_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                "traffic": None, "kernel": "k_nudft (F^H d, 16 coils per pass)"}
+    k_tot, k_n = kernel_ms("k_nudft_mfma")
+    k_ms = k_tot / max(k_n, 1)
+    log(f"k_nudft_mfma: {k_n} launches, {k_ms:.2f} ms average")
+    flops = MFMA_FLOPS_PER_PAIR * pairs
+    achieved = flops / (k_ms * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "achieved": round(achieved, 1), "peak": PEAK_FP16_MFMA_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP16_MFMA_TFLOPS, 4),
+                "traffic": None,
+                "kernel": "k_nudft_mfma<2,false> (F^H d, 32 coils in one pass)",
+                "kernel_ms": round(k_ms, 3),
+                "work": "24 fp16 MFMA flops per (coil, sample, voxel) pair"}
 
     result = {
         "metric": "NUDFT GSample*vox/s (direct field-corrected F^H d); time to rel-residual 1e-5 "
@@ -219,6 +241,7 @@ def main():
         dxr, dxi = ctx.alloc((nv,)), ctx.alloc((nv,))
         call("imp_toep_apply", h, ptr(dbr), ptr(dbi), ptr(dxr), ptr(dxi))
         ctx.synchronize()
+        call("imp_profile_reset", ctx.h)
         e0.record(stream)
         for _ in range(3):
             call("imp_toep_apply", h, ptr(dbr), ptr(dbi), ptr(dxr), ptr(dxi))
@@ -226,6 +249,24 @@ def main():
         e1.sync()
         t_apply = e1.ms_since(e0) / 3
         log(f"Toeplitz plan {t_plan:.3f} s, apply {t_apply:.2f} ms")
+        # HBM roofline of the FFT passes: algorithmic bytes per launch (one
+        # segment l, all coils; complex64 = 8 B; the padded grid keeps the
+        # N1 x 2N2 rows that hold data)
+        half = 2 * nv                                   # N1 x 2 N2 points
+        pass_bytes = {
+            "toep_pass_a": 8 * nv + 4 * nv + 8 * ncoils * nv + 8 * ncoils * half,
+            "toep_pass_c": 8 * ncoils * half * 2 + 8 * 4 * nv,
+            "toep_pass_e": 8 * ncoils * half + 8 * ncoils * nv,
+        }
+        passes = {}
+        for name, nbytes in pass_bytes.items():
+            tot, n = kernel_ms(name)
+            avg = tot / max(n, 1)
+            passes[name] = {"launches": n, "avg_ms": round(avg, 4),
+                            "algorithmic_bytes": nbytes,
+                            "gbs": round(nbytes / (avg * 1e-3) / 1e9, 1) if avg > 0 else None}
+            log(f"{name}: {n} launches, {avg:.4f} ms avg, {passes[name]['gbs']} GB/s")
+        call("imp_profile_enable", ctx.h, C.c_int(0))
         if args.pmc_mode:
             # calibration kernel for the byte counters: a 256 MiB device copy
             hip = C.CDLL("libamdhip64.so")
@@ -253,6 +294,7 @@ def main():
             "fhd_s": round(ms * 1e-3, 4), "q_s": round(t_q, 4),
             "q_direct_equiv_gsample_vox_s": round(q_pairs / t_q / 1e9, 1),
             "toeplitz_plan_s": round(t_plan, 4), "toeplitz_apply_ms": round(t_apply, 3),
+            "toeplitz_passes": passes,
             "cg_plain_iterations": s["iterations"], "cg_plain_converged": s["converged"],
             "cg_plain_s": round(t_cg, 4), "cg_plain_rel_residual": s["rel_residual"],
             "cg_plain_true_rel_residual": s["true_rel_residual"],

@@ -103,7 +103,7 @@ int bruteforce_cg(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *k
     // buffers: y, Af, yAf, q  [2nk];  X, GtAf, grad, d, CtCf, CtCd [2nx];  cbuf [2nx float2]
     const size_t bytes = (4 * 2 * nk + 6 * 2 * nx) * sizeof(float) + 2 * nx * sizeof(float2);
     void *ws = nullptr;
-    IMP_TRY(workspace(ctx, 6, bytes, &ws));
+    IMP_TRY(workspace(ctx, WS_BFORCE, bytes, &ws));
     float *yv = (float *)ws, *Af = yv + 2 * nk, *yAf = Af + 2 * nk, *q = yAf + 2 * nk;
     float *X = q + 2 * nk, *GtAf = X + 2 * nx, *grad = GtAf + 2 * nx, *dv = grad + 2 * nx,
           *CtCf = dv + 2 * nx, *CtCd = CtCf + 2 * nx;

@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <set>
 #include <string>
+#include <vector>
 
 #include "impatient.h"
 
@@ -49,6 +50,26 @@ struct Workspace {
     size_t bytes = 0;
 };
 
+// One timed launch: HIP events recorded on the context stream around it.
+struct ProfRec {
+    const char *name;
+    hipEvent_t begin, end;
+};
+
+// Independent scratch regions of a context (a call may use several at once).
+enum WsSlot {
+    WS_SPOS = 0,   // NUDFT stream positions
+    WS_LANES,      // NUDFT lane positions
+    WS_PARTIAL,    // NUDFT chunk partial sums
+    WS_BPACK,      // NUDFT packed weights
+    WS_W,          // NUDFT complex weights
+    WS_MAXBITS,    // NUDFT weight scale
+    WS_BFORCE,     // brute-force CG vectors
+    WS_REG,        // regularizer / TV staging
+    WS_QHALF,      // Q on the half grid (Hermitian fill)
+    WS_COUNT
+};
+
 }  // namespace imp
 
 struct imp_ctx {
@@ -56,7 +77,7 @@ struct imp_ctx {
     hipStream_t stream = nullptr;
     int num_cus = 256;
     // growable scratch regions; index selects an independent region
-    imp::Workspace ws[8];
+    imp::Workspace ws[imp::WS_COUNT];
     // FP64 reduction scratch for dot products (device)
     double *dscratch = nullptr;
     size_t dscratch_n = 0;
@@ -66,6 +87,10 @@ struct imp_ctx {
     float2 *twiddle = nullptr;
     // live Toeplitz operators created on this context
     std::set<struct imp_toep *> toeps;
+    // kernel timer (imp_profile_*): off by default
+    bool prof_on = false;
+    std::vector<imp::ProfRec> prof_recs;   // recorded this session
+    std::vector<hipEvent_t> prof_free;     // reusable events
 };
 
 namespace imp {
@@ -73,6 +98,11 @@ namespace imp {
 // Returns a device scratch region of at least `bytes` (grows, never shrinks).
 int workspace(imp_ctx *ctx, int slot, size_t bytes, void **out);
 int ensure_dscratch(imp_ctx *ctx, size_t n);
+
+// Kernel timer: prof_begin records an event before a launch and returns a
+// record index (-1 when the timer is off); prof_end records the closing event.
+int prof_begin(imp_ctx *ctx, const char *name);
+void prof_end(imp_ctx *ctx, int rec);
 
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
 

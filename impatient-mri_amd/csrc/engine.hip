@@ -86,6 +86,41 @@ extern "C" const char *imp_status_string(int s) {
 
 extern "C" const char *imp_last_error(void) { return imp::last_error(); }
 
+namespace imp {
+
+int prof_begin(imp_ctx *ctx, const char *name) {
+    if (!ctx->prof_on) return -1;
+    hipEvent_t ev[2];
+    for (int k = 0; k < 2; ++k) {
+        if (!ctx->prof_free.empty()) {
+            ev[k] = ctx->prof_free.back();
+            ctx->prof_free.pop_back();
+        } else if (hipEventCreate(&ev[k]) != hipSuccess) {
+            (void)hipGetLastError();
+            if (k == 1) ctx->prof_free.push_back(ev[0]);
+            return -1;  // timing is best effort; the launch itself proceeds
+        }
+    }
+    (void)hipEventRecord(ev[0], ctx->stream);
+    ctx->prof_recs.push_back(ProfRec{name, ev[0], ev[1]});
+    return (int)ctx->prof_recs.size() - 1;
+}
+
+void prof_end(imp_ctx *ctx, int rec) {
+    if (rec < 0 || rec >= (int)ctx->prof_recs.size()) return;
+    (void)hipEventRecord(ctx->prof_recs[rec].end, ctx->stream);
+}
+
+static void prof_release(imp_ctx *ctx) {
+    for (auto &r : ctx->prof_recs) {
+        ctx->prof_free.push_back(r.begin);
+        ctx->prof_free.push_back(r.end);
+    }
+    ctx->prof_recs.clear();
+}
+
+}  // namespace imp
+
 extern "C" int imp_ctx_create(int device, imp_ctx **out) {
     if (!out) return fail(IMP_EINVAL, "imp_ctx_create: out is NULL");
     *out = nullptr;
@@ -130,6 +165,8 @@ extern "C" int imp_ctx_destroy(imp_ctx *ctx) {
         op->ctx = nullptr;
     }
     ctx->toeps.clear();
+    prof_release(ctx);
+    for (hipEvent_t e : ctx->prof_free) (void)hipEventDestroy(e);
     for (auto &w : ctx->ws)
         if (w.ptr) (void)hipFree(w.ptr);
     if (ctx->dscratch) (void)hipFree(ctx->dscratch);
@@ -143,6 +180,38 @@ extern "C" int imp_ctx_destroy(imp_ctx *ctx) {
 extern "C" int imp_ctx_stream(imp_ctx *ctx, void **s) {
     if (!ctx || !s) return fail(IMP_EINVAL, "imp_ctx_stream: NULL argument");
     *s = (void *)ctx->stream;
+    return IMP_OK;
+}
+
+extern "C" int imp_profile_enable(imp_ctx *ctx, int on) {
+    if (!ctx) return fail(IMP_EINVAL, "imp_profile_enable: NULL ctx");
+    ctx->prof_on = on != 0;
+    return IMP_OK;
+}
+
+extern "C" int imp_profile_reset(imp_ctx *ctx) {
+    if (!ctx) return fail(IMP_EINVAL, "imp_profile_reset: NULL ctx");
+    IMP_HIP(hipStreamSynchronize(ctx->stream));
+    prof_release(ctx);
+    return IMP_OK;
+}
+
+extern "C" int imp_profile_read(imp_ctx *ctx, const char *kernel, double *total_ms,
+                                int *launches) {
+    if (!ctx || !kernel || !total_ms || !launches)
+        return fail(IMP_EINVAL, "imp_profile_read: NULL argument");
+    IMP_HIP(hipStreamSynchronize(ctx->stream));
+    double sum = 0.0;
+    int n = 0;
+    for (const auto &r : ctx->prof_recs) {
+        if (std::strcmp(r.name, kernel) != 0) continue;
+        float ms = 0.f;
+        IMP_HIP(hipEventElapsedTime(&ms, r.begin, r.end));
+        sum += ms;
+        ++n;
+    }
+    *total_ms = sum;
+    *launches = n;
     return IMP_OK;
 }
 
@@ -235,7 +304,7 @@ extern "C" int imp_fd_reg_apply(imp_ctx *ctx, int N1, int N2, int N3, float lamb
         return fail(IMP_EINVAL, "imp_fd_reg_apply: bad arguments");
     const long n = (long)N1 * N2 * N3;
     void *buf = nullptr;
-    IMP_TRY(workspace(ctx, 7, 2 * n * sizeof(float2), &buf));
+    IMP_TRY(workspace(ctx, WS_REG, 2 * n * sizeof(float2), &buf));
     float2 *pc = (float2 *)buf, *sc = pc + n;
     IMP_TRY(split_to_c(ctx, p_r, p_i, pc, n));
     IMP_TRY(fd_reg(ctx, N1, N2, N3, lambda, pc, w, sc, false));
@@ -249,7 +318,7 @@ extern "C" int imp_tv_weight(imp_ctx *ctx, int N1, int N2, int N3, const float *
         return fail(IMP_EINVAL, "imp_tv_weight: bad arguments");
     const long n = (long)N1 * N2 * N3;
     void *buf = nullptr;
-    IMP_TRY(workspace(ctx, 7, n * sizeof(float2), &buf));
+    IMP_TRY(workspace(ctx, WS_REG, n * sizeof(float2), &buf));
     IMP_TRY(split_to_c(ctx, x_r, x_i, (float2 *)buf, n));
     IMP_TRY(tv_weight(ctx, N1, N2, N3, (const float2 *)buf, w));
     return IMP_OK;

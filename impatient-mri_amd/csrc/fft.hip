@@ -450,6 +450,7 @@ int toep_apply_normal(imp_ctx *ctx, ToepPlan &P, const float2 *p, float2 *out) {
     for (int l = 0; l < P.nseg; ++l) {
         const float tau_l = P.t0 + P.tau * (float)l;
         // pass A
+        int rec = prof_begin(ctx, "toep_pass_a");
         IMP_L_SWITCH(Ml, {
             const int NL = contig_lines<L_>();
             const long nlines = (long)C * nimg_lines;
@@ -457,9 +458,11 @@ int toep_apply_normal(imp_ctx *ctx, ToepPlan &P, const float2 *p, float2 *out) {
                                ctx->stream, p, P.sens, P.fcyc, tau_l, nimg_lines, nlast, P.N1,
                                N2o, A, C, P.g1, tw_last);
         });
+        prof_end(ctx, rec);
         IMP_CHECK_LAUNCH();
         // pass B (3D): x axis, batches (c, y), lines z in [0, M3), input x < N2
         if (d3) {
+            rec = prof_begin(ctx, "toep_pass_b");
             IMP_L_SWITCH(P.M2, {
                 const int Pn = panel_lines<L_>();
                 dim3 grid(ceil_div(P.M3, Pn), C * P.N1);
@@ -467,22 +470,26 @@ int toep_apply_normal(imp_ctx *ctx, ToepPlan &P, const float2 *p, float2 *out) {
                                    (long)P.M2 * P.M3, (long)P.M3, P.M3, P.N2, P.M2, nullptr,
                                    tw_x);
             });
+            prof_end(ctx, rec);
             IMP_CHECK_LAUNCH();
         }
         // pass C: y axis with Qf multiply, batches c, lines (x,z) in [0, A*Ml)
         {
             const int nl = A * Ml;
             const float2 *qf = P.qf + (long)l * P.G;
+            rec = prof_begin(ctx, "toep_pass_c");
             IMP_L_SWITCH(P.M1, {
                 const int Pn = panel_lines<L_>();
                 dim3 grid(ceil_div(nl, Pn), C);
                 hipLaunchKernelGGL((k_fft_panel<L_, 2>), grid, dim3(FT), 0, ctx->stream, P.g1,
                                    (long)P.N1 * nl, (long)nl, nl, P.N1, P.N1, qf, tw_y);
             });
+            prof_end(ctx, rec);
             IMP_CHECK_LAUNCH();
         }
         // pass D (3D): inverse x, keep x < N2
         if (d3) {
+            rec = prof_begin(ctx, "toep_pass_d");
             IMP_L_SWITCH(P.M2, {
                 const int Pn = panel_lines<L_>();
                 dim3 grid(ceil_div(P.M3, Pn), C * P.N1);
@@ -490,9 +497,11 @@ int toep_apply_normal(imp_ctx *ctx, ToepPlan &P, const float2 *p, float2 *out) {
                                    (long)P.M2 * P.M3, (long)P.M3, P.M3, P.M2, P.N2, nullptr,
                                    tw_x);
             });
+            prof_end(ctx, rec);
             IMP_CHECK_LAUNCH();
         }
         // pass E: inverse last axis, conj(S_c), partial sums over coil groups
+        rec = prof_begin(ctx, "toep_pass_e");
         IMP_L_SWITCH(Ml, {
             const int NL = contig_lines<L_>();
             dim3 grid(ceil_div(nimg_lines, NL), ngroups);
@@ -500,10 +509,13 @@ int toep_apply_normal(imp_ctx *ctx, ToepPlan &P, const float2 *p, float2 *out) {
                                P.sens, nimg_lines, nlast, P.N1, N2o, A, C, cpg, P.partial,
                                tw_last);
         });
+        prof_end(ctx, rec);
         IMP_CHECK_LAUNCH();
         // pass F
+        rec = prof_begin(ctx, "toep_pass_f");
         hipLaunchKernelGGL(k_toep_pass_f, dim3(ceil_div(nvox, 256)), dim3(256), 0, ctx->stream,
                            P.partial, ngroups, nvox, P.fcyc, tau_l, scale, l > 0 ? 1 : 0, out);
+        prof_end(ctx, rec);
         IMP_CHECK_LAUNCH();
     }
     return IMP_OK;

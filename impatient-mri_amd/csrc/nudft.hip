@@ -21,13 +21,17 @@
 //   k_nudft_valu  few right-hand sides: per-lane complex MACs, streamed
 //                 operands read with scalar loads (wave-uniform).
 //   k_nudft_mfma  many right-hand sides: the MACs are a complex GEMM
-//                 E(points x samples) * W(samples x rhs).  E is generated per
-//                 lane directly in the MFMA A-operand layout of
-//                 v_mfma_f32_32x32x16_f16; W is pre-packed as B operands.
-//                 FP32 accuracy from fp16 MFMAs: E and W are split into
-//                 hi + lo fp16 parts (scaled by powers of two to stay in the
-//                 normal range) and E W = Eh Wh + Eh Wl + El Wh (the dropped
+//                 E(points x samples) * W(samples x rhs) on
+//                 v_mfma_f32_32x32x16_f16.  Each lane evaluates one phase
+//                 per sample for its own point and both cos and sin of it;
+//                 v_permlane32_swap turns the two half-waves' values into the
+//                 A operands of two 32-point M tiles.  W is pre-packed as B
+//                 operands.  FP32 accuracy from fp16 MFMAs: E and W are split
+//                 into hi + lo fp16 parts (lo may be subnormal; the MFMA keeps
+//                 fp16 subnormals) and E W = Eh Wh + Eh Wl + El Wh (the dropped
 //                 El Wl term is below 2^-22 relative).
+// Stream positions are stored per step of 8 samples as [x8][y8][w8][z8]
+// doubles, so one step's positions arrive in three or four scalar loads.
 // Sample sums are blocked and folded into compensated (two-sum) totals;
 // chunk partials are reduced in FP64.
 #include "common.h"
@@ -48,8 +52,13 @@ constexpr int NU_BLK = 32;        // VALU: samples per FP32 block before the com
 constexpr int MF_K = 8;           // MFMA: samples per 32x32x16 step (8 Re + 8 Im rows of K)
 constexpr int MF_FOLD = 32;       // MFMA: steps per compensated fold (256 samples)
 constexpr int MF_COLS = 16;       // MFMA: right-hand sides per 32-column tile (Re + Im)
-constexpr float E_SCALE = 4096.0f;  // 2^12: keeps the lo parts of E in fp16 normal range
-constexpr int W_EXP = 12;           // W is scaled so max |W| < 2^12
+constexpr int W_EXP = 12;         // W is scaled so max |W| < 2^12
+constexpr int MF_WAVE_PTS = 64;   // MFMA: points per wave (two 32-point M tiles)
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// doubles per step of MF_K stream positions: x, y, w (+ z in 3D)
+__host__ __device__ constexpr int pos_stride(bool d3) { return (d3 ? 4 : 3) * MF_K; }
 
 struct alignas(16) BPair {
     half8 hi;
@@ -60,12 +69,12 @@ __device__ __forceinline__ float rev_fract(double u) { return (float)__builtin_a
 
 // ---------------------------------------------------------------- VALU kernel
 // Weights stream: [group][n_stream][2*G] floats (Re, Im per rhs); positions
-// double4 [n_stream].  EPI_COMBINE sums conj(S_c) * acc_c over the rhs.
+// in the per-step layout.  EPI_COMBINE sums conj(S_c) * acc_c over the rhs.
 enum { EPI_COMBINE = 0, EPI_PERCOIL = 1 };
 
 template <int G, bool D3, int EPI>
 __global__ __launch_bounds__(NU_THREADS) void k_nudft_valu(
-    const float4 *__restrict__ lanes, int n_lane, const double4 *__restrict__ spos,
+    const float4 *__restrict__ lanes, int n_lane, const double *__restrict__ spos,
     const float *__restrict__ wstream, int n_stream, int chunk_len, float sign, int nrhs,
     const float *__restrict__ s_r, const float *__restrict__ s_i,
     float2 *__restrict__ partial) {
@@ -90,11 +99,11 @@ __global__ __launch_bounds__(NU_THREADS) void k_nudft_valu(
 #pragma unroll
         for (int g = 0; g < G; ++g) { ar[g] = ai[g] = 0.f; }
         for (int j = b; j < be; ++j) {
-            const double4 B = spos[j];
-            double u = B.w * pw;
-            if constexpr (D3) u = __builtin_fma(B.z, pz, u);
-            u = __builtin_fma(B.y, py, u);
-            u = __builtin_fma(B.x, px, u);
+            const double *B = spos + (size_t)(j / MF_K) * pos_stride(D3) + (j % MF_K);
+            double u = B[2 * MF_K] * pw;
+            if constexpr (D3) u = __builtin_fma(B[3 * MF_K], pz, u);
+            u = __builtin_fma(B[MF_K], py, u);
+            u = __builtin_fma(B[0], px, u);
             const float r = rev_fract(u);
             const float cs = __builtin_amdgcn_cosf(r);
             const float sn = sign * __builtin_amdgcn_sinf(r);
@@ -147,108 +156,156 @@ __global__ __launch_bounds__(NU_THREADS) void k_nudft_valu(
 }
 
 // ---------------------------------------------------------------- MFMA kernel
-// One wave owns 32 lane points (one M tile) and TILES 32-column N tiles
-// (16 rhs x {Re, Im} each).  Per step of 8 samples each lane evaluates the
-// 8 phases of its point and keeps cos (lanes 0-31: the Re rows of K) or
-// sin (lanes 32-63: the Im rows, via cos(u - 1/4)); B rows hold the matching
-// W coefficients (packed by k_pack_mfma).  Output planes:
-// partial[chunk][TILES*32][n_lane] (plane = tile*32 + part*16 + rhs%16).
+// One wave owns 64 points (M tiles A = points 0-31, B = points 32-63) and
+// TILES 32-column N tiles (16 rhs x {Re, Im} each).  Per step of 8 samples
+// lane l evaluates the phases of point l and splits cos and sin into fp16
+// hi/lo vectors; v_permlane32_swap(cos, sin) then yields
+// [cos of A | sin of A] and [cos of B | sin of B], i.e. the A operands with
+// K rows (Re|Im, sample) that k_pack_mfma's B operands expect.  B operands
+// of the next step are fetched while the current one is used (bpack holds
+// one extra zero step).  tot stays in registers and the two-sum
+// compensation in LDS.  Output planes: partial[chunk][TILES*32][n_lane]
+// (plane = tile*32 + part*16 + rhs%16).  `one` is 1.0f, passed at run time
+// so the lo split compiles to v_fma_mix.
 template <int TILES, bool D3>
-__global__ __launch_bounds__(NU_THREADS) __attribute__((amdgpu_waves_per_eu(TILES == 1 ? 4 : 3)))
+__global__ __launch_bounds__(NU_THREADS) __attribute__((amdgpu_waves_per_eu(TILES == 1 ? 3 : 2)))
 void k_nudft_mfma(
-    const float4 *__restrict__ lanes, int n_lane, const double4 *__restrict__ spos,
+    const float4 *__restrict__ lanes, int n_lane, const double *__restrict__ spos,
     const BPair *__restrict__ bpack, int n_steps, int steps_per_chunk,
-    float *__restrict__ partial) {
+    float *__restrict__ partial, float one) {
+    constexpr int NA = 2 * TILES;  // accumulators [m tile][n tile]
+    constexpr int WAVES = NU_THREADS / 64;
+    __shared__ float cmp_lds[WAVES][NA * 16][64];
     const int l = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int mt = blockIdx.x * (NU_THREADS / 64) + wave;
-    if (mt * 32 >= n_lane) return;
+    const int wt = blockIdx.x * WAVES + wave;
     const int h = l >> 5;
-    const int a = mt * 32 + (l & 31);
-    const float4 Pf = lanes[min(a, n_lane - 1)];
+    const float4 Pf = lanes[min(wt * MF_WAVE_PTS + l, n_lane - 1)];
     const double px = Pf.x, py = Pf.y, pz = Pf.z, pw = Pf.w;
-    const double quarter = h ? -0.25 : 0.0;   // sin(2 pi u) = cos(2 pi (u - 1/4))
-
     const int chunk = blockIdx.y;
     const int s0 = chunk * steps_per_chunk;
     const int s1 = min(n_steps, s0 + steps_per_chunk);
 
-    floatx16 acc[TILES];
-    float tot[TILES][16], cmp[TILES][16];
+    floatx16 acc[NA];
+    float tot[NA][16];
 #pragma unroll
-    for (int t = 0; t < TILES; ++t) {
+    for (int q = 0; q < NA; ++q) {
 #pragma unroll
-        for (int i = 0; i < 16; ++i) { acc[t][i] = 0.f; tot[t][i] = 0.f; cmp[t][i] = 0.f; }
+        for (int i = 0; i < 16; ++i) {
+            acc[q][i] = 0.f;
+            tot[q][i] = 0.f;
+            cmp_lds[wave][q * 16 + i][l] = 0.f;
+        }
     }
+    BPair nxt[TILES];
+#pragma unroll
+    for (int t = 0; t < TILES; ++t) nxt[t] = bpack[((size_t)min(s0, n_steps) * TILES + t) * 64 + l];
 
-    int since_fold = 0;
-    for (int st = s0; st < s1; ++st) {
-        half8 ahi, alo;
-#pragma unroll
-        for (int j = 0; j < MF_K; ++j) {
-            const double4 B = spos[st * MF_K + j];
-            double u = __builtin_fma(B.w, pw, quarter);
-            if constexpr (D3) u = __builtin_fma(B.z, pz, u);
-            u = __builtin_fma(B.y, py, u);
-            u = __builtin_fma(B.x, px, u);
-            const float v = __builtin_amdgcn_cosf(rev_fract(u)) * E_SCALE;
-            const _Float16 vh = (_Float16)v;
-            ahi[j] = vh;
-            alo[j] = (_Float16)(v - (float)vh);
-        }
-#pragma unroll
-        for (int t = 0; t < TILES; ++t) {
-            const BPair bp = bpack[((size_t)st * TILES + t) * 64 + l];
-            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, bp.hi, acc[t], 0, 0, 0);
-            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ahi, bp.lo, acc[t], 0, 0, 0);
-            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(alo, bp.hi, acc[t], 0, 0, 0);
-        }
-        if (++since_fold == MF_FOLD || st + 1 == s1) {
-            since_fold = 0;
+    for (int f0 = s0; f0 < s1; f0 += MF_FOLD) {
+        const int f1 = min(s1, f0 + MF_FOLD);
+        for (int st = f0; st < f1; ++st) {
+            BPair cur[TILES];
 #pragma unroll
             for (int t = 0; t < TILES; ++t) {
+                cur[t] = nxt[t];
+                nxt[t] = bpack[((size_t)(st + 1) * TILES + t) * 64 + l];
+            }
+            const double *B = spos + (size_t)st * pos_stride(D3);
+            half8 ch, cl, sh, sl;
 #pragma unroll
-                for (int i = 0; i < 16; ++i) {
-                    float s, e;
-                    two_sum(tot[t][i], acc[t][i], s, e);
-                    tot[t][i] = s;
-                    cmp[t][i] += e;
-                    acc[t][i] = 0.f;
-                }
+            for (int j = 0; j < MF_K; ++j) {
+                double u = B[2 * MF_K + j] * pw;
+                if constexpr (D3) u = __builtin_fma(B[3 * MF_K + j], pz, u);
+                u = __builtin_fma(B[MF_K + j], py, u);
+                u = __builtin_fma(B[j], px, u);
+                const float r = rev_fract(u);
+                const float c = __builtin_amdgcn_cosf(r);
+                const float sn = __builtin_amdgcn_sinf(r);
+                const _Float16 c16 = (_Float16)c, s16 = (_Float16)sn;
+                ch[j] = c16;
+                sh[j] = s16;
+                cl[j] = (_Float16)__builtin_fmaf(c, one, -(float)c16);
+                sl[j] = (_Float16)__builtin_fmaf(sn, one, -(float)s16);
+            }
+            const u32x4 chu = __builtin_bit_cast(u32x4, ch), shu = __builtin_bit_cast(u32x4, sh);
+            const u32x4 clu = __builtin_bit_cast(u32x4, cl), slu = __builtin_bit_cast(u32x4, sl);
+            u32x4 hA, hB, lA, lB;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const auto rh = __builtin_amdgcn_permlane32_swap(chu[i], shu[i], false, false);
+                hA[i] = rh[0];
+                hB[i] = rh[1];
+                const auto rl = __builtin_amdgcn_permlane32_swap(clu[i], slu[i], false, false);
+                lA[i] = rl[0];
+                lB[i] = rl[1];
+            }
+            const half8 AhA = __builtin_bit_cast(half8, hA), AhB = __builtin_bit_cast(half8, hB);
+            const half8 AlA = __builtin_bit_cast(half8, lA), AlB = __builtin_bit_cast(half8, lB);
+#pragma unroll
+            for (int t = 0; t < TILES; ++t) {
+                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(AhA, cur[t].hi, acc[t], 0, 0, 0);
+                acc[TILES + t] =
+                    __builtin_amdgcn_mfma_f32_32x32x16_f16(AhB, cur[t].hi, acc[TILES + t], 0, 0, 0);
+            }
+#pragma unroll
+            for (int t = 0; t < TILES; ++t) {
+                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(AhA, cur[t].lo, acc[t], 0, 0, 0);
+                acc[TILES + t] =
+                    __builtin_amdgcn_mfma_f32_32x32x16_f16(AhB, cur[t].lo, acc[TILES + t], 0, 0, 0);
+            }
+#pragma unroll
+            for (int t = 0; t < TILES; ++t) {
+                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(AlA, cur[t].hi, acc[t], 0, 0, 0);
+                acc[TILES + t] =
+                    __builtin_amdgcn_mfma_f32_32x32x16_f16(AlB, cur[t].hi, acc[TILES + t], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < NA; ++q) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                float sum, err;
+                two_sum(tot[q][i], acc[q][i], sum, err);
+                tot[q][i] = sum;
+                cmp_lds[wave][q * 16 + i][l] += err;
+                acc[q][i] = 0.f;
             }
         }
     }
     // D layout of 32x32 MFMA: lane l, register i -> column l%32,
     // row 8*(i/4) + 4*(l/32) + i%4
     const int n = l & 31;
-    const int nplanes = TILES * 32;
+    constexpr int nplanes = TILES * 32;
 #pragma unroll
-    for (int t = 0; t < TILES; ++t) {
-        float *__restrict__ plane = partial + ((size_t)chunk * nplanes + t * 32 + n) * n_lane;
+    for (int m = 0; m < 2; ++m) {
 #pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const int row = 8 * (i >> 2) + 4 * h + (i & 3);
-            const int ap = mt * 32 + row;
-            if (ap < n_lane) plane[ap] = tot[t][i] + cmp[t][i];
+        for (int t = 0; t < TILES; ++t) {
+            const int q = m * TILES + t;
+            float *__restrict__ plane = partial + ((size_t)chunk * nplanes + t * 32 + n) * n_lane;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int ap = wt * MF_WAVE_PTS + m * 32 + 8 * (i >> 2) + 4 * h + (i & 3);
+                if (ap < n_lane) plane[ap] = tot[q][i] + cmp_lds[wave][q * 16 + i][l];
+            }
         }
     }
 }
 
 // ---- packing ---------------------------------------------------------------
 
-// stream positions: {x, y, z, w} as FP64, zero-padded to n_pad
+// stream positions as FP64 in the per-step layout [x8][y8][w8][z8 if 3D],
+// zero-padded to a multiple of MF_K samples
 __global__ void k_spos(int n, int n_pad, const float *x, const float *y, const float *z,
-                       const float *w, double wscale, double4 *out) {
+                       const float *w, double wscale, double *out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_pad) return;
-    double4 v = make_double4(0.0, 0.0, 0.0, 0.0);
-    if (i < n) {
-        v.x = x[i];
-        v.y = y[i];
-        v.z = z ? (double)z[i] : 0.0;
-        v.w = w ? (double)w[i] * wscale : 0.0;
-    }
-    out[i] = v;
+    const bool d3 = z != nullptr;
+    double *o = out + (size_t)(i / MF_K) * pos_stride(d3) + (i % MF_K);
+    const bool in = i < n;
+    o[0] = in ? (double)x[i] : 0.0;
+    o[MF_K] = in ? (double)y[i] : 0.0;
+    o[2 * MF_K] = in && w ? (double)w[i] * wscale : 0.0;
+    if (d3) o[3 * MF_K] = in ? (double)z[i] : 0.0;
 }
 
 __global__ void k_lanes_from_arrays(int n, const float *x, const float *y, const float *z,
@@ -259,17 +316,66 @@ __global__ void k_lanes_from_arrays(int n, const float *x, const float *y, const
                            w ? (float)((double)w[i] * wscale) : 0.f);
 }
 
-// standard Q grid (main_mri.cu:461-500): y outer, x, z inner, r = (x/Nx, y/Ny, z/Nz)
-__global__ void k_lanes_qgrid(int Nx, int Ny, int Nz, float4 *lanes) {
+// Half of the standard Q grid.  Q(-r) = conj(Q(r)) because the Q weights are
+// real, so only rows y <= 0 (grid indices [0, n1)) and the points of rows
+// y > 0 whose mirror falls off the grid (x = -Nx, or z = -Nz in 3D; ne_row
+// per row) are evaluated; k_q_fill mirrors the rest.
+__device__ __forceinline__ void qhalf_edge(int e, int Ny, int M2, int M3, int ne_row, int &iy,
+                                           int &ix, int &iz) {
+    iy = Ny + 1 + e / ne_row;
+    const int r = e % ne_row;
+    if (M3 > 1) {
+        ix = r < M2 ? r : 0;
+        iz = r < M2 ? 0 : r - M2 + 1;
+    } else {
+        ix = 0;
+        iz = 0;
+    }
+}
+
+__global__ void k_lanes_qhalf(int Nx, int Ny, int Nz, int n1, int ne_row, int n_comp,
+                              float4 *lanes) {
     const int M2 = 2 * Nx, M3 = Nz > 1 ? 2 * Nz : 1;
-    const int n = 2 * Ny * M2 * M3;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int iz = i % M3;
-    const int ix = (i / M3) % M2;
-    const int iy = i / (M3 * M2);
-    lanes[i] = make_float4((float)(ix - Nx) / (float)Nx, (float)(iy - Ny) / (float)Ny,
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_comp) return;
+    int iy, ix, iz;
+    if (j < n1) {
+        iz = j % M3;
+        ix = (j / M3) % M2;
+        iy = j / (M3 * M2);
+    } else {
+        qhalf_edge(j - n1, Ny, M2, M3, ne_row, iy, ix, iz);
+    }
+    lanes[j] = make_float4((float)(ix - Nx) / (float)Nx, (float)(iy - Ny) / (float)Ny,
                            Nz > 1 ? (float)(iz - Nz) / (float)Nz : 0.f, 0.f);
+}
+
+// half-grid values [l][n_comp] -> full grid [l][nq]
+__global__ void k_q_fill(int Nx, int Ny, int Nz, int n1, int ne_row, int n_comp, int nseg,
+                         const float *hr, const float *hi, float *q_r, float *q_i) {
+    const int M2 = 2 * Nx, M3 = Nz > 1 ? 2 * Nz : 1;
+    const long nq = 2L * Ny * M2 * M3;
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nq) return;
+    const int iz = (int)(g % M3);
+    const int ix = (int)((g / M3) % M2);
+    const int iy = (int)(g / ((long)M3 * M2));
+    long src;
+    float sgn = 1.f;
+    if (iy <= Ny) {
+        src = g;
+    } else if (ix == 0 || (M3 > 1 && iz == 0)) {
+        const int r = M3 > 1 ? (iz == 0 ? ix : M2 + iz - 1) : 0;
+        src = n1 + (long)(iy - Ny - 1) * ne_row + r;
+    } else {
+        const int my = 2 * Ny - iy, mx = 2 * Nx - ix, mz = M3 > 1 ? 2 * Nz - iz : 0;
+        src = ((long)my * M2 + mx) * M3 + mz;
+        sgn = -1.f;
+    }
+    for (int l = 0; l < nseg; ++l) {
+        q_r[(size_t)l * nq + g] = hr[(size_t)l * n_comp + src];
+        q_i[(size_t)l * nq + g] = sgn * hi[(size_t)l * n_comp + src];
+    }
 }
 
 // Weights W[c][b] (complex, rhs-major) for the three operators.
@@ -430,14 +536,13 @@ __global__ void k_reduce_percoil(const float2 *partial, int nchunks, int nrhs, i
 }
 
 // MFMA planes [chunk][tiles*32][n] -> combine with conj(S_c) (mode 0),
-// per-rhs output (mode 1), per-rhs prefix sums (mode 2); undoes the
-// E and W scaling.
+// per-rhs output (mode 1), per-rhs prefix sums (mode 2); undoes the W scaling.
 __global__ void k_reduce_mfma(const float *partial, int nchunks, int tiles, int nrhs, int n,
                               int mode, const unsigned int *maxbits, const float *s_r,
                               const float *s_i, float *out_r, float *out_i) {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= n) return;
-    const double unscale = ldexp(1.0, -w_shift(*maxbits)) / (double)E_SCALE;
+    const double unscale = ldexp(1.0, -w_shift(*maxbits));
     const int nplanes = tiles * 32;
     double orr = 0.0, oi = 0.0, cr = 0.0, ci = 0.0;
     for (int c = 0; c < nrhs; ++c) {
@@ -477,7 +582,7 @@ __global__ void k_reduce_mfma(const float *partial, int nchunks, int tiles, int 
 
 // ---- dispatch ----------------------------------------------------------------
 
-using VFn = void (*)(const float4 *, int, const double4 *, const float *, int, int, float, int,
+using VFn = void (*)(const float4 *, int, const double *, const float *, int, int, float, int,
                      const float *, const float *, float2 *);
 
 template <int G, int EPI>
@@ -494,7 +599,7 @@ VFn pick_valu(int G, bool d3) {
     }
 }
 
-using MFn = void (*)(const float4 *, int, const double4 *, const BPair *, int, int, float *);
+using MFn = void (*)(const float4 *, int, const double *, const BPair *, int, int, float *, float);
 
 MFn pick_mfma(int tiles, bool d3) {
     if (tiles == 1) return d3 ? (MFn)k_nudft_mfma<1, true> : (MFn)k_nudft_mfma<1, false>;
@@ -518,7 +623,7 @@ enum class Out { Combine, PerRhs, PerRhsCumulative };
 struct Job {
     const float4 *lanes;
     int n_lane;
-    const double4 *spos;     // padded to a multiple of MF_K
+    const double *spos;      // per-step layout, padded to a multiple of MF_K
     int n_stream;
     const float2 *W;         // [nrhs][n_stream]
     int nrhs;
@@ -541,26 +646,28 @@ int run_job(imp_ctx *ctx, const Job &J) {
         // max |W| for the power-of-two scale
         unsigned int *maxbits = nullptr;
         void *mb = nullptr, *bp = nullptr, *part = nullptr;
-        IMP_TRY(workspace(ctx, 5, 256, &mb));
+        IMP_TRY(workspace(ctx, WS_MAXBITS, 256, &mb));
         maxbits = (unsigned int *)mb;
         IMP_HIP(hipMemsetAsync(maxbits, 0, sizeof(unsigned int), st));
         hipLaunchKernelGGL(k_absmax, dim3(512), dim3(256), 0, st, J.W,
                            (size_t)J.nrhs * J.n_stream, maxbits);
         IMP_CHECK_LAUNCH();
-        const size_t nb = (size_t)n_steps * tiles * 64;
-        IMP_TRY(workspace(ctx, 3, nb * sizeof(BPair), &bp));
+        // one extra zero step: the kernel prefetches step st + 1
+        const size_t nb = (size_t)(n_steps + 1) * tiles * 64;
+        IMP_TRY(workspace(ctx, WS_BPACK, nb * sizeof(BPair), &bp));
         hipLaunchKernelGGL(k_pack_mfma, dim3(ceil_div((long)nb, 256)), dim3(256), 0, st,
-                           J.n_stream, J.nrhs, tiles, n_steps, J.sign, J.W, maxbits,
+                           J.n_stream, J.nrhs, tiles, n_steps + 1, J.sign, J.W, maxbits,
                            (BPair *)bp);
         IMP_CHECK_LAUNCH();
-        const int mtiles = ceil_div(n, 32);
-        const long wgs = ceil_div(mtiles, NU_THREADS / 64);
+        const long wgs = ceil_div(n, MF_WAVE_PTS * (NU_THREADS / 64));
         const int chunks = pick_chunks(ctx, wgs, n_steps, 4 * MF_FOLD);
         const int spc = ceil_div(n_steps, chunks);
-        IMP_TRY(workspace(ctx, 2, (size_t)chunks * tiles * 32 * n * sizeof(float), &part));
+        IMP_TRY(workspace(ctx, WS_PARTIAL, (size_t)chunks * tiles * 32 * n * sizeof(float), &part));
+        const int rec = prof_begin(ctx, "k_nudft_mfma");
         hipLaunchKernelGGL(pick_mfma(tiles, J.d3), dim3((unsigned)wgs, chunks), dim3(NU_THREADS),
                            0, st, J.lanes, n, J.spos, (const BPair *)bp, n_steps, spc,
-                           (float *)part);
+                           (float *)part, 1.0f);
+        prof_end(ctx, rec);
         IMP_CHECK_LAUNCH();
         const int mode = J.out == Out::Combine ? 0 : (J.out == Out::PerRhs ? 1 : 2);
         hipLaunchKernelGGL(k_reduce_mfma, dim3(ceil_div(n, 256)), dim3(256), 0, st,
@@ -573,7 +680,7 @@ int run_job(imp_ctx *ctx, const Job &J) {
     const int G = J.nrhs <= 1 ? 1 : (J.nrhs <= 2 ? 2 : 4);
     const int groups = ceil_div(J.nrhs, G);
     void *ws = nullptr, *part = nullptr;
-    IMP_TRY(workspace(ctx, 3, (size_t)groups * J.n_stream * 2 * G * sizeof(float), &ws));
+    IMP_TRY(workspace(ctx, WS_BPACK, (size_t)groups * J.n_stream * 2 * G * sizeof(float), &ws));
     hipLaunchKernelGGL(k_pack_valu, dim3(ceil_div(J.n_stream, 256), groups), dim3(256), 0, st,
                        J.n_stream, J.nrhs, G, J.W, (float *)ws);
     IMP_CHECK_LAUNCH();
@@ -582,11 +689,13 @@ int run_job(imp_ctx *ctx, const Job &J) {
     int chunk_len = ceil_div(ceil_div(J.n_stream, chunks), NU_BLK) * NU_BLK;
     const bool combine = J.out == Out::Combine;
     const size_t nparts = combine ? (size_t)groups * chunks : (size_t)chunks * J.nrhs;
-    IMP_TRY(workspace(ctx, 2, nparts * n * sizeof(float2), &part));
+    IMP_TRY(workspace(ctx, WS_PARTIAL, nparts * n * sizeof(float2), &part));
     VFn fn = combine ? pick_valu<EPI_COMBINE>(G, J.d3) : pick_valu<EPI_PERCOIL>(G, J.d3);
+    const int rec = prof_begin(ctx, "k_nudft_valu");
     hipLaunchKernelGGL(fn, dim3(lane_blocks, chunks, groups), dim3(NU_THREADS), 0, st, J.lanes,
                        n, J.spos, (const float *)ws, J.n_stream, chunk_len, J.sign, J.nrhs,
                        J.s_r, J.s_i, (float2 *)part);
+    prof_end(ctx, rec);
     IMP_CHECK_LAUNCH();
     if (combine) {
         hipLaunchKernelGGL(k_reduce_combine, dim3(ceil_div(n, 256)), dim3(256), 0, st,
@@ -601,14 +710,15 @@ int run_job(imp_ctx *ctx, const Job &J) {
 }
 
 int build_spos(imp_ctx *ctx, int n, const float *x, const float *y, const float *z,
-               const float *w, double wscale, const double4 **out) {
+               const float *w, double wscale, const double **out) {
     void *p = nullptr;
     const int n_pad = ceil_div(n, MF_K) * MF_K;
-    IMP_TRY(workspace(ctx, 0, (size_t)n_pad * sizeof(double4), &p));
+    const size_t doubles = (size_t)(n_pad / MF_K) * pos_stride(z != nullptr);
+    IMP_TRY(workspace(ctx, WS_SPOS, doubles * sizeof(double), &p));
     hipLaunchKernelGGL(k_spos, dim3(ceil_div(n_pad, 256)), dim3(256), 0, ctx->stream, n, n_pad,
-                       x, y, z, w, wscale, (double4 *)p);
+                       x, y, z, w, wscale, (double *)p);
     IMP_CHECK_LAUNCH();
-    *out = (const double4 *)p;
+    *out = (const double *)p;
     return IMP_OK;
 }
 
@@ -627,15 +737,15 @@ int nudft_adjoint(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *k
         (phi_r && !phi_i))
         return fail(IMP_EINVAL, "nudft_adjoint: bad arguments");
     const bool d3 = kz && z;
-    const double4 *spos = nullptr;
+    const double *spos = nullptr;
     IMP_TRY(build_spos(ctx, num_k, kx, ky, d3 ? kz : nullptr, t, 1.0, &spos));
     void *lanes = nullptr, *W = nullptr;
-    IMP_TRY(workspace(ctx, 1, (size_t)num_x * sizeof(float4), &lanes));
+    IMP_TRY(workspace(ctx, WS_LANES, (size_t)num_x * sizeof(float4), &lanes));
     hipLaunchKernelGGL(k_lanes_from_arrays, dim3(ceil_div(num_x, 256)), dim3(256), 0,
                        ctx->stream, num_x, x, y, d3 ? z : nullptr, fm, 1.0 / (2.0 * M_PI),
                        (float4 *)lanes);
     IMP_CHECK_LAUNCH();
-    IMP_TRY(workspace(ctx, 4, (size_t)num_k * ncoils * sizeof(float2), &W));
+    IMP_TRY(workspace(ctx, WS_W, (size_t)num_k * ncoils * sizeof(float2), &W));
     hipLaunchKernelGGL(k_w_adjoint, dim3(ceil_div((long)num_k * ncoils, 256)), dim3(256), 0,
                        ctx->stream, num_k, ncoils, phi_r, phi_i, d_r, d_i, (float2 *)W);
     IMP_CHECK_LAUNCH();
@@ -653,14 +763,14 @@ int nudft_forward(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *k
         !x || !y || !rho_r || !rho_i || !d_r || !d_i || (s_r && !s_i))
         return fail(IMP_EINVAL, "nudft_forward: bad arguments");
     const bool d3 = kz && z;
-    const double4 *spos = nullptr;
+    const double *spos = nullptr;
     IMP_TRY(build_spos(ctx, num_x, x, y, d3 ? z : nullptr, fm, 1.0 / (2.0 * M_PI), &spos));
     void *lanes = nullptr, *W = nullptr;
-    IMP_TRY(workspace(ctx, 1, (size_t)num_k * sizeof(float4), &lanes));
+    IMP_TRY(workspace(ctx, WS_LANES, (size_t)num_k * sizeof(float4), &lanes));
     hipLaunchKernelGGL(k_lanes_from_arrays, dim3(ceil_div(num_k, 256)), dim3(256), 0,
                        ctx->stream, num_k, kx, ky, d3 ? kz : nullptr, t, 1.0, (float4 *)lanes);
     IMP_CHECK_LAUNCH();
-    IMP_TRY(workspace(ctx, 4, (size_t)num_x * ncoils * sizeof(float2), &W));
+    IMP_TRY(workspace(ctx, WS_W, (size_t)num_x * ncoils * sizeof(float2), &W));
     hipLaunchKernelGGL(k_w_forward, dim3(ceil_div((long)num_x * ncoils, 256)), dim3(256), 0,
                        ctx->stream, num_x, ncoils, rho_r, rho_i, s_r, s_i, (float2 *)W);
     IMP_CHECK_LAUNCH();
@@ -688,6 +798,8 @@ int toeplitz_q(imp_ctx *ctx, int num_k, const float *kx, const float *ky,
     } else if (num_q <= 0) {
         return fail(IMP_EINVAL, "toeplitz_q: num_q must be positive");
     }
+    if (std_grid && Nz > 1 && !kz)
+        return fail(IMP_EINVAL, "toeplitz_q: a 3D grid (Nz = %d) needs kz", Nz);
     const bool d3 = kz && (std_grid ? Nz > 1 : gz != nullptr);
     const int nseg = ntime_segs + 1;
     if (nseg > 32) return fail(IMP_EUNSUPPORTED, "toeplitz_q: at most 31 time segments");
@@ -696,27 +808,46 @@ int toeplitz_q(imp_ctx *ctx, int num_k, const float *kx, const float *ky,
     const float t0 = t_host_first_last[0];
     const float tlast = t_host_first_last[1];
     const float tau = ntime_segs > 0 ? (tlast - t0) / (float)ntime_segs : (tlast - t0);
-    const double4 *spos = nullptr;
+    const double *spos = nullptr;
     IMP_TRY(build_spos(ctx, num_k, kx, ky, d3 ? kz : nullptr, nullptr, 0.0, &spos));
+    // Standard grid (main_mri.cu:461-500; y outer, x, z inner, r = (x/Nx,
+    // y/Ny, z/Nz)): evaluate the half grid and fill by Hermitian symmetry.
+    const int M2 = 2 * Nx, M3 = Nz > 1 ? 2 * Nz : 1;
+    const int n1 = std_grid ? (Ny + 1) * M2 * M3 : 0;
+    const int ne_row = M3 > 1 ? M2 + M3 - 1 : 1;
+    const int n_comp = std_grid ? n1 + (Ny - 1) * ne_row : num_q;
     void *lanes = nullptr, *W = nullptr;
-    IMP_TRY(workspace(ctx, 1, (size_t)num_q * sizeof(float4), &lanes));
+    IMP_TRY(workspace(ctx, WS_LANES, (size_t)n_comp * sizeof(float4), &lanes));
     if (std_grid) {
-        hipLaunchKernelGGL(k_lanes_qgrid, dim3(ceil_div(num_q, 256)), dim3(256), 0, ctx->stream,
-                           Nx, Ny, d3 ? Nz : 1, (float4 *)lanes);
+        hipLaunchKernelGGL(k_lanes_qhalf, dim3(ceil_div(n_comp, 256)), dim3(256), 0, ctx->stream,
+                           Nx, Ny, Nz, n1, ne_row, n_comp, (float4 *)lanes);
     } else {
         hipLaunchKernelGGL(k_lanes_from_arrays, dim3(ceil_div(num_q, 256)), dim3(256), 0,
                            ctx->stream, num_q, gx, gy, d3 ? gz : nullptr, nullptr, 0.0,
                            (float4 *)lanes);
     }
     IMP_CHECK_LAUNCH();
-    IMP_TRY(workspace(ctx, 4, (size_t)num_k * nseg * sizeof(float2), &W));
+    IMP_TRY(workspace(ctx, WS_W, (size_t)num_k * nseg * sizeof(float2), &W));
     hipLaunchKernelGGL(k_w_q, dim3(ceil_div((long)num_k * nseg, 256)), dim3(256), 0,
                        ctx->stream, num_k, nseg, t, phi_r, phi_i, tau, t0, (float2 *)W);
     IMP_CHECK_LAUNCH();
-    Job J{(const float4 *)lanes, num_q, spos, num_k, (const float2 *)W, nseg, d3, +1.f,
-          (flags & IMP_Q_CUMULATIVE) ? Out::PerRhsCumulative : Out::PerRhs, nullptr, nullptr,
-          q_r, q_i};
-    return run_job(ctx, J);
+    const Out mode = (flags & IMP_Q_CUMULATIVE) ? Out::PerRhsCumulative : Out::PerRhs;
+    if (!std_grid) {
+        Job J{(const float4 *)lanes, num_q, spos, num_k, (const float2 *)W, nseg, d3, +1.f,
+              mode, nullptr, nullptr, q_r, q_i};
+        return run_job(ctx, J);
+    }
+    void *half = nullptr;
+    IMP_TRY(workspace(ctx, WS_QHALF, 2 * (size_t)nseg * n_comp * sizeof(float), &half));
+    float *h_r = (float *)half, *h_i = h_r + (size_t)nseg * n_comp;
+    Job J{(const float4 *)lanes, n_comp, spos, num_k, (const float2 *)W, nseg, d3, +1.f,
+          mode, nullptr, nullptr, h_r, h_i};
+    IMP_TRY(run_job(ctx, J));
+    hipLaunchKernelGGL(k_q_fill, dim3(ceil_div(num_q, 256)), dim3(256), 0, ctx->stream, Nx, Ny,
+                       Nz, n1, ne_row, n_comp, nseg, (const float *)h_r, (const float *)h_i, q_r,
+                       q_i);
+    IMP_CHECK_LAUNCH();
+    return IMP_OK;
 }
 
 }  // namespace imp

@@ -65,6 +65,15 @@ int imp_ctx_destroy(imp_ctx *ctx);
 int imp_ctx_stream(imp_ctx *ctx, void **hip_stream);
 int imp_ctx_synchronize(imp_ctx *ctx);
 
+/* Kernel timer: when enabled, the context records HIP events on its stream
+ * around its main kernels.  imp_profile_read sums the recorded durations of
+ * one kernel name ("k_nudft_mfma", "k_nudft_valu", "toeplitz_apply", ...)
+ * since the last reset.  Stands in for the reference's mriTimer singleton
+ * (xcpplib timers in main_mri.cu / recon.cu), per context instead of global. */
+int imp_profile_enable(imp_ctx *ctx, int on);
+int imp_profile_reset(imp_ctx *ctx);
+int imp_profile_read(imp_ctx *ctx, const char *kernel, double *total_ms, int *launches);
+
 /* Device-memory helpers for callers without an allocator of their own. */
 int imp_device_alloc(imp_ctx *ctx, size_t bytes, void **dptr);
 int imp_device_free(imp_ctx *ctx, void *dptr);

@@ -92,6 +92,37 @@ def test_q_matches_oracle(ctx, N, Nz, L):
     assert err < OP_TOL
 
 
+@pytest.mark.parametrize("Nx,Ny,Nz", [(6, 10, 1), (5, 3, 3), (4, 1, 2)])
+def test_q_nonsquare_grid_and_hermitian_fill(ctx, Nx, Ny, Nz):
+    """The standard grid is evaluated on half the points and mirrored:
+    check every point against the FP64 oracle on non-square grids (mirror
+    edges x = -Nx, z = -Nz; Ny = 1 has no mirrored row) and that the mirror
+    pairs are exact conjugates."""
+    from impatient import ops
+    from oracle import oracle as orc
+    rng = np.random.default_rng(5)
+    M = 300
+    kx = rng.uniform(-Nx / 2, Nx / 2, M).astype(np.float32)
+    ky = rng.uniform(-Ny / 2, Ny / 2, M).astype(np.float32)
+    kz = rng.uniform(-Nz / 2, Nz / 2, M).astype(np.float32) if Nz > 1 else None
+    t = np.linspace(0, 4e-3, M, dtype=np.float32)
+    L = 3
+    got = ops.toeplitz_q(ctx, kx, ky, kz, t, Nx, Ny, Nz, L)
+    ref = orc.toeplitz_q(kx, ky, kz, t, Nx, Ny, Nz, L)
+    err = rel_l2(got, ref)
+    print(f"Q {Nx}x{Ny}x{Nz}: rel_l2={err:.2e}")
+    assert err < OP_TOL
+    # rows y > 0 with a mirror on the grid are filled from rows y < 0
+    M3 = 2 * Nz if Nz > 1 else 1
+    g = got.reshape(L + 1, 2 * Ny, 2 * Nx, M3)
+    z0 = 1 if M3 > 1 else 0
+    for iy in range(Ny + 1, 2 * Ny):
+        for ix in range(1, 2 * Nx):
+            for iz in range(z0, M3):
+                mz = 2 * Nz - iz if M3 > 1 else 0
+                assert np.array_equal(g[:, iy, ix, iz], np.conj(g[:, 2 * Ny - iy, 2 * Nx - ix, mz]))
+
+
 def test_q_cumulative_compat(ctx):
     """IMP_Q_CUMULATIVE returns the prefix sums over l (reference direct path)."""
     from impatient import ops, IMP_Q_CUMULATIVE
