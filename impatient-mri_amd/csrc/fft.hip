@@ -16,6 +16,7 @@
 //   pass F  out (+)= e^{+iw tau_l} * sum of coil-group partials
 // Each pass reads and writes its lines once; Qf_l stays in HBM.
 #include "common.h"
+#include "fftcore.h"
 #include "toeplitz.h"
 
 #include <cmath>
@@ -25,268 +26,224 @@ namespace imp {
 
 namespace {
 
-constexpr int FT = 256;  // threads per FFT workgroup
+using fftc::cmul;
 
-__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
-    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
-// multiply by -i (forward) or +i (inverse)
-template <bool INV>
-__device__ __forceinline__ float2 rot90(float2 a) {
-    return INV ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
-}
-
-template <int R, bool INV>
-__device__ __forceinline__ void dft(float2 *v) {
-    if constexpr (R == 2) {
-        const float2 a = v[0], b = v[1];
-        v[0] = cadd(a, b);
-        v[1] = csub(a, b);
-    } else {  // R == 4
-        const float2 s02 = cadd(v[0], v[2]), d02 = csub(v[0], v[2]);
-        const float2 s13 = cadd(v[1], v[3]), d13 = rot90<INV>(csub(v[1], v[3]));
-        v[0] = cadd(s02, s13);
-        v[2] = csub(s02, s13);
-        v[1] = cadd(d02, d13);
-        v[3] = csub(d02, d13);
-    }
-}
-
-constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
-
-// One Stockham radix-R stage over NL lines of length L held in LDS (line
-// stride LS), register staged: read all, barrier, write all, barrier.
-// tw[m] = exp(-2 pi i m / L).
-template <int L, int NL, int R, bool INV>
-__device__ __forceinline__ void stockham_stage(float2 *buf, int LS, int Ns,
-                                               const float2 *__restrict__ tw) {
-    constexpr int Q = L / R;
-    constexpr int NB = NL * Q / FT;
-    static_assert(NB >= 1 && NL * Q % FT == 0, "stage too small for the workgroup");
-    float2 v[NB][R];
-    const int tid = threadIdx.x;
-#pragma unroll
-    for (int b = 0; b < NB; ++b) {
-        const int bf = tid + b * FT;
-        const int line = bf / Q, j = bf % Q;
-        const float2 *x = buf + line * LS;
-#pragma unroll
-        for (int r = 0; r < R; ++r) v[b][r] = x[j + r * Q];
-        const int k = j & (Ns - 1);
-        const int tstep = (L / (Ns * R)) * k;
-#pragma unroll
-        for (int r = 1; r < R; ++r) {
-            float2 w = tw[r * tstep];
-            if (INV) w.y = -w.y;
-            v[b][r] = cmul(v[b][r], w);
+// Runs all stages of one transform.  FIRST / LAST are the first stage's
+// loads and the last stage's stores (global memory or registers); FMID runs
+// before the first stage's stores (a barrier when the LDS lines may still be
+// read by an earlier transform); LLD / LST address the lines in LDS.
+// Every thread of the workgroup must call this (it holds barriers).
+template <int L, bool INV, class FIRST, class FMID, class LAST, class LLD, class LST>
+__device__ __forceinline__ void run_fft(int tl, const float2 *__restrict__ tw, FIRST first,
+                                        FMID fmid, LAST last, LLD lld, LST lst) {
+    constexpr int S = fftc::num_stages(L);
+    static_assert(S >= 1 && S <= 3, "lengths up to 4096");
+    auto nop = [] {};
+    auto bar = [] { __syncthreads(); };
+    if constexpr (S == 1) {
+        fftc::stage<L, 0, INV, true>(tl, tw, first, fmid, last);
+    } else {
+        fftc::stage<L, 0, INV, true>(tl, tw, first, fmid, lst);
+        __syncthreads();
+        if constexpr (S == 3) {
+            fftc::stage<L, 1, INV, false>(tl, tw, lld, bar, lst);
+            __syncthreads();
         }
-        dft<R, INV>(v[b]);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int b = 0; b < NB; ++b) {
-        const int bf = tid + b * FT;
-        const int line = bf / Q, j = bf % Q;
-        const int k = j & (Ns - 1);
-        float2 *y = buf + line * LS + (j - k) * R + k;
-#pragma unroll
-        for (int r = 0; r < R; ++r) y[r * Ns] = v[b][r];
-    }
-    __syncthreads();
-}
-
-// Full FFT of NL lines in LDS.  Caller has synchronised after filling buf.
-template <int L, int NL, bool INV>
-__device__ __forceinline__ void fft_lds(float2 *buf, int LS, const float2 *__restrict__ tw) {
-    constexpr int LOG = ilog2(L);
-    int Ns = 1;
-    if constexpr (LOG & 1) {
-        stockham_stage<L, NL, 2, INV>(buf, LS, Ns, tw);
-        Ns = 2;
-    }
-    if constexpr (LOG >= 2) {
-#pragma unroll
-        for (int s = 0; s < LOG / 2; ++s) {
-            stockham_stage<L, NL, 4, INV>(buf, LS, Ns, tw);
-            Ns *= 4;
-        }
+        fftc::stage<L, S - 1, INV, true>(tl, tw, lld, nop, last);
     }
 }
 
+// ---------------------------------------------------------------- rows
+// Contiguous lines: T = L/16 consecutive threads per line, so each global
+// load or store of a stage covers T consecutive elements.
+constexpr int ROW_THREADS = 256;
 template <int L>
-constexpr int contig_lines() { return (4 * FT) / L > 1 ? (4 * FT) / L : 1; }
-
+constexpr int row_lines() { return ROW_THREADS / fftc::threads_per_line(L); }
 template <int L>
-constexpr int panel_lines() {
-    return L >= 1024 ? 8 : ((4 * FT) / L > 16 ? (4 * FT) / L : 16);
-}
+constexpr int row_lds() { return fftc::num_stages(L) > 1 ? row_lines<L>() * fftc::line_lds(L) : 1; }
 
-// ---------------------------------------------------------------- generic
 // In-place FFT of `nlines` contiguous lines of length L.
 template <int L, bool INV>
-__global__ __launch_bounds__(FT) void k_fft_contig(float2 *data, long nlines,
-                                                   const float2 *__restrict__ tw) {
-    constexpr int NL = contig_lines<L>();
-    __shared__ float2 buf[NL * L];
-    const long line0 = (long)blockIdx.x * NL;
-    for (int e = threadIdx.x; e < NL * L; e += FT) {
-        const long line = line0 + e / L;
-        buf[e] = line < nlines ? data[line * L + (e % L)] : make_float2(0.f, 0.f);
-    }
-    __syncthreads();
-    fft_lds<L, NL, INV>(buf, L, tw);
-    for (int e = threadIdx.x; e < NL * L; e += FT) {
-        const long line = line0 + e / L;
-        if (line < nlines) data[line * L + (e % L)] = buf[e];
-    }
+__global__ __launch_bounds__(ROW_THREADS) void k_fft_rows(float2 *data, long nlines,
+                                                         const float2 *__restrict__ tw) {
+    constexpr int T = fftc::threads_per_line(L);
+    __shared__ float2 buf[row_lds<L>()];
+    const int tl = threadIdx.x % T, ln = threadIdx.x / T;
+    const long line = (long)blockIdx.x * row_lines<L>() + ln;
+    const bool ok = line < nlines;
+    float2 *src = data + (ok ? line : 0) * L;
+    float2 *lb = buf + (fftc::num_stages(L) > 1 ? ln * fftc::line_lds(L) : 0);
+    run_fft<L, INV>(
+        tl, tw, [&](int, int, int i) { return ok ? src[i] : make_float2(0.f, 0.f); }, [] {},
+        [&](int, int, int i, float2 v) {
+            if (ok) src[i] = v;
+        },
+        [&](int, int, int i) { return lb[fftc::lpad(i)]; },
+        [&](int, int, int i, float2 v) { lb[fftc::lpad(i)] = v; });
 }
 
-// Strided lines: line j of batch b starts at b*bstride + j, element i at
-// + i*es.  Input elements i >= n_in are zero; outputs i < n_out are stored.
-// MODE 0: forward, 1: inverse, 2: forward, multiply by qf (same addressing
-// with batch stride 0), inverse.
-template <int L, int MODE>
-__global__ __launch_bounds__(FT) void k_fft_panel(float2 *data, long bstride, long es,
-                                                  int nlines, int n_in, int n_out,
-                                                  const float2 *__restrict__ qf,
-                                                  const float2 *__restrict__ tw) {
-    constexpr int P = panel_lines<L>();
-    constexpr int LS = L + 1;
-    __shared__ float2 buf[P * LS];
-    const int j0 = blockIdx.x * P;
-    float2 *base = data + (long)blockIdx.y * bstride;
-    for (int e = threadIdx.x; e < P * L; e += FT) {
-        const int i = e / P, j = e % P;
-        float2 v = make_float2(0.f, 0.f);
-        if (i < n_in && j0 + j < nlines) v = base[i * es + j0 + j];
-        buf[j * LS + i] = v;
-    }
-    __syncthreads();
-    fft_lds<L, P, MODE == 1>(buf, LS, tw);
-    if constexpr (MODE == 2) {
-        for (int e = threadIdx.x; e < P * L; e += FT) {
-            const int i = e / P, j = e % P;
-            if (j0 + j < nlines) buf[j * LS + i] = cmul(buf[j * LS + i], qf[i * es + j0 + j]);
-        }
-        __syncthreads();
-        fft_lds<L, P, true>(buf, LS, tw);
-    }
-    for (int e = threadIdx.x; e < P * L; e += FT) {
-        const int i = e / P, j = e % P;
-        if (i < n_out && j0 + j < nlines) base[i * es + j0 + j] = buf[j * LS + i];
+// ---------------------------------------------------------------- columns
+// Strided lines (element i of line j at i*es + j): P adjacent lines per
+// workgroup, thread = (line-local index, line) with the line fastest, so each
+// global access covers P consecutive complex values.  LDS holds [i][line]
+// with a row stride of P + 1.
+template <int L>
+constexpr int col_lines() {
+    return fftc::threads_per_line(L) * 8 <= 512 ? (fftc::threads_per_line(L) * 16 <= 256 ? 16 : 8)
+                                                : 512 / fftc::threads_per_line(L);
+}
+template <int L>
+constexpr int col_threads() { return col_lines<L>() * fftc::threads_per_line(L); }
+
+// MODE 0: forward, 1: inverse, 2: forward, times qf (same addressing with
+// batch stride 0), inverse.  HALF_IN: inputs i >= L/2 are zero (the
+// zero-padded axis of the 2x grid); HALF_OUT: only outputs i < L/2 are
+// stored (the cropped axis).  blockIdx.y selects the batch (bstride).
+template <int L, int MODE, bool HALF_IN, bool HALF_OUT>
+__global__ __launch_bounds__(col_threads<L>()) void k_fft_cols(
+    float2 *data, long bstride, long es, int nlines, const float2 *__restrict__ qf,
+    const float2 *__restrict__ tw) {
+    constexpr int P = col_lines<L>(), PS = P + 1;
+    constexpr int E = fftc::vals_per_thread(L);
+    __shared__ float2 buf[fftc::num_stages(L) > 1 ? L * PS : 1];
+    const int c = threadIdx.x % P, tl = threadIdx.x / P;
+    // Workgroups are dispatched round-robin over the 8 XCDs: blocks b and
+    // b + 8 of each group of 16 take adjacent panels, so the two halves of a
+    // 128-byte row segment are fetched through the same L2.
+    const int bx = blockIdx.x;
+    const int panel = (gridDim.x & 15) == 0 ? (bx & ~15) | ((bx & 7) << 1) | ((bx >> 3) & 1) : bx;
+    const int j = panel * P + c;
+    const bool ok = j < nlines;
+    float2 *base = data + (long)blockIdx.y * bstride + (ok ? j : 0);
+    auto lld = [&](int, int, int i) { return buf[i * PS + c]; };
+    auto lst = [&](int, int, int i, float2 v) { buf[i * PS + c] = v; };
+    // element i = tl + T m: i < L/2 exactly when m < E/2 (known per register)
+    auto gld = [&](int b, int r, int i) {
+        const bool in = !HALF_IN || fftc::first_reg<L>(b, r) < E / 2;
+        return (ok && in) ? base[(long)i * es] : make_float2(0.f, 0.f);
+    };
+    auto gst = [&](int b, int r, int i, float2 v) {
+        const bool in = !HALF_OUT || fftc::last_reg<L>(b, r) < E / 2;
+        if (ok && in) base[(long)i * es] = v;
+    };
+    if constexpr (MODE == 0) {
+        run_fft<L, false>(tl, tw, gld, [] {}, gst, lld, lst);
+    } else if constexpr (MODE == 1) {
+        run_fft<L, true>(tl, tw, gld, [] {}, gst, lld, lst);
+    } else {
+        float2 regs[E];
+        const float2 *q = qf + (ok ? j : 0);
+        run_fft<L, false>(
+            tl, tw, gld, [] {},
+            [&](int b, int r, int i, float2 v) {
+                regs[fftc::last_reg<L>(b, r)] = ok ? cmul(v, q[(long)i * es]) : v;
+            },
+            lld, lst);
+        run_fft<L, true>(
+            tl, tw, [&](int b, int r, int) { return regs[fftc::first_reg<L>(b, r)]; },
+            [] { __syncthreads(); }, gst, lld, lst);
     }
 }
 
 // ---------------------------------------------------------------- pass A
-// Lines (c, o): image line o (N1*N2o lines of nlast elements, contiguous),
-// v = S_c * e^{-i 2pi f tau_l} * p, zero-padded to L, forward FFT, written
-// to G1 line ((c*N1 + y)*A + x)*L with y = o / N2o, x = o % N2o.
+// Lines (c, o): image line o (N1*N2o lines of nlast = L/2 elements,
+// contiguous), v = S_c * e^{-i 2pi f tau_l} * p, zero-padded to L, forward
+// FFT, written to G1 line ((c*N1 + y)*A + x)*L with y = o / N2o, x = o % N2o.
 template <int L>
-__global__ __launch_bounds__(FT) void k_toep_pass_a(
+__global__ __launch_bounds__(ROW_THREADS) void k_toep_pass_a(
     const float2 *__restrict__ p, const float2 *__restrict__ sens,
-    const float *__restrict__ f, float tau_l, int nimg_lines, int nlast, int N1,
-    int N2o, int A, int ncoils, float2 *__restrict__ g1, const float2 *__restrict__ tw) {
-    constexpr int NL = contig_lines<L>();
-    __shared__ float2 buf[NL * L];
+    const float *__restrict__ f, float tau_l, int nimg_lines, int N1, int N2o, int A,
+    int ncoils, float2 *__restrict__ g1, const float2 *__restrict__ tw) {
+    constexpr int T = fftc::threads_per_line(L);
+    constexpr int E = fftc::vals_per_thread(L);
+    constexpr int nlast = L / 2;
+    __shared__ float2 buf[row_lds<L>()];
+    const int tl = threadIdx.x % T, ln = threadIdx.x / T;
     const long nvox = (long)nimg_lines * nlast;
-    const long line0 = (long)blockIdx.x * NL;  // flattened (c, o)
-    const long nlines = (long)ncoils * nimg_lines;
-    for (int e = threadIdx.x; e < NL * L; e += FT) {
-        const long line = line0 + e / L;
-        const int i = e % L;
-        float2 v = make_float2(0.f, 0.f);
-        if (line < nlines && i < nlast) {
-            const int c = (int)(line / nimg_lines);
-            const long o = line % nimg_lines;
-            const long idx = o * nlast + i;
+    const long line = (long)blockIdx.x * row_lines<L>() + ln;  // flattened (c, o)
+    const bool ok = line < (long)ncoils * nimg_lines;
+    const int c = ok ? (int)(line / nimg_lines) : 0;
+    const long o = ok ? line % nimg_lines : 0;
+    const long y = o / N2o, x = o % N2o;
+    const float2 *pl = p + o * nlast;
+    const float *fl = f + o * nlast;
+    const float2 *sl = sens ? sens + (long)c * nvox + o * nlast : nullptr;
+    float2 *dst = g1 + (((long)c * N1 + y) * A + x) * L;
+    float2 *lb = buf + (fftc::num_stages(L) > 1 ? ln * fftc::line_lds(L) : 0);
+    run_fft<L, false>(
+        tl, tw,
+        [&](int b, int r, int i) {
+            if (!ok || fftc::first_reg<L>(b, r) >= E / 2) return make_float2(0.f, 0.f);
             float sn, cs;
-            sincos_rev(frac_near(f[idx] * tau_l), &sn, &cs);
-            const float2 pv = p[idx];
+            sincos_rev(frac_near(fl[i] * tau_l), &sn, &cs);
+            const float2 pv = pl[i];
             // e^{-i 2pi f tau} p
-            float2 u = make_float2(pv.x * cs + pv.y * sn, pv.y * cs - pv.x * sn);
-            if (sens) u = cmul(sens[(long)c * nvox + idx], u);
-            v = u;
-        }
-        buf[e] = v;
-    }
-    __syncthreads();
-    fft_lds<L, NL, false>(buf, L, tw);
-    for (int e = threadIdx.x; e < NL * L; e += FT) {
-        const long line = line0 + e / L;
-        if (line >= nlines) continue;
-        const int c = (int)(line / nimg_lines);
-        const long o = line % nimg_lines;
-        const long y = o / N2o, x = o % N2o;
-        g1[(((long)c * N1 + y) * A + x) * L + (e % L)] = buf[e];
-    }
+            const float2 u = make_float2(pv.x * cs + pv.y * sn, pv.y * cs - pv.x * sn);
+            return sl ? cmul(sl[i], u) : u;
+        },
+        [] {},
+        [&](int, int, int i, float2 v) {
+            if (ok) dst[i] = v;
+        },
+        [&](int, int, int i) { return lb[fftc::lpad(i)]; },
+        [&](int, int, int i, float2 v) { lb[fftc::lpad(i)] = v; });
 }
 
 // ---------------------------------------------------------------- pass E
-// For NL image lines and the coils of one coil group: inverse FFT of the G1
-// lines, keep the first nlast elements, sum conj(S_c) * z_c over the group,
-// write partial[group][voxel].
+// For one image line and the coils of one coil group: inverse FFT of the G1
+// lines, keep the first nlast elements, sum conj(S_c) * z_c over the group
+// in registers, write partial[group][voxel].
 template <int L>
-__global__ __launch_bounds__(FT) void k_toep_pass_e(
-    const float2 *__restrict__ g1, const float2 *__restrict__ sens, int nimg_lines,
-    int nlast, int N1, int N2o, int A, int ncoils, int coils_per_group,
-    float2 *__restrict__ partial, const float2 *__restrict__ tw) {
-    constexpr int NL = contig_lines<L>();
-    constexpr int PER = (NL * L + FT - 1) / FT;
-    __shared__ float2 buf[NL * L];
+__global__ __launch_bounds__(ROW_THREADS) void k_toep_pass_e(
+    const float2 *__restrict__ g1, const float2 *__restrict__ sens, int nimg_lines, int N1,
+    int N2o, int A, int ncoils, int coils_per_group, float2 *__restrict__ partial,
+    const float2 *__restrict__ tw) {
+    constexpr int T = fftc::threads_per_line(L);
+    constexpr int E = fftc::vals_per_thread(L);
+    constexpr int nlast = L / 2;
+    __shared__ float2 buf[row_lds<L>()];
+    const int tl = threadIdx.x % T, ln = threadIdx.x / T;
     const long nvox = (long)nimg_lines * nlast;
-    const long o0 = (long)blockIdx.x * NL;
+    const long o = (long)blockIdx.x * row_lines<L>() + ln;
+    const bool ok = o < nimg_lines;
+    const long oc = ok ? o : 0;
+    const long y = oc / N2o, x = oc % N2o;
     const int group = blockIdx.y;
     const int c0 = group * coils_per_group;
     const int c1 = min(ncoils, c0 + coils_per_group);
-    float2 acc[PER];
+    float2 *lb = buf + (fftc::num_stages(L) > 1 ? ln * fftc::line_lds(L) : 0);
+    // only the kept half (i < nlast, i.e. register m < E/2) is accumulated
+    float2 acc[E / 2];
 #pragma unroll
-    for (int q = 0; q < PER; ++q) acc[q] = make_float2(0.f, 0.f);
+    for (int m = 0; m < E / 2; ++m) acc[m] = make_float2(0.f, 0.f);
     for (int c = c0; c < c1; ++c) {
-        for (int e = threadIdx.x; e < NL * L; e += FT) {
-            const long o = o0 + e / L;
-            float2 v = make_float2(0.f, 0.f);
-            if (o < nimg_lines) {
-                const long y = o / N2o, x = o % N2o;
-                v = g1[(((long)c * N1 + y) * A + x) * L + (e % L)];
-            }
-            buf[e] = v;
-        }
-        __syncthreads();
-        fft_lds<L, NL, true>(buf, L, tw);
-#pragma unroll
-        for (int q = 0; q < PER; ++q) {
-            const int e = threadIdx.x + q * FT;
-            if (e < NL * L) {
-                const long o = o0 + e / L;
-                const int i = e % L;
-                if (o < nimg_lines && i < nlast) {
-                    const float2 z = buf[e];
-                    if (sens) {
-                        const float2 s = sens[(long)c * nvox + o * nlast + i];
+        const float2 *src = g1 + (((long)c * N1 + y) * A + x) * L;
+        const float2 *sl = sens ? sens + (long)c * nvox + oc * nlast : nullptr;
+        run_fft<L, true>(
+            tl, tw, [&](int, int, int i) { return ok ? src[i] : make_float2(0.f, 0.f); },
+            [] { __syncthreads(); },  // the previous coil's last stage read these lines
+            [&](int b, int r, int i, float2 z) {
+                const int m = fftc::last_reg<L>(b, r);
+                if (m < E / 2) {
+                    float2 &a = acc[m];
+                    if (sl) {
+                        const float2 s = ok ? sl[i] : make_float2(0.f, 0.f);
                         // conj(s) * z
-                        acc[q].x += s.x * z.x + s.y * z.y;
-                        acc[q].y += s.x * z.y - s.y * z.x;
+                        a.x += s.x * z.x + s.y * z.y;
+                        a.y += s.x * z.y - s.y * z.x;
                     } else {
-                        acc[q].x += z.x;
-                        acc[q].y += z.y;
+                        a.x += z.x;
+                        a.y += z.y;
                     }
                 }
-            }
-        }
-        __syncthreads();  // buf reused by the next coil
+            },
+            [&](int, int, int i) { return lb[fftc::lpad(i)]; },
+            [&](int, int, int i, float2 v) { lb[fftc::lpad(i)] = v; });
     }
+    if (!ok) return;
 #pragma unroll
-    for (int q = 0; q < PER; ++q) {
-        const int e = threadIdx.x + q * FT;
-        if (e < NL * L) {
-            const long o = o0 + e / L;
-            const int i = e % L;
-            if (o < nimg_lines && i < nlast)
-                partial[(long)group * nvox + o * nlast + i] = acc[q];
-        }
-    }
+    for (int m = 0; m < E / 2; ++m)
+        partial[(long)group * nvox + oc * nlast + tl + T * m] = acc[m];
 }
 
 // ---------------------------------------------------------------- pass F
@@ -377,26 +334,27 @@ int fft_grid(imp_ctx *ctx, float2 *data, int M1, int M2, int M3, bool inverse) {
     IMP_TRY(twiddles(ctx, Ml, &tw));
     const long nl = (long)M1 * (d3 ? M2 : 1);
     IMP_L_SWITCH(Ml, {
-        const int NL = contig_lines<L_>();
+        const int NL = row_lines<L_>();
         if (inverse)
-            hipLaunchKernelGGL((k_fft_contig<L_, true>), dim3(ceil_div(nl, NL)), dim3(FT), 0,
-                               ctx->stream, data, nl, tw);
+            hipLaunchKernelGGL((k_fft_rows<L_, true>), dim3(ceil_div(nl, NL)), dim3(ROW_THREADS),
+                               0, ctx->stream, data, nl, tw);
         else
-            hipLaunchKernelGGL((k_fft_contig<L_, false>), dim3(ceil_div(nl, NL)), dim3(FT), 0,
-                               ctx->stream, data, nl, tw);
+            hipLaunchKernelGGL((k_fft_rows<L_, false>), dim3(ceil_div(nl, NL)), dim3(ROW_THREADS),
+                               0, ctx->stream, data, nl, tw);
     });
     IMP_CHECK_LAUNCH();
     if (d3) {  // x axis: batch y, lines z
         IMP_TRY(twiddles(ctx, M2, &tw));
         IMP_L_SWITCH(M2, {
-            const int P = panel_lines<L_>();
-            dim3 grid(ceil_div(M3, P), M1);
+            dim3 grid(ceil_div(M3, col_lines<L_>()), M1);
             if (inverse)
-                hipLaunchKernelGGL((k_fft_panel<L_, 1>), grid, dim3(FT), 0, ctx->stream, data,
-                                   (long)M2 * M3, (long)M3, M3, M2, M2, nullptr, tw);
+                hipLaunchKernelGGL((k_fft_cols<L_, 1, false, false>), grid,
+                                   dim3(col_threads<L_>()), 0, ctx->stream, data, (long)M2 * M3,
+                                   (long)M3, M3, nullptr, tw);
             else
-                hipLaunchKernelGGL((k_fft_panel<L_, 0>), grid, dim3(FT), 0, ctx->stream, data,
-                                   (long)M2 * M3, (long)M3, M3, M2, M2, nullptr, tw);
+                hipLaunchKernelGGL((k_fft_cols<L_, 0, false, false>), grid,
+                                   dim3(col_threads<L_>()), 0, ctx->stream, data, (long)M2 * M3,
+                                   (long)M3, M3, nullptr, tw);
         });
         IMP_CHECK_LAUNCH();
     }
@@ -404,14 +362,13 @@ int fft_grid(imp_ctx *ctx, float2 *data, int M1, int M2, int M3, bool inverse) {
     IMP_TRY(twiddles(ctx, M1, &tw));
     const int nlines = M2 * M3;
     IMP_L_SWITCH(M1, {
-        const int P = panel_lines<L_>();
-        dim3 grid(ceil_div(nlines, P), 1);
+        dim3 grid(ceil_div(nlines, col_lines<L_>()), 1);
         if (inverse)
-            hipLaunchKernelGGL((k_fft_panel<L_, 1>), grid, dim3(FT), 0, ctx->stream, data, 0L,
-                               (long)nlines, nlines, M1, M1, nullptr, tw);
+            hipLaunchKernelGGL((k_fft_cols<L_, 1, false, false>), grid, dim3(col_threads<L_>()),
+                               0, ctx->stream, data, 0L, (long)nlines, nlines, nullptr, tw);
         else
-            hipLaunchKernelGGL((k_fft_panel<L_, 0>), grid, dim3(FT), 0, ctx->stream, data, 0L,
-                               (long)nlines, nlines, M1, M1, nullptr, tw);
+            hipLaunchKernelGGL((k_fft_cols<L_, 0, false, false>), grid, dim3(col_threads<L_>()),
+                               0, ctx->stream, data, 0L, (long)nlines, nlines, nullptr, tw);
     });
     IMP_CHECK_LAUNCH();
     return IMP_OK;
@@ -437,6 +394,8 @@ int toep_apply_normal(imp_ctx *ctx, ToepPlan &P, const float2 *p, float2 *out) {
     const int A = d3 ? P.M2 : 1;
     const int nlast = d3 ? P.N3 : P.N2;
     const int Ml = d3 ? P.M3 : P.M2;
+    if (2 * nlast != Ml)
+        return fail(IMP_EINVAL, "Toeplitz apply: last axis %d is not 2 x %d", Ml, nlast);
     const int nimg_lines = P.N1 * N2o;
     const long nvox = P.nvox;
     const float scale = (float)(1.0 / (double)P.G);
@@ -452,11 +411,10 @@ int toep_apply_normal(imp_ctx *ctx, ToepPlan &P, const float2 *p, float2 *out) {
         // pass A
         int rec = prof_begin(ctx, "toep_pass_a");
         IMP_L_SWITCH(Ml, {
-            const int NL = contig_lines<L_>();
             const long nlines = (long)C * nimg_lines;
-            hipLaunchKernelGGL((k_toep_pass_a<L_>), dim3(ceil_div(nlines, NL)), dim3(FT), 0,
-                               ctx->stream, p, P.sens, P.fcyc, tau_l, nimg_lines, nlast, P.N1,
-                               N2o, A, C, P.g1, tw_last);
+            hipLaunchKernelGGL((k_toep_pass_a<L_>), dim3(ceil_div(nlines, row_lines<L_>())),
+                               dim3(ROW_THREADS), 0, ctx->stream, p, P.sens, P.fcyc, tau_l,
+                               nimg_lines, P.N1, N2o, A, C, P.g1, tw_last);
         });
         prof_end(ctx, rec);
         IMP_CHECK_LAUNCH();
@@ -464,11 +422,10 @@ int toep_apply_normal(imp_ctx *ctx, ToepPlan &P, const float2 *p, float2 *out) {
         if (d3) {
             rec = prof_begin(ctx, "toep_pass_b");
             IMP_L_SWITCH(P.M2, {
-                const int Pn = panel_lines<L_>();
-                dim3 grid(ceil_div(P.M3, Pn), C * P.N1);
-                hipLaunchKernelGGL((k_fft_panel<L_, 0>), grid, dim3(FT), 0, ctx->stream, P.g1,
-                                   (long)P.M2 * P.M3, (long)P.M3, P.M3, P.N2, P.M2, nullptr,
-                                   tw_x);
+                dim3 grid(ceil_div(P.M3, col_lines<L_>()), C * P.N1);
+                hipLaunchKernelGGL((k_fft_cols<L_, 0, true, false>), grid, dim3(col_threads<L_>()),
+                                   0, ctx->stream, P.g1, (long)P.M2 * P.M3, (long)P.M3, P.M3,
+                                   nullptr, tw_x);
             });
             prof_end(ctx, rec);
             IMP_CHECK_LAUNCH();
@@ -479,10 +436,9 @@ int toep_apply_normal(imp_ctx *ctx, ToepPlan &P, const float2 *p, float2 *out) {
             const float2 *qf = P.qf + (long)l * P.G;
             rec = prof_begin(ctx, "toep_pass_c");
             IMP_L_SWITCH(P.M1, {
-                const int Pn = panel_lines<L_>();
-                dim3 grid(ceil_div(nl, Pn), C);
-                hipLaunchKernelGGL((k_fft_panel<L_, 2>), grid, dim3(FT), 0, ctx->stream, P.g1,
-                                   (long)P.N1 * nl, (long)nl, nl, P.N1, P.N1, qf, tw_y);
+                dim3 grid(ceil_div(nl, col_lines<L_>()), C);
+                hipLaunchKernelGGL((k_fft_cols<L_, 2, true, true>), grid, dim3(col_threads<L_>()),
+                                   0, ctx->stream, P.g1, (long)P.N1 * nl, (long)nl, nl, qf, tw_y);
             });
             prof_end(ctx, rec);
             IMP_CHECK_LAUNCH();
@@ -491,11 +447,10 @@ int toep_apply_normal(imp_ctx *ctx, ToepPlan &P, const float2 *p, float2 *out) {
         if (d3) {
             rec = prof_begin(ctx, "toep_pass_d");
             IMP_L_SWITCH(P.M2, {
-                const int Pn = panel_lines<L_>();
-                dim3 grid(ceil_div(P.M3, Pn), C * P.N1);
-                hipLaunchKernelGGL((k_fft_panel<L_, 1>), grid, dim3(FT), 0, ctx->stream, P.g1,
-                                   (long)P.M2 * P.M3, (long)P.M3, P.M3, P.M2, P.N2, nullptr,
-                                   tw_x);
+                dim3 grid(ceil_div(P.M3, col_lines<L_>()), C * P.N1);
+                hipLaunchKernelGGL((k_fft_cols<L_, 1, false, true>), grid, dim3(col_threads<L_>()),
+                                   0, ctx->stream, P.g1, (long)P.M2 * P.M3, (long)P.M3, P.M3,
+                                   nullptr, tw_x);
             });
             prof_end(ctx, rec);
             IMP_CHECK_LAUNCH();
@@ -503,11 +458,9 @@ int toep_apply_normal(imp_ctx *ctx, ToepPlan &P, const float2 *p, float2 *out) {
         // pass E: inverse last axis, conj(S_c), partial sums over coil groups
         rec = prof_begin(ctx, "toep_pass_e");
         IMP_L_SWITCH(Ml, {
-            const int NL = contig_lines<L_>();
-            dim3 grid(ceil_div(nimg_lines, NL), ngroups);
-            hipLaunchKernelGGL((k_toep_pass_e<L_>), grid, dim3(FT), 0, ctx->stream, P.g1,
-                               P.sens, nimg_lines, nlast, P.N1, N2o, A, C, cpg, P.partial,
-                               tw_last);
+            dim3 grid(ceil_div(nimg_lines, row_lines<L_>()), ngroups);
+            hipLaunchKernelGGL((k_toep_pass_e<L_>), grid, dim3(ROW_THREADS), 0, ctx->stream, P.g1,
+                               P.sens, nimg_lines, P.N1, N2o, A, C, cpg, P.partial, tw_last);
         });
         prof_end(ctx, rec);
         IMP_CHECK_LAUNCH();
