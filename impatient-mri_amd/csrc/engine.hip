@@ -383,10 +383,11 @@ extern "C" int imp_toep_create(imp_ctx *ctx, int N1, int N2, int N3, int ncoils,
     P.tau = tau;
     P.t0 = t0;
     P.lambda = lambda;
-    // coil groups for pass E: enough workgroups to fill the GPU
+    // coil groups for pass E: enough workgroups to fill the GPU (pass E
+    // holds 4096 / Ml image lines per workgroup, 256 when Ml < 16)
     {
         const int Ml = N3 > 1 ? P.M3 : P.M2;
-        const int nl = (4 * 256) / Ml > 1 ? (4 * 256) / Ml : 1;
+        const int nl = Ml >= 16 ? 4096 / Ml : 256;
         const long wgs_per_group = (long)ceil_div((long)N1 * (N3 > 1 ? N2 : 1), nl);
         int groups = 1;
         while (groups < ncoils && wgs_per_group * groups < 4L * ctx->num_cus) groups *= 2;
@@ -410,6 +411,9 @@ extern "C" int imp_toep_create(imp_ctx *ctx, int N1, int N2, int N3, int ncoils,
     ALLOC(P.vp, (size_t)P.nvox * sizeof(float2));
     ALLOC(P.vq, (size_t)P.nvox * sizeof(float2));
     ALLOC(P.vbest, (size_t)P.nvox * sizeof(float2));
+    ALLOC(P.vz, (size_t)P.nvox * sizeof(float2));
+    ALLOC(P.pw, (size_t)P.nvox * sizeof(float));
+    ALLOC(P.pw_chan, (size_t)P.nvox * sizeof(float));
     ALLOC(P.scal, 16 * sizeof(double));
 #undef ALLOC
     if (st != IMP_OK) {
@@ -420,6 +424,11 @@ extern "C" int imp_toep_create(imp_ctx *ctx, int N1, int N2, int N3, int ncoils,
                        fm, P.fcyc, P.nvox);
     if (s_r) st = split_to_c(ctx, s_r, s_i, P.sens, (long)ncoils * P.nvox);
     if (st == IMP_OK) st = toep_prepare_qf(ctx, P, q_r, q_i);
+    if (st == IMP_OK) st = precond_chan(ctx, P, q_r, q_i);
+    if (st == IMP_OK &&
+        hipMemcpyAsync(P.pw_chan, P.pw, (size_t)P.nvox * sizeof(float), hipMemcpyDeviceToDevice,
+                       ctx->stream) != hipSuccess)
+        st = fail(IMP_EHIP, "imp_toep_create: %s", hipGetErrorString(hipGetLastError()));
     if (st == IMP_OK && hipStreamSynchronize(ctx->stream) != hipSuccess)
         st = fail(IMP_EHIP, "imp_toep_create: %s", hipGetErrorString(hipGetLastError()));
     if (st != IMP_OK) {
@@ -427,6 +436,61 @@ extern "C" int imp_toep_create(imp_ctx *ctx, int N1, int N2, int N3, int ncoils,
         return st;
     }
     *out = op;
+    return IMP_OK;
+}
+
+namespace {
+__global__ void k_check_weights(const float *w, long n, unsigned int *bad) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && !(w[i] > 0.f && w[i] < INFINITY)) atomicOr(bad, 1u);
+}
+}  // namespace
+
+extern "C" int imp_toep_set_precond(imp_toep *op, const float *w) {
+    if (!op || !op->ctx) return fail(IMP_EINVAL, "imp_toep_set_precond: bad operator");
+    ToepPlan &P = op->P;
+    imp_ctx *ctx = op->ctx;
+    const size_t bytes = (size_t)P.nvox * sizeof(float);
+    if (!w) {
+        IMP_HIP(hipMemcpyAsync(P.pw, P.pw_chan, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        IMP_HIP(hipStreamSynchronize(ctx->stream));
+        return IMP_OK;
+    }
+    // reject weights that would make M indefinite (checked before copying)
+    IMP_TRY(ensure_dscratch(ctx, 16));
+    unsigned int *bad = (unsigned int *)ctx->dscratch;
+    IMP_HIP(hipMemsetAsync(bad, 0, sizeof(unsigned int), ctx->stream));
+    hipLaunchKernelGGL(k_check_weights, dim3(ceil_div(P.nvox, 256)), dim3(256), 0, ctx->stream, w,
+                       P.nvox, bad);
+    IMP_CHECK_LAUNCH();
+    unsigned int hbad = 0;
+    IMP_HIP(hipMemcpyAsync(&hbad, bad, sizeof(unsigned int), hipMemcpyDeviceToHost, ctx->stream));
+    IMP_HIP(hipStreamSynchronize(ctx->stream));
+    if (hbad)
+        return fail(IMP_EINVAL, "imp_toep_set_precond: weights must be positive and finite");
+    IMP_HIP(hipMemcpyAsync(P.pw, w, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    IMP_HIP(hipStreamSynchronize(ctx->stream));
+    return IMP_OK;
+}
+
+extern "C" int imp_toep_get_precond(imp_toep *op, float *w) {
+    if (!op || !op->ctx || !w) return fail(IMP_EINVAL, "imp_toep_get_precond: bad arguments");
+    IMP_HIP(hipMemcpyAsync(w, op->P.pw, (size_t)op->P.nvox * sizeof(float),
+                           hipMemcpyDeviceToDevice, op->ctx->stream));
+    IMP_HIP(hipStreamSynchronize(op->ctx->stream));
+    return IMP_OK;
+}
+
+extern "C" int imp_toep_precond_apply(imp_toep *op, const float *r_r, const float *r_i,
+                                      float *z_r, float *z_i) {
+    if (!op || !op->ctx || !r_r || !r_i || !z_r || !z_i)
+        return fail(IMP_EINVAL, "imp_toep_precond_apply: bad arguments");
+    ToepPlan &P = op->P;
+    imp_ctx *ctx = op->ctx;
+    IMP_TRY(split_to_c(ctx, r_r, r_i, P.vx, P.nvox));
+    IMP_TRY(precond_apply(ctx, P, P.vx, P.vq));
+    IMP_TRY(c_to_split(ctx, P.vq, z_r, z_i, P.nvox));
+    IMP_HIP(hipStreamSynchronize(ctx->stream));
     return IMP_OK;
 }
 

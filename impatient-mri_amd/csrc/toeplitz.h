@@ -26,8 +26,12 @@ struct ToepPlan {
     float *w = nullptr;           // [nvox] TV weight or null (W = I), not owned
     // CG work vectors (interleaved)
     float2 *vb = nullptr, *vx = nullptr, *vr = nullptr, *vp = nullptr,
-           *vq = nullptr, *vbest = nullptr;
+           *vq = nullptr, *vbest = nullptr, *vz = nullptr;
     double *scal = nullptr;       // device CG scalars
+    // circulant preconditioner M^-1 = F^H diag(pw) F on the image grid
+    // (natural DFT order, 1/nvox of the inverse DFT folded in); all > 0
+    float *pw = nullptr;
+    float *pw_chan = nullptr;     // the Chan weights computed at create time
 };
 
 int twiddles(imp_ctx *ctx, int L, const float2 **out);
@@ -48,6 +52,12 @@ int axpy_c(imp_ctx *ctx, float alpha, const float2 *x, float2 *y, long n);  // y
 int toep_cg(imp_ctx *ctx, ToepPlan &P, const float2 *b, float2 *x, int max_iter,
             double tol, int flags, imp_cg_stats *stats);
 int toep_apply_full(imp_ctx *ctx, ToepPlan &P, const float2 *p, float2 *out);
+// T. Chan's optimal circulant approximation of the summed Toeplitz kernel
+// sum_l Q_l (standard Q grid) plus lambda times the periodic Laplacian;
+// writes P.pw.
+int precond_chan(imp_ctx *ctx, ToepPlan &P, const float *q_r, const float *q_i);
+// z = M^-1 r
+int precond_apply(imp_ctx *ctx, ToepPlan &P, const float2 *r, float2 *z);
 
 // NUDFT (nudft.hip)
 int nudft_adjoint(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *kx,

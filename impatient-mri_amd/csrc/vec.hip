@@ -162,7 +162,8 @@ enum {
     SC_IMPROVED = 5, // 1 if this iteration improved the best residual
     SC_BREAK = 6,    // breakdown flag
     SC_PQ = 7,
-    SC_N = 8
+    SC_RHO = 8,      // r^H z (preconditioned) or |r|^2 (plain)
+    SC_N = 9
 };
 
 __global__ void k_cg_init(double *sc, const double *partials, int np) {
@@ -172,8 +173,25 @@ __global__ void k_cg_init(double *sc, const double *partials, int np) {
         for (int i = 0; i < np; ++i) s += partials[i];
         sc[SC_BETA] = s;
         sc[SC_BEST] = s;
+        sc[SC_RHO] = s;
         sc[SC_IMPROVED] = 0.0;
         sc[SC_BREAK] = 0.0;
+    }
+}
+
+// PCG: rho = Re(r^H z) from partials; ratio = rho_new / rho_old (init: just rho)
+__global__ __launch_bounds__(VT) void k_pcg_rho(const double *__restrict__ partials, int np,
+                                                double *__restrict__ sc, int init) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < np; i += VT) s += partials[i];
+    s = block_sum(s);
+    if (threadIdx.x == 0) {
+        if (!init) {
+            const double rho = sc[SC_RHO];
+            sc[SC_RATIO] = rho > 0.0 ? s / rho : 0.0;
+        }
+        sc[SC_RHO] = s;
+        if (!(s > 0.0) && sc[SC_BETA] > 0.0) sc[SC_BREAK] = 1.0;  // M not positive definite
     }
 }
 
@@ -188,7 +206,7 @@ __global__ __launch_bounds__(VT) void k_cg_alpha(const double *__restrict__ part
             sc[SC_BREAK] = 1.0;
             sc[SC_ALPHA] = 0.0;
         } else {
-            sc[SC_ALPHA] = sc[SC_BETA] / s;
+            sc[SC_ALPHA] = sc[SC_RHO] / s;
         }
     }
 }
@@ -217,14 +235,19 @@ __global__ __launch_bounds__(VT) void k_cg_update_xr(float2 *__restrict__ x, flo
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
+// new |r|^2: best-iterate tracking; plain CG (pcg = 0) also sets
+// ratio = |r_new|^2 / |r_old|^2 and rho
 __global__ __launch_bounds__(VT) void k_cg_beta(const double *__restrict__ partials, int np,
-                                                double *__restrict__ sc) {
+                                                double *__restrict__ sc, int pcg) {
     double s = 0.0;
     for (int i = threadIdx.x; i < np; i += VT) s += partials[i];
     s = block_sum(s);
     if (threadIdx.x == 0) {
-        const double beta = sc[SC_BETA];
-        sc[SC_RATIO] = beta > 0.0 ? s / beta : 0.0;
+        if (!pcg) {
+            const double beta = sc[SC_BETA];
+            sc[SC_RATIO] = beta > 0.0 ? s / beta : 0.0;
+            sc[SC_RHO] = s;
+        }
         sc[SC_BETA] = s;
         if (s < sc[SC_BEST]) {
             sc[SC_BEST] = s;
@@ -235,7 +258,7 @@ __global__ __launch_bounds__(VT) void k_cg_beta(const double *__restrict__ parti
     }
 }
 
-// p = r + ratio p; if improved: best = x
+// p = r + ratio p (r: the residual, or M^-1 r in PCG); if improved: best = x
 __global__ void k_cg_update_p(float2 *__restrict__ p, const float2 *__restrict__ r,
                               const float2 *__restrict__ x, float2 *__restrict__ best, long n,
                               const double *__restrict__ sc, int track_best) {
@@ -248,6 +271,67 @@ __global__ void k_cg_update_p(float2 *__restrict__ p, const float2 *__restrict__
     pv.y = rv.y + ratio * pv.y;
     p[i] = pv;
     if (track_best && sc[SC_IMPROVED] != 0.0) best[i] = x[i];
+}
+
+__global__ void k_scale_by_w(float2 *__restrict__ z, const float *__restrict__ w, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const float s = w[i];
+        z[i] = make_float2(z[i].x * s, z[i].y * s);
+    }
+}
+
+// T. Chan's optimal circulant of a Toeplitz matrix with kernel t (per axis
+// c_j = ((N - j) t_j + j t_{j-N}) / N), applied on every axis to
+// t = sum_l Q_l on the standard Q grid ([2N1][2N2][M3], offset -N per axis).
+__global__ void k_chan_circulant(const float *__restrict__ q_r, const float *__restrict__ q_i,
+                                 int nseg, int N1, int N2, int N3, float2 *__restrict__ c) {
+    const long n = (long)N1 * N2 * N3;
+    const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int M2 = 2 * N2, M3 = N3 > 1 ? 2 * N3 : 1;
+    const long G = 2L * N1 * M2 * M3;
+    const int j3 = (int)(j % N3), j2 = (int)((j / N3) % N2), j1 = (int)(j / ((long)N3 * N2));
+    double ar = 0.0, ai = 0.0;
+    for (int a1 = 0; a1 < 2; ++a1) {
+        const double w1 = a1 ? (double)j1 / N1 : (double)(N1 - j1) / N1;
+        if (w1 == 0.0) continue;
+        const int m1 = j1 + (1 - a1) * N1;
+        for (int a2 = 0; a2 < 2; ++a2) {
+            const double w2 = a2 ? (double)j2 / N2 : (double)(N2 - j2) / N2;
+            if (w2 == 0.0) continue;
+            const int m2 = j2 + (1 - a2) * N2;
+            for (int a3 = 0; a3 < (N3 > 1 ? 2 : 1); ++a3) {
+                const double w3 = N3 > 1 ? (a3 ? (double)j3 / N3 : (double)(N3 - j3) / N3) : 1.0;
+                if (w3 == 0.0) continue;
+                const int m3 = N3 > 1 ? j3 + (1 - a3) * N3 : 0;
+                const long idx = ((long)m1 * M2 + m2) * M3 + m3;
+                double sr = 0.0, si = 0.0;
+                for (int l = 0; l < nseg; ++l) {
+                    sr += q_r[(long)l * G + idx];
+                    si += q_i[(long)l * G + idx];
+                }
+                ar += w1 * w2 * w3 * sr;
+                ai += w1 * w2 * w3 * si;
+            }
+        }
+    }
+    c[j] = make_float2((float)ar, (float)ai);
+}
+
+// pw = 1 / (nvox * max(Re(DFT c) + lambda * periodic-Laplacian eigenvalue,
+// 1e-3 * DC eigenvalue)); the floor keeps M positive definite.
+__global__ void k_precond_weights(const float2 *__restrict__ chat, int N1, int N2, int N3,
+                                  float lambda, float *__restrict__ w) {
+    const long n = (long)N1 * N2 * N3;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int k3 = (int)(i % N3), k2 = (int)((i / N3) % N2), k1 = (int)(i / ((long)N3 * N2));
+    double lap = 2.0 - 2.0 * cospi(2.0 * k1 / N1) + 2.0 - 2.0 * cospi(2.0 * k2 / N2);
+    if (N3 > 1) lap += 2.0 - 2.0 * cospi(2.0 * k3 / N3);
+    const double dc = (double)chat[0].x;
+    const double eig = fmax((double)chat[i].x + (double)lambda * lap, 1e-3 * fabs(dc));
+    w[i] = eig > 0.0 ? (float)(1.0 / (eig * (double)n)) : 1.f;
 }
 
 __global__ void k_sub_c(const float2 *a, const float2 *b, float2 *out, long n) {
@@ -324,6 +408,33 @@ int toep_apply_full(imp_ctx *ctx, ToepPlan &P, const float2 *p, float2 *out) {
     return IMP_OK;
 }
 
+int precond_chan(imp_ctx *ctx, ToepPlan &P, const float *q_r, const float *q_i) {
+    const long n = P.nvox;
+    float2 *c = P.vz;  // scratch: the circulant's first column, then its DFT
+    hipLaunchKernelGGL(k_chan_circulant, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, q_r,
+                       q_i, P.nseg, P.N1, P.N2, P.N3, c);
+    IMP_CHECK_LAUNCH();
+    IMP_TRY(fft_grid(ctx, c, P.N1, P.N2, P.N3, false));
+    hipLaunchKernelGGL(k_precond_weights, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, c,
+                       P.N1, P.N2, P.N3, P.lambda, P.pw);
+    IMP_CHECK_LAUNCH();
+    return IMP_OK;
+}
+
+int precond_apply(imp_ctx *ctx, ToepPlan &P, const float2 *r, float2 *z) {
+    const long n = P.nvox;
+    const int rec = prof_begin(ctx, "precond_apply");
+    if (z != r)
+        IMP_HIP(hipMemcpyAsync(z, r, n * sizeof(float2), hipMemcpyDeviceToDevice, ctx->stream));
+    IMP_TRY(fft_grid(ctx, z, P.N1, P.N2, P.N3, false));
+    hipLaunchKernelGGL(k_scale_by_w, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, z, P.pw,
+                       n);
+    IMP_CHECK_LAUNCH();
+    IMP_TRY(fft_grid(ctx, z, P.N1, P.N2, P.N3, true));
+    prof_end(ctx, rec);
+    return IMP_OK;
+}
+
 int toep_cg(imp_ctx *ctx, ToepPlan &P, const float2 *b, float2 *x, int max_iter, double tol,
             int flags, imp_cg_stats *stats) {
     const auto t_start = std::chrono::steady_clock::now();
@@ -333,6 +444,8 @@ int toep_cg(imp_ctx *ctx, ToepPlan &P, const float2 *b, float2 *x, int max_iter,
     double *partials = ctx->dscratch;
     double *sc = P.scal;
     const bool best = (flags & IMP_CG_BEST_ITERATE) != 0;
+    const bool pc = (flags & IMP_CG_PRECOND) != 0;
+    if (pc && (!P.pw || !P.vz)) return fail(IMP_EINVAL, "PCG: operator has no preconditioner");
     const dim3 g1(ceil_div(n, 256)), b1(256);
 
     // |b|^2
@@ -355,12 +468,22 @@ int toep_cg(imp_ctx *ctx, ToepPlan &P, const float2 *b, float2 *x, int max_iter,
         IMP_HIP(hipMemcpyAsync(P.vr, b, n * sizeof(float2), hipMemcpyDeviceToDevice, ctx->stream));
     }
     IMP_CHECK_LAUNCH();
-    IMP_HIP(hipMemcpyAsync(P.vp, P.vr, n * sizeof(float2), hipMemcpyDeviceToDevice, ctx->stream));
     IMP_HIP(hipMemcpyAsync(P.vbest, x, n * sizeof(float2), hipMemcpyDeviceToDevice, ctx->stream));
     hipLaunchKernelGGL(k_dot_re, dim3(nb), dim3(VT), 0, ctx->stream, P.vr, P.vr, n, partials);
     IMP_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_cg_init, dim3(1), dim3(1), 0, ctx->stream, sc, partials, nb);
     IMP_CHECK_LAUNCH();
+    if (pc) {  // z = M^-1 r, p = z, rho = Re(r^H z)
+        IMP_TRY(precond_apply(ctx, P, P.vr, P.vz));
+        hipLaunchKernelGGL(k_dot_re, dim3(nb), dim3(VT), 0, ctx->stream, P.vr, P.vz, n, partials);
+        hipLaunchKernelGGL(k_pcg_rho, dim3(1), dim3(VT), 0, ctx->stream, partials, nb, sc, 1);
+        IMP_CHECK_LAUNCH();
+        IMP_HIP(hipMemcpyAsync(P.vp, P.vz, n * sizeof(float2), hipMemcpyDeviceToDevice,
+                               ctx->stream));
+    } else {
+        IMP_HIP(hipMemcpyAsync(P.vp, P.vr, n * sizeof(float2), hipMemcpyDeviceToDevice,
+                               ctx->stream));
+    }
 
     int iter = 0, best_iter = 0;
     bool converged = false;
@@ -371,9 +494,16 @@ int toep_cg(imp_ctx *ctx, ToepPlan &P, const float2 *b, float2 *x, int max_iter,
         hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(VT), 0, ctx->stream, partials, nb, sc);
         hipLaunchKernelGGL(k_cg_update_xr, dim3(nb), dim3(VT), 0, ctx->stream, x, P.vr, P.vp,
                            P.vq, n, sc, partials);
-        hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(VT), 0, ctx->stream, partials, nb, sc);
-        hipLaunchKernelGGL(k_cg_update_p, g1, b1, 0, ctx->stream, P.vp, P.vr, x, P.vbest, n, sc,
-                           best ? 1 : 0);
+        hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(VT), 0, ctx->stream, partials, nb, sc,
+                           pc ? 1 : 0);
+        if (pc) {
+            IMP_TRY(precond_apply(ctx, P, P.vr, P.vz));
+            hipLaunchKernelGGL(k_dot_re, dim3(nb), dim3(VT), 0, ctx->stream, P.vr, P.vz, n,
+                               partials);
+            hipLaunchKernelGGL(k_pcg_rho, dim3(1), dim3(VT), 0, ctx->stream, partials, nb, sc, 0);
+        }
+        hipLaunchKernelGGL(k_cg_update_p, g1, b1, 0, ctx->stream, P.vp, pc ? P.vz : P.vr, x,
+                           P.vbest, n, sc, best ? 1 : 0);
         IMP_CHECK_LAUNCH();
         IMP_HIP(hipMemcpyAsync(ctx->host_scalars, sc, SC_N * sizeof(double),
                                hipMemcpyDeviceToHost, ctx->stream));
@@ -381,8 +511,8 @@ int toep_cg(imp_ctx *ctx, ToepPlan &P, const float2 *b, float2 *x, int max_iter,
         ++iter;
         const double *h = ctx->host_scalars;
         if (h[SC_BREAK] != 0.0)
-            return fail(IMP_EBREAKDOWN, "CG breakdown at iteration %d: p^H A p = %g", iter,
-                        h[SC_PQ]);
+            return fail(IMP_EBREAKDOWN, "CG breakdown at iteration %d: p^H A p = %g, r^H z = %g",
+                        iter, h[SC_PQ], h[SC_RHO]);
         beta2 = h[SC_BETA];
         if (h[SC_IMPROVED] != 0.0) best_iter = iter;
         if (std::sqrt(beta2 / bnorm) < tol) {

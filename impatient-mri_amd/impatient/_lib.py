@@ -26,6 +26,7 @@ IMP_Q_CUMULATIVE = 0x1
 IMP_CG_BEST_ITERATE = 0x1
 IMP_CG_USE_X0 = 0x2
 IMP_CG_TRUE_RESIDUAL = 0x4
+IMP_CG_PRECOND = 0x8
 IMP_COMPAT_Q_CUMULATIVE = 0x1
 
 

@@ -179,6 +179,30 @@ class Toeplitz:
         return _cplx(*self._out), st.as_dict()
 
 
+    def precond_weights(self):
+        """Current circulant preconditioner weights (imp_toep_get_precond),
+        shaped like the image, natural DFT order."""
+        w = self.ctx.alloc((self.nvox,))
+        call("imp_toep_get_precond", self.h, ptr(w))
+        return w.get().reshape(self.shape)
+
+    def set_precond(self, w=None):
+        """Set the circulant weights (None restores the Chan weights)."""
+        if w is None:
+            call("imp_toep_set_precond", self.h, None)
+            return
+        dw = self.ctx.to_device(_f32(np.asarray(w).reshape(-1)))
+        call("imp_toep_set_precond", self.h, ptr(dw))
+
+    def precond_apply(self, r):
+        r = np.asarray(r).reshape(-1)
+        self._in[0].upload(_f32(r.real))
+        self._in[1].upload(_f32(r.imag))
+        call("imp_toep_precond_apply", self.h, ptr(self._in[0]), ptr(self._in[1]),
+             ptr(self._out[0]), ptr(self._out[1]))
+        return _cplx(*self._out)
+
+
 def bruteforce_cg(ctx: Context, kx, ky, t, x, y, fm, d, sens, cg_num, lam, x0):
     """bruteForceGpu's CG from x0 (cg_num - 1 iterations); d is [C, M]."""
     d = np.atleast_2d(np.asarray(d))

@@ -190,6 +190,8 @@ typedef struct {
 #define IMP_CG_BEST_ITERATE   0x1  /* return argmin-residual iterate (recon.cu:2107-2110) */
 #define IMP_CG_USE_X0         0x2  /* start from x instead of 0                 */
 #define IMP_CG_TRUE_RESIDUAL  0x4  /* recompute ||Ax-b||/||b|| after the solve  */
+#define IMP_CG_PRECOND        0x8  /* preconditioned CG with the operator's
+                                      circulant M^-1 (imp_toep_set_precond)   */
 
 /* CG on A x = b (toeplitz_recon's loop, recon.cu:2011-2133).  All scalars
  * (dot products, alpha, beta) are FP64 and stay on the device; tol is on
@@ -197,6 +199,21 @@ typedef struct {
 int imp_toep_cg(imp_toep *op, const float *b_r, const float *b_i,
                 float *x_r, float *x_i, int max_iter, double tol, int flags,
                 imp_cg_stats *stats);
+
+/* Circulant preconditioner of a Toeplitz operator: M^-1 = F^H diag(w) F
+ * with F the DFT of the N1 x N2 x N3 image grid (periodic), w[k] > 0 in
+ * natural DFT order, including the 1/(N1 N2 N3) of the inverse DFT, so M is
+ * Hermitian positive definite and PCG stays valid.  imp_toep_create sets w
+ * to T. Chan's optimal circulant approximation of sum_l Q_l plus lambda
+ * times the periodic Laplacian spectrum.  set: copy w (device, N1*N2*N3
+ * floats, every entry > 0 and finite; NULL restores the Chan weights);
+ * get: copy the current weights out (device).  No counterpart in the
+ * reference, whose CG is unpreconditioned (recon.cu:2011-2133). */
+int imp_toep_set_precond(imp_toep *op, const float *w);
+int imp_toep_get_precond(imp_toep *op, float *w);
+/* z = M^-1 r (device, interleaved Re/Im split as r_r/r_i -> z_r/z_i). */
+int imp_toep_precond_apply(imp_toep *op, const float *r_r, const float *r_i, float *z_r,
+                           float *z_i);
 
 /* ---------------------------------------------- brute-force CG (a11)
  * bruteForceGpu semantics (bruteForce/bruteForceGpu.cu:142-813): Fessler CG

@@ -245,6 +245,93 @@ def toeplitz_cg(apply, b, max_iter, tol=1e-16, best_iterate=True, vec_dtype=np.c
     return (xbest if best_iterate else x), it, hist
 
 
+def chan_circulant(t, dims):
+    """T. Chan's optimal circulant of a multilevel Toeplitz matrix whose
+    kernel t is given on the 2N grid (axis d index m = delta + N_d, delta in
+    [-N_d, N_d); the standard Q-grid layout of main_mri.cu:461-500):
+    per axis c_j = ((N - j) t_j + j t_{j-N}) / N, j in [0, N).  Returns the
+    first column c, shaped dims.  Not in the reference (its CG is
+    unpreconditioned); T. Chan, SIAM J. Sci. Stat. Comput. 9 (1988)."""
+    t = np.asarray(t, np.complex128).reshape([2 * n if n > 1 else 1 for n in dims])
+    c = t
+    for ax, n in enumerate(dims):
+        if n == 1:
+            continue
+        j = np.arange(n)
+        shape = [1] * len(dims)
+        shape[ax] = n
+        w0 = ((n - j) / n).reshape(shape)
+        w1 = (j / n).reshape(shape)
+        pos = np.take(c, n + j, axis=ax)      # t_j
+        neg = np.take(c, j, axis=ax)          # t_{j - N}
+        c = w0 * pos + w1 * neg
+    return c
+
+
+def periodic_laplacian_eigs(dims):
+    """Eigenvalues sum_d (2 - 2 cos(2 pi k_d / N_d)) of sum_d D_d^T D_d with
+    the periodic first differences of fd_reg, in natural DFT order."""
+    out = np.zeros(dims)
+    for ax, n in enumerate(dims):
+        if n == 1:
+            continue
+        shape = [1] * len(dims)
+        shape[ax] = n
+        out = out + (2 - 2 * np.cos(2 * np.pi * np.arange(n) / n)).reshape(shape)
+    return out
+
+
+def chan_precond_weights(q, dims, lam):
+    """Weights of M^-1 = F^H diag(w) F (unnormalised DFT F, so w includes the
+    1/prod(dims) of the inverse): eigenvalues of chan_circulant(sum_l Q_l)
+    plus lam times the Laplacian spectrum, floored at 1e-3 of the DC one."""
+    t = np.asarray(q).reshape(np.asarray(q).shape[0], -1).sum(axis=0)
+    c = chan_circulant(t, dims)
+    eig = np.fft.fftn(c).real + lam * periodic_laplacian_eigs(dims)
+    dc = np.fft.fftn(c).real.flat[0]
+    eig = np.maximum(eig, 1e-3 * abs(dc))
+    return 1.0 / (eig * np.prod(dims))
+
+
+def circulant_apply(w, r):
+    """z = IDFT(w * DFT(r)) * prod(dims) with w from chan_precond_weights."""
+    r = np.asarray(r, np.complex128).reshape(w.shape)
+    return np.fft.ifftn(w * np.fft.fftn(r)) * w.size
+
+
+def pcg(apply, precond, b, max_iter, tol=1e-16, best_iterate=True):
+    """Textbook preconditioned CG (the IMP_CG_PRECOND path of imp_toep_cg):
+    rho = Re(r^H z), alpha = rho / Re(p^H A p), p = z + (rho'/rho) p; the stop
+    test and best-iterate tracking use the unpreconditioned |r|."""
+    b = np.asarray(b, np.complex128).reshape(-1)
+    x = np.zeros_like(b)
+    r = b.copy()
+    z = np.asarray(precond(r)).reshape(-1)
+    p = z.copy()
+    rho = np.vdot(r, z).real
+    normb = np.vdot(b, b).real
+    best, xbest = normb, x.copy()
+    hist = []
+    it = 0
+    while it < max_iter:
+        q = np.asarray(apply(p)).reshape(-1)
+        alpha = rho / np.vdot(p, q).real
+        x = x + alpha * p
+        r = r - alpha * q
+        rr = np.vdot(r, r).real
+        z = np.asarray(precond(r)).reshape(-1)
+        rho_new = np.vdot(r, z).real
+        p = z + (rho_new / rho) * p
+        rho = rho_new
+        hist.append(np.sqrt(rr / normb))
+        it += 1
+        if rr < best:
+            best, xbest = rr, x.copy()
+        if np.sqrt(rr / normb) < tol:
+            break
+    return (xbest if best_iterate else x), it, hist
+
+
 def bruteforce_cg(F, FH, R, y, x0, cg_num, vec_dtype=np.complex128):
     """bruteForceGpu's CG (bruteForce/bruteForceGpu.cu:470-654):
     cg_num - 1 iterations from x0 with grad = F^H(y - F x) - R x,
