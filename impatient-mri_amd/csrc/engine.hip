@@ -473,6 +473,54 @@ extern "C" int imp_toep_set_precond(imp_toep *op, const float *w) {
     return IMP_OK;
 }
 
+namespace {
+__device__ __forceinline__ float signed_freq(int k, int n) {
+    return (float)(k < (n + 1) / 2 ? k : k - n) / (float)n;
+}
+// radial bin of frequency i (natural DFT order), rho = |k/N| in [0, rmax]
+__global__ void k_precond_radial(const float *chan, const float *gains, int nbins, int N1,
+                                 int N2, int N3, float *w) {
+    const long n = (long)N1 * N2 * N3;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int k3 = (int)(i % N3), k2 = (int)((i / N3) % N2), k1 = (int)(i / ((long)N3 * N2));
+    const float f1 = signed_freq(k1, N1), f2 = signed_freq(k2, N2);
+    const float f3 = N3 > 1 ? signed_freq(k3, N3) : 0.f;
+    const int dims = 2 + (N3 > 1 ? 1 : 0);
+    const float rmax = 0.5f * sqrtf((float)dims);
+    const float rho = sqrtf(f1 * f1 + f2 * f2 + f3 * f3);
+    const int b = min(nbins - 1, (int)(rho / rmax * nbins));
+    w[i] = chan[i] * gains[b];
+}
+}  // namespace
+
+extern "C" int imp_toep_set_precond_radial(imp_toep *op, const float *gains, int nbins) {
+    if (!op || !op->ctx || !gains || nbins < 1 || nbins > 4096)
+        return fail(IMP_EINVAL, "imp_toep_set_precond_radial: bad arguments");
+    for (int b = 0; b < nbins; ++b)
+        if (!(gains[b] > 0.f && std::isfinite(gains[b])))
+            return fail(IMP_EINVAL, "imp_toep_set_precond_radial: gain %d is %g", b, gains[b]);
+    ToepPlan &P = op->P;
+    imp_ctx *ctx = op->ctx;
+    IMP_TRY(ensure_dscratch(ctx, (size_t)(nbins + 1) / 2 + 16));
+    float *dg = (float *)ctx->dscratch;
+    IMP_HIP(hipMemcpyAsync(dg, gains, nbins * sizeof(float), hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(k_precond_radial, dim3(ceil_div(P.nvox, 256)), dim3(256), 0, ctx->stream,
+                       P.pw_chan, dg, nbins, P.N1, P.N2, P.N3, P.pw);
+    IMP_CHECK_LAUNCH();
+    IMP_HIP(hipStreamSynchronize(ctx->stream));
+    return IMP_OK;
+}
+
+extern "C" int imp_toep_set_fieldmap(imp_toep *op, const float *fm) {
+    if (!op || !op->ctx) return fail(IMP_EINVAL, "imp_toep_set_fieldmap: bad operator");
+    ToepPlan &P = op->P;
+    hipLaunchKernelGGL(k_fm_to_cycles, dim3(ceil_div(P.nvox, 256)), dim3(256), 0,
+                       op->ctx->stream, fm, P.fcyc, P.nvox);
+    IMP_CHECK_LAUNCH();
+    return IMP_OK;
+}
+
 extern "C" int imp_toep_get_precond(imp_toep *op, float *w) {
     if (!op || !op->ctx || !w) return fail(IMP_EINVAL, "imp_toep_get_precond: bad arguments");
     IMP_HIP(hipMemcpyAsync(w, op->P.pw, (size_t)op->P.nvox * sizeof(float),

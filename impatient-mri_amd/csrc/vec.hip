@@ -460,10 +460,21 @@ int toep_cg(imp_ctx *ctx, ToepPlan &P, const float2 *b, float2 *x, int max_iter,
     }
     // r = b - A x0 (x0 = 0: r = b, the reference's wasted apply at
     // recon.cu:2036 is skipped since A 0 = 0)
-    if (flags & IMP_CG_USE_X0) {
+    bool from_x0 = (flags & IMP_CG_USE_X0) != 0;
+    double start_res = -1.0;
+    if (from_x0) {
         IMP_TRY(toep_apply_full(ctx, P, x, P.vq));
         hipLaunchKernelGGL(k_sub_c, g1, b1, 0, ctx->stream, b, P.vq, P.vr, n);
-    } else {
+        IMP_CHECK_LAUNCH();
+        if (flags & IMP_CG_FALLBACK_ZERO) {
+            // the zero start has residual |b|: keep x0 only if it does better
+            double r0 = 0.0;
+            IMP_TRY(dot_re(ctx, P.vr, P.vr, n, &r0));
+            start_res = std::sqrt(r0 / bnorm);
+            if (!(r0 < bnorm)) from_x0 = false;
+        }
+    }
+    if (!from_x0) {
         IMP_HIP(hipMemsetAsync(x, 0, n * sizeof(float2), ctx->stream));
         IMP_HIP(hipMemcpyAsync(P.vr, b, n * sizeof(float2), hipMemcpyDeviceToDevice, ctx->stream));
     }
@@ -530,6 +541,8 @@ int toep_cg(imp_ctx *ctx, ToepPlan &P, const float2 *b, float2 *x, int max_iter,
         stats->rel_residual = std::sqrt(beta2 / bnorm);
         stats->best_rel_residual = std::sqrt(best_res / bnorm);
         stats->true_rel_residual = -1.0;
+        stats->start_used = from_x0 ? 1 : 0;
+        stats->start_rel_residual = start_res;
     }
     if (flags & IMP_CG_TRUE_RESIDUAL) {
         IMP_TRY(toep_apply_full(ctx, P, x, P.vq));

@@ -27,6 +27,7 @@ IMP_CG_BEST_ITERATE = 0x1
 IMP_CG_USE_X0 = 0x2
 IMP_CG_TRUE_RESIDUAL = 0x4
 IMP_CG_PRECOND = 0x8
+IMP_CG_FALLBACK_ZERO = 0x10
 IMP_COMPAT_Q_CUMULATIVE = 0x1
 
 
@@ -40,7 +41,8 @@ class CGStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("converged", C.c_int),
                 ("best_iteration", C.c_int), ("rel_residual", C.c_double),
                 ("best_rel_residual", C.c_double), ("true_rel_residual", C.c_double),
-                ("seconds", C.c_double)]
+                ("seconds", C.c_double), ("start_used", C.c_int),
+                ("start_rel_residual", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -10,7 +10,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (CGStats, Context, DeviceArray, IMP_CG_BEST_ITERATE, call, ptr)
+from ._lib import (CGStats, Context, DeviceArray, IMP_CG_BEST_ITERATE, call, load, ptr)
 
 
 def _f32(a):
@@ -219,3 +219,72 @@ def bruteforce_cg(ctx: Context, kx, ky, t, x, y, fm, d, sens, cg_num, lam, x0):
          ptr(a[1]), ptr(a[2]), ptr(a[3]), ptr(a[4]), ptr(afm), ptr(dr), ptr(di), ptr(sr),
          ptr(si), C.c_int(cg_num), C.c_float(lam), ptr(xr), ptr(xi))
     return _cplx(xr, xi)
+
+
+class CNN:
+    """Learned warm-start network (imp_cnn_*): x = z + s * net(z / s)."""
+
+    def __init__(self, ctx: Context, H: int, W: int, seed: int = 1):
+        self.ctx, self.H, self.W = ctx, H, W
+        self.n = H * W
+        h = C.c_void_p()
+        call("imp_cnn_create", ctx.h, C.c_int(H), C.c_int(W), C.c_ulonglong(seed), C.byref(h))
+        self.h = h
+        lib = load()
+        lib.imp_cnn_num_params.restype = C.c_int
+        self.nparam = int(lib.imp_cnn_num_params())
+        self._z = _pair(ctx, self.n)
+        self._t = _pair(ctx, self.n)
+        self._x = _pair(ctx, self.n)
+
+    def close(self):
+        if getattr(self, "h", None):
+            call("imp_cnn_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_params(self):
+        out = np.zeros(self.nparam, np.float32)
+        call("imp_cnn_get_params", self.h, out.ctypes.data_as(C.c_void_p), C.c_int(self.nparam))
+        return out
+
+    def set_params(self, params):
+        params = _f32(np.asarray(params).reshape(-1))
+        call("imp_cnn_set_params", self.h, params.ctypes.data_as(C.c_void_p),
+             C.c_int(self.nparam))
+
+    def _load(self, pair, v):
+        v = np.asarray(v).reshape(-1)
+        pair[0].upload(_f32(v.real))
+        pair[1].upload(_f32(v.imag))
+
+    def forward(self, z):
+        self._load(self._z, z)
+        call("imp_cnn_forward", self.h, ptr(self._z[0]), ptr(self._z[1]), ptr(self._x[0]),
+             ptr(self._x[1]))
+        return _cplx(*self._x).reshape(self.H, self.W)
+
+    def gradient(self, z, t):
+        self._load(self._z, z)
+        self._load(self._t, t)
+        g = np.zeros(self.nparam, np.float32)
+        loss = C.c_double()
+        call("imp_cnn_gradient", self.h, ptr(self._z[0]), ptr(self._z[1]), ptr(self._t[0]),
+             ptr(self._t[1]), g.ctypes.data_as(C.c_void_p), C.byref(loss))
+        return loss.value, g
+
+    def train_step_device(self, zr, zi, tr, ti, lr):
+        """One Adam step on device-resident (z, t) (DeviceArray re/im pairs)."""
+        loss = C.c_double()
+        call("imp_cnn_train_step", self.h, ptr(zr), ptr(zi), ptr(tr), ptr(ti), C.c_float(lr),
+             C.byref(loss))
+        return loss.value
+
+    def forward_device(self, zr, zi, xr, xi):
+        call("imp_cnn_forward", self.h, ptr(zr), ptr(zi), ptr(xr), ptr(xi))
+

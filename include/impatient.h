@@ -173,6 +173,11 @@ int imp_toep_create(imp_ctx *ctx, int N1, int N2, int N3, int ncoils,
 int imp_toep_destroy(imp_toep *op);
 /* Replace the TV weight W of lambda R (device [N], NULL = identity). */
 int imp_toep_set_weight(imp_toep *op, const float *w);
+/* Replace the field map (device [N], rad/s, NULL = 0); Q_l and the
+ * preconditioner do not depend on it, so one plan serves every field map of
+ * a trajectory (fMRI frames, training draws).  No counterpart in the
+ * reference, which rebuilds everything per call (main_mri.cu:359-620). */
+int imp_toep_set_fieldmap(imp_toep *op, const float *fm);
 /* out = A p (device pointers, [N] each). */
 int imp_toep_apply(imp_toep *op, const float *p_r, const float *p_i,
                    float *out_r, float *out_i);
@@ -185,6 +190,8 @@ typedef struct {
     double best_rel_residual; /* same, for the returned iterate              */
     double true_rel_residual; /* ||A x - b|| / ||b|| recomputed (flag)       */
     double seconds;           /* wall time of the solve                      */
+    int    start_used;        /* 1: started from x0; 0: from zero            */
+    double start_rel_residual;/* ||b - A x0|| / ||b|| of the given x0 (-1: none) */
 } imp_cg_stats;
 
 #define IMP_CG_BEST_ITERATE   0x1  /* return argmin-residual iterate (recon.cu:2107-2110) */
@@ -192,6 +199,8 @@ typedef struct {
 #define IMP_CG_TRUE_RESIDUAL  0x4  /* recompute ||Ax-b||/||b|| after the solve  */
 #define IMP_CG_PRECOND        0x8  /* preconditioned CG with the operator's
                                       circulant M^-1 (imp_toep_set_precond)   */
+#define IMP_CG_FALLBACK_ZERO  0x10 /* with USE_X0: start from 0 instead when
+                                      ||b - A x0|| >= ||b|| (stats report it)  */
 
 /* CG on A x = b (toeplitz_recon's loop, recon.cu:2011-2133).  All scalars
  * (dot products, alpha, beta) are FP64 and stay on the device; tol is on
@@ -211,9 +220,38 @@ int imp_toep_cg(imp_toep *op, const float *b_r, const float *b_i,
  * reference, whose CG is unpreconditioned (recon.cu:2011-2133). */
 int imp_toep_set_precond(imp_toep *op, const float *w);
 int imp_toep_get_precond(imp_toep *op, float *w);
+/* w = w_chan * g(rho): Chan weights times a radial gain profile g given at
+ * nbins equally spaced radii rho = |k / N| in [0, sqrt(dims)/2] (host array,
+ * piecewise constant per bin, every gain > 0 and finite). */
+int imp_toep_set_precond_radial(imp_toep *op, const float *gains, int nbins);
 /* z = M^-1 r (device, interleaved Re/Im split as r_r/r_i -> z_r/z_i). */
 int imp_toep_precond_apply(imp_toep *op, const float *r_r, const float *r_i, float *z_r,
                            float *z_i);
+
+/* ------------------------------------------------ learned warm start
+ * A 6-layer 3x3 residual CNN (32 channels, bf16 weights and activations,
+ * FP32 accumulation on v_mfma_f32_32x32x16_bf16) with its own inference and
+ * training kernels.  Input z is a linear estimate of the solution (e.g.
+ * M^-1 b); with s = rms(z) the network returns x = z + s * net(z / s).
+ * Images are H x W (device, split Re/Im, row-major), W a multiple of 64.
+ * Parameters: imp_cnn_num_params() floats, per layer [9][32 co][32 ci]
+ * weights then 32 biases (FP32 masters; bf16 copies feed the MFMA).
+ * No counterpart in the reference (its CG starts from zero, recon.cu:1871). */
+typedef struct imp_cnn imp_cnn;
+int imp_cnn_create(imp_ctx *ctx, int H, int W, unsigned long long seed, imp_cnn **out);
+int imp_cnn_destroy(imp_cnn *net);
+int imp_cnn_num_params(void);
+int imp_cnn_get_params(imp_cnn *net, float *host, int n);
+int imp_cnn_set_params(imp_cnn *net, const float *host, int n);
+/* x = z + s * net(z / s) */
+int imp_cnn_forward(imp_cnn *net, const float *z_r, const float *z_i, float *x_r, float *x_i);
+/* loss = 0.5 |x - t|^2 / |t|^2 (x as in imp_cnn_forward) and its gradient
+ * with respect to the parameters (host, imp_cnn_num_params floats, or NULL) */
+int imp_cnn_gradient(imp_cnn *net, const float *z_r, const float *z_i, const float *t_r,
+                     const float *t_i, float *grad_host, double *loss);
+/* one Adam step (beta 0.9 / 0.999) on the pair (z, t) */
+int imp_cnn_train_step(imp_cnn *net, const float *z_r, const float *z_i, const float *t_r,
+                       const float *t_i, float lr, double *loss);
 
 /* ---------------------------------------------- brute-force CG (a11)
  * bruteForceGpu semantics (bruteForce/bruteForceGpu.cu:142-813): Fessler CG

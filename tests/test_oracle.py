@@ -413,3 +413,60 @@ def test_pcg_matches_solve_and_needs_fewer_iterations():
     xc, itc, hc = orc.toeplitz_cg(A, b, 500, tol=1e-10)
     assert rel_l2(xp, xs) < 1e-8 and rel_l2(xc, xs) < 1e-8
     assert itp < 0.7 * itc, (itp, itc)
+
+
+# --------------------------------------------------------- learned start
+def test_cnn_oracle_backward_matches_finite_differences():
+    """The oracle's backward pass (FP64, no bf16 rounding) is the derivative
+    of its forward loss: central differences on a sample of parameters from
+    every layer, weights and biases."""
+    from oracle import cnn
+    rng = np.random.default_rng(31)
+    p = cnn.random_params(3)
+    z = rng.standard_normal((5, 7)) + 1j * rng.standard_normal((5, 7))
+    t = z + 0.3 * (rng.standard_normal((5, 7)) + 1j * rng.standard_normal((5, 7)))
+    loss, g = cnn.loss_and_grad(p, z, t, bf16_round=False)
+    idx = np.flatnonzero(cnn.trainable_mask())
+    pick = rng.choice(idx, 40, replace=False)
+    pick = np.concatenate([pick, [l * cnn.PSZ + cnn.WSZ for l in range(cnn.NLAYER)]])
+    h = 1e-6
+    for i in pick:
+        pp, pm = p.copy(), p.copy()
+        pp[i] += h
+        pm[i] -= h
+        lp, _ = cnn.loss_and_grad(pp, z, t, bf16_round=False)
+        lm, _ = cnn.loss_and_grad(pm, z, t, bf16_round=False)
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i])) + 1e-7, (i, fd, g[i])
+
+
+def test_cnn_oracle_conv_is_correlation_and_adjoint():
+    """conv3x3 against a direct per-pixel loop, and <conv(a), g> = <a, bwd(g)>
+    (the backward-data pass is the adjoint of the convolution)."""
+    from oracle import cnn
+    rng = np.random.default_rng(32)
+    a = rng.standard_normal((4, 6, cnn.CH))
+    W = rng.standard_normal((9, cnn.CH, cnn.CH))
+    out = cnn.conv3x3(a, W)
+    y, x, co = 0, 5, 3
+    ref = 0.0
+    for tap in range(9):
+        yy, xx = y + tap // 3 - 1, x + tap % 3 - 1
+        if 0 <= yy < 4 and 0 <= xx < 6:
+            ref += a[yy, xx] @ W[tap, co]
+    assert np.isclose(out[y, x, co], ref)
+    g = rng.standard_normal(out.shape)
+    assert np.isclose(np.sum(out * g), np.sum(a * cnn.conv3x3_bwd_data(g, W)))
+    dW, _ = cnn.wgrad(a, g)
+    # <conv(a; W), g> is linear in W with gradient dW
+    assert np.isclose(np.sum(out * g), np.sum(W * dW))
+
+
+def test_bf16_rounding_is_nearest_even():
+    from oracle import cnn
+    one = np.float32(1.0)
+    ulp = 2.0 ** -7
+    assert cnn.bf16(np.float32(1.0 + ulp / 2)) == 1.0          # tie -> even
+    assert cnn.bf16(np.float32(1.0 + 3 * ulp / 2)) == 1.0 + 2 * ulp
+    assert cnn.bf16(np.float32(1.0 + ulp / 2 + 2 ** -20)) == 1.0 + ulp
+    assert cnn.bf16(one) == 1.0 and cnn.bf16(np.float32(-3.0)) == -3.0
