@@ -1,0 +1,121 @@
+"""GPU parity at the full sizes of the workload configurations.
+
+The FP64 oracle is too slow for whole images at these sizes, so every
+operator output is compared at a seeded sample of output points (the direct
+sums are independent per output point), and the 3D Toeplitz apply of the
+64x64x16 configuration against the full FP64 oracle apply.  Tolerances as
+for the small cases: relative L2 < 2e-6 for operators.
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+OP_TOL = 2e-6
+
+
+def _pick(n, k, seed):
+    return np.sort(np.random.default_rng(seed).choice(n, k, replace=False))
+
+
+def _data(ctx, p, snr_db=40.0):
+    from impatient import ops, synth
+    d = ops.nudft_forward(ctx, p.kx, p.ky, p.kz, p.t, p.x, p.y, p.z, p.fm, p.rho, p.sens,
+                          p.ncoils)
+    return synth.add_noise(np.asarray(d).reshape(p.ncoils, -1), snr_db, p.meta["seed"])
+
+
+@pytest.mark.parametrize("idx,nvox_pick", [(1, 48), (3, 12)])
+def test_adjoint_at_config_size(ctx, idx, nvox_pick):
+    """F^H d of configurations 1 (64x64x16, 4 coils, stack of spirals) and 3
+    (512x512, 32 coils) at sampled voxels."""
+    from impatient import ops, synth
+    from oracle import oracle as orc
+    p = synth.config(idx)
+    d = _data(ctx, p)
+    got = ops.nudft_adjoint(ctx, p.kx, p.ky, p.kz, p.t, p.x, p.y, p.z, p.fm, d, p.sens)
+    v = _pick(p.nvox, nvox_pick, idx)
+    z = None if p.z is None else p.z[v]
+    sens = None if p.sens is None else p.sens[:, v]
+    ref = orc.nudft_adjoint(p.kx, p.ky, p.kz, p.t, p.x[v], p.y[v], z, p.fm[v], d, sens)
+    err = rel_l2(np.asarray(got).reshape(-1)[v], ref)
+    print(f"{p.name} F^H d at {nvox_pick} voxels: rel_l2={err:.2e}")
+    assert err < OP_TOL
+
+
+def test_forward_at_config1_size(ctx):
+    from impatient import ops, synth
+    from oracle import oracle as orc
+    p = synth.config(1)
+    got = np.asarray(ops.nudft_forward(ctx, p.kx, p.ky, p.kz, p.t, p.x, p.y, p.z, p.fm, p.rho,
+                                       p.sens, p.ncoils)).reshape(p.ncoils, -1)
+    k = _pick(p.nk, 64, 11)
+    ref = orc.nudft_forward(p.kx[k], p.ky[k], p.kz[k], p.t[k], p.x, p.y, p.z, p.fm, p.rho,
+                            p.sens)
+    err = rel_l2(got[:, k], ref)
+    print(f"{p.name} F rho at 64 samples x {p.ncoils} coils: rel_l2={err:.2e}")
+    assert err < OP_TOL
+
+
+def test_q_at_config1_size(ctx):
+    """Q_l, l = 0..6, on the 128x128x32 grid of configuration 1 at sampled
+    grid points (explicit-grid oracle at those points)."""
+    from impatient import ops, synth
+    from oracle import oracle as orc
+    p = synth.config(1)
+    L = p.ntime_segs
+    got = ops.toeplitz_q(ctx, p.kx, p.ky, p.kz, p.t, p.Nx, p.Ny, p.Nz, L)
+    gx, gy, gz = orc.q_grid(p.Nx, p.Ny, p.Nz)
+    g = _pick(len(gx), 96, 12)
+    ref = orc.toeplitz_q(p.kx, p.ky, p.kz, p.t, p.Nx, p.Ny, p.Nz, L, grid=(gx[g], gy[g], gz[g]))
+    err = rel_l2(np.asarray(got)[:, g], ref)
+    print(f"{p.name} Q ({L + 1} segments) at 96 grid points: rel_l2={err:.2e}")
+    assert err < OP_TOL
+
+
+def test_toeplitz_apply_at_config1_size(ctx):
+    """The 3D time-segmented SENSE Toeplitz apply (64x64x16, 4 coils, L = 6,
+    lambda 1e-3) against the full FP64 oracle apply."""
+    from impatient import ops, synth
+    from oracle import oracle as orc
+    p = synth.config(1)
+    L = p.ntime_segs
+    q = ops.toeplitz_q(ctx, p.kx, p.ky, p.kz, p.t, p.Nx, p.Ny, p.Nz, L)
+    tau, t0 = orc.segment_timing(p.t, L)
+    op = ops.Toeplitz(ctx, p.shape, q, tau, t0, p.fm, p.sens, p.lam)
+    rng = np.random.default_rng(13)
+    v = (rng.standard_normal(p.nvox) + 1j * rng.standard_normal(p.nvox)).astype(np.complex64)
+    got = op.apply(v)
+    ref = orc.toeplitz_apply(v.reshape(p.shape), np.asarray(q, np.complex128), tau, t0, p.fm,
+                             p.sens, p.lam).reshape(-1)
+    err = rel_l2(got, ref)
+    print(f"{p.name} Toeplitz apply: rel_l2={err:.2e}")
+    assert err < OP_TOL
+
+
+def test_config1_solves_to_tolerance(ctx):
+    """Configuration 1's reconstruction (20 CG iterations from F^H d) and the
+    preconditioned solve to 1e-5: recomputed true residuals, and the PCG image
+    against the plain-CG image run to the same residual."""
+    from impatient import ops, synth, IMP_CG_BEST_ITERATE, IMP_CG_PRECOND, IMP_CG_TRUE_RESIDUAL
+    from oracle import oracle as orc
+    p = synth.config(1)
+    L = p.ntime_segs
+    d = _data(ctx, p)
+    b = ops.nudft_adjoint(ctx, p.kx, p.ky, p.kz, p.t, p.x, p.y, p.z, p.fm, d, p.sens)
+    q = ops.toeplitz_q(ctx, p.kx, p.ky, p.kz, p.t, p.Nx, p.Ny, p.Nz, L)
+    tau, t0 = orc.segment_timing(p.t, L)
+    op = ops.Toeplitz(ctx, p.shape, q, tau, t0, p.fm, p.sens, p.lam)
+    fl = IMP_CG_BEST_ITERATE | IMP_CG_TRUE_RESIDUAL
+    x20, s20 = op.cg(b, p.cg_iters, 0.0, fl)
+    xc, sc = op.cg(b, 2000, 1e-5, fl)
+    xp, sp = op.cg(b, 2000, 1e-5, fl | IMP_CG_PRECOND)
+    print(f"{p.name}: 20 it true res {s20['true_rel_residual']:.3e}; CG to 1e-5 "
+          f"{sc['iterations']} it; PCG {sp['iterations']} it (true {sp['true_rel_residual']:.2e});"
+          f" |xp - xc| / |xc| = {rel_l2(xp, xc):.2e}")
+    assert s20["iterations"] == p.cg_iters and s20["true_rel_residual"] < 1.0
+    assert sc["converged"] == 1 and sp["converged"] == 1
+    assert sp["true_rel_residual"] < 1e-5 and sp["iterations"] <= sc["iterations"]
+    assert rel_l2(xp, xc) < 1e-2
