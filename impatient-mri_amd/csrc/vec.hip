@@ -339,6 +339,21 @@ __global__ void k_sub_c(const float2 *a, const float2 *b, float2 *out, long n) {
     if (i < n) out[i] = make_float2(a[i].x - b[i].x, a[i].y - b[i].y);
 }
 
+// owns an instantiated CG iteration graph
+struct CgGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~CgGraph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+__global__ void k_copy_c(const float2 *__restrict__ a, float2 *__restrict__ b, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+
 int red_blocks(long n) {
     long nb = (n + VT * 4 - 1) / (VT * 4);
     if (nb < 1) nb = 1;
@@ -424,8 +439,10 @@ int precond_chan(imp_ctx *ctx, ToepPlan &P, const float *q_r, const float *q_i) 
 int precond_apply(imp_ctx *ctx, ToepPlan &P, const float2 *r, float2 *z) {
     const long n = P.nvox;
     const int rec = prof_begin(ctx, "precond_apply");
-    if (z != r)
-        IMP_HIP(hipMemcpyAsync(z, r, n * sizeof(float2), hipMemcpyDeviceToDevice, ctx->stream));
+    if (z != r) {
+        hipLaunchKernelGGL(k_copy_c, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, r, z, n);
+        IMP_CHECK_LAUNCH();
+    }
     IMP_TRY(fft_grid(ctx, z, P.N1, P.N2, P.N3, false));
     hipLaunchKernelGGL(k_scale_by_w, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, z, P.pw,
                        n);
@@ -496,10 +513,9 @@ int toep_cg(imp_ctx *ctx, ToepPlan &P, const float2 *b, float2 *x, int max_iter,
                                ctx->stream));
     }
 
-    int iter = 0, best_iter = 0;
-    bool converged = false;
-    double beta2 = 0.0;
-    while (iter < max_iter) {
+    // one iteration: every launch is a kernel on ctx->stream with device
+    // scalars, so the sequence can be captured into a HIP graph
+    auto iteration = [&]() -> int {
         IMP_TRY(toep_apply_full(ctx, P, P.vp, P.vq));
         hipLaunchKernelGGL(k_dot_re, dim3(nb), dim3(VT), 0, ctx->stream, P.vp, P.vq, n, partials);
         hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(VT), 0, ctx->stream, partials, nb, sc);
@@ -516,6 +532,33 @@ int toep_cg(imp_ctx *ctx, ToepPlan &P, const float2 *b, float2 *x, int max_iter,
         hipLaunchKernelGGL(k_cg_update_p, g1, b1, 0, ctx->stream, P.vp, pc ? P.vz : P.vr, x,
                            P.vbest, n, sc, best ? 1 : 0);
         IMP_CHECK_LAUNCH();
+        return IMP_OK;
+    };
+    CgGraph graph;
+    if (flags & IMP_CG_GRAPH) {
+        // kernel timer events are not recorded into the graph
+        const bool prof = ctx->prof_on;
+        ctx->prof_on = false;
+        IMP_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        const int st_cap = iteration();
+        hipGraph_t g = nullptr;
+        const hipError_t e_end = hipStreamEndCapture(ctx->stream, &g);
+        ctx->prof_on = prof;
+        graph.graph = g;
+        if (st_cap != IMP_OK) return st_cap;
+        if (e_end != hipSuccess) return fail(IMP_EHIP, "CG graph capture: %s", hipGetErrorString(e_end));
+        IMP_HIP(hipGraphInstantiate(&graph.exec, graph.graph, nullptr, nullptr, 0));
+    }
+
+    int iter = 0, best_iter = 0;
+    bool converged = false;
+    double beta2 = 0.0;
+    while (iter < max_iter) {
+        if (graph.exec) {
+            IMP_HIP(hipGraphLaunch(graph.exec, ctx->stream));
+        } else {
+            IMP_TRY(iteration());
+        }
         IMP_HIP(hipMemcpyAsync(ctx->host_scalars, sc, SC_N * sizeof(double),
                                hipMemcpyDeviceToHost, ctx->stream));
         IMP_HIP(hipStreamSynchronize(ctx->stream));

@@ -28,6 +28,7 @@ IMP_CG_USE_X0 = 0x2
 IMP_CG_TRUE_RESIDUAL = 0x4
 IMP_CG_PRECOND = 0x8
 IMP_CG_FALLBACK_ZERO = 0x10
+IMP_CG_GRAPH = 0x20
 IMP_COMPAT_Q_CUMULATIVE = 0x1
 
 

@@ -201,6 +201,9 @@ typedef struct {
                                       circulant M^-1 (imp_toep_set_precond)   */
 #define IMP_CG_FALLBACK_ZERO  0x10 /* with USE_X0: start from 0 instead when
                                       ||b - A x0|| >= ||b|| (stats report it)  */
+#define IMP_CG_GRAPH          0x20 /* capture one iteration (every kernel of the
+                                      apply, dots, scalar updates, M^-1) into a
+                                      HIP graph and replay it per iteration    */
 
 /* CG on A x = b (toeplitz_recon's loop, recon.cu:2011-2133).  All scalars
  * (dot products, alpha, beta) are FP64 and stay on the device; tol is on

@@ -117,3 +117,26 @@ def test_pcg_converges_in_fewer_iterations(ctx):
     assert sp["true_rel_residual"] < 1e-5
     assert sp["iterations"] < 0.5 * sc["iterations"]
     assert rel_l2(xp, xc) < 0.1
+
+
+@pytest.mark.parametrize("pc", [False, True])
+def test_graph_captured_iterations_match_eager(ctx, pc):
+    """IMP_CG_GRAPH replays one captured iteration per CG step (with the
+    host read-back and synchronisation between replays); every replay must
+    reproduce the eager iteration exactly: same iterate, same residual
+    history end point, same iteration count."""
+    from impatient import (ops, IMP_CG_BEST_ITERATE, IMP_CG_GRAPH, IMP_CG_PRECOND,
+                           IMP_CG_TRUE_RESIDUAL)
+    from oracle import oracle as orc
+    p, q, tau, t0 = _setup(32, 1, 3, 4)
+    op = ops.Toeplitz(ctx, p.shape, q.astype(np.complex64), tau, t0, p.fm, p.sens, 1e-2)
+    y = orc.nudft_forward(p.kx, p.ky, None, p.t, p.x, p.y, None, p.fm, p.rho, p.sens)
+    b = orc.nudft_adjoint(p.kx, p.ky, None, p.t, p.x, p.y, None, p.fm, y, p.sens)
+    fl = IMP_CG_BEST_ITERATE | IMP_CG_TRUE_RESIDUAL | (IMP_CG_PRECOND if pc else 0)
+    xe, se = op.cg(b.astype(np.complex64), 60, 1e-6, fl)
+    xg, sg = op.cg(b.astype(np.complex64), 60, 1e-6, fl | IMP_CG_GRAPH)
+    print(f"pc={pc}: eager {se['iterations']} it res {se['rel_residual']:.3e}; "
+          f"graph {sg['iterations']} it res {sg['rel_residual']:.3e}")
+    assert se["iterations"] == sg["iterations"] and se["iterations"] > 5
+    assert se["rel_residual"] == sg["rel_residual"]
+    assert np.array_equal(xe, xg)
