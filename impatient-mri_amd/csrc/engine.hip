@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "common.h"
+#include "fftcore.h"
 #include "toeplitz.h"
 
 namespace imp {
@@ -362,8 +363,9 @@ extern "C" int imp_toep_create(imp_ctx *ctx, int N1, int N2, int N3, int ncoils,
         return fail(IMP_EINVAL, "imp_toep_create: bad arguments");
     if (!toep_sizes_supported(N1, N2, N3))
         return fail(IMP_EUNSUPPORTED,
-                    "imp_toep_create: 2x grid (%d,%d,%d) needs power-of-two lengths 2..2048",
-                    2 * N1, 2 * N2, N3 > 1 ? 2 * N3 : 1);
+                    "imp_toep_create: image (%d,%d,%d) needs lengths 2^k or 3 * 2^k whose "
+                    "doubles are at most 2048",
+                    N1, N2, N3);
     *out = nullptr;
     imp_toep *op = new (std::nothrow) imp_toep();
     if (!op) return fail(IMP_ENOMEM, "imp_toep_create: host allocation");
@@ -384,10 +386,10 @@ extern "C" int imp_toep_create(imp_ctx *ctx, int N1, int N2, int N3, int ncoils,
     P.t0 = t0;
     P.lambda = lambda;
     // coil groups for pass E: enough workgroups to fill the GPU (pass E
-    // holds 4096 / Ml image lines per workgroup, 256 when Ml < 16)
+    // holds 256 / (threads per line) image lines per workgroup)
     {
         const int Ml = N3 > 1 ? P.M3 : P.M2;
-        const int nl = Ml >= 16 ? 4096 / Ml : 256;
+        const int nl = 256 / fftc::threads_per_line(Ml);
         const long wgs_per_group = (long)ceil_div((long)N1 * (N3 > 1 ? N2 : 1), nl);
         int groups = 1;
         while (groups < ncoils && wgs_per_group * groups < 4L * ctx->num_cus) groups *= 2;

@@ -28,6 +28,18 @@ namespace {
 
 using fftc::cmul;
 
+// Stages S .. num_stages - 2, each LDS -> LDS (in place, so a barrier
+// between its loads and its stores and one after its stores).
+template <int L, int S, bool INV, class LLD, class LST>
+__device__ __forceinline__ void middle_stages(int tl, const float2 *__restrict__ tw, LLD lld,
+                                              LST lst) {
+    if constexpr (S < fftc::num_stages(L) - 1) {
+        fftc::stage<L, S, INV, false>(tl, tw, lld, [] { __syncthreads(); }, lst);
+        __syncthreads();
+        middle_stages<L, S + 1, INV>(tl, tw, lld, lst);
+    }
+}
+
 // Runs all stages of one transform.  FIRST / LAST are the first stage's
 // loads and the last stage's stores (global memory or registers); FMID runs
 // before the first stage's stores (a barrier when the LDS lines may still be
@@ -37,18 +49,13 @@ template <int L, bool INV, class FIRST, class FMID, class LAST, class LLD, class
 __device__ __forceinline__ void run_fft(int tl, const float2 *__restrict__ tw, FIRST first,
                                         FMID fmid, LAST last, LLD lld, LST lst) {
     constexpr int S = fftc::num_stages(L);
-    static_assert(S >= 1 && S <= 3, "lengths up to 4096");
     auto nop = [] {};
-    auto bar = [] { __syncthreads(); };
     if constexpr (S == 1) {
         fftc::stage<L, 0, INV, true>(tl, tw, first, fmid, last);
     } else {
         fftc::stage<L, 0, INV, true>(tl, tw, first, fmid, lst);
         __syncthreads();
-        if constexpr (S == 3) {
-            fftc::stage<L, 1, INV, false>(tl, tw, lld, bar, lst);
-            __syncthreads();
-        }
+        middle_stages<L, 1, INV>(tl, tw, lld, lst);
         fftc::stage<L, S - 1, INV, true>(tl, tw, lld, nop, last);
     }
 }
@@ -300,28 +307,42 @@ __global__ void k_ifftshift_q(const float *__restrict__ qr, const float *__restr
         case 512: { constexpr int L_ = 512; BODY; } break;   \
         case 1024: { constexpr int L_ = 1024; BODY; } break; \
         case 2048: { constexpr int L_ = 2048; BODY; } break; \
+        case 24: { constexpr int L_ = 24; BODY; } break;     \
+        case 48: { constexpr int L_ = 48; BODY; } break;     \
+        case 96: { constexpr int L_ = 96; BODY; } break;     \
+        case 192: { constexpr int L_ = 192; BODY; } break;   \
+        case 384: { constexpr int L_ = 384; BODY; } break;   \
+        case 768: { constexpr int L_ = 768; BODY; } break;   \
+        case 1536: { constexpr int L_ = 1536; BODY; } break; \
         default: return fail(IMP_EUNSUPPORTED, "FFT length %d unsupported", Lv); \
     }
 
-bool fft_len_ok(int L) { return L >= 2 && L <= 2048 && (L & (L - 1)) == 0; }
+// lengths with an instantiated kernel set: 2^k (2..2048), 3 * 2^k (24..1536)
+bool fft_len_ok(int L) {
+    return fftc::fft_length_supported(L) && L <= 2048 && (L % 3 != 0 || L >= 24);
+}
 
 }  // namespace
 
 // ------------------------------------------------------------ twiddles
 int twiddles(imp_ctx *ctx, int L, const float2 **out) {
-    // table for length L lives at offset L of a 4096-entry buffer
+    // table for length L: powers of two at offset L, lengths 3 * 2^k at
+    // offset 4096 + L (8192 entries, FP64 values rounded once)
     if (!fft_len_ok(L)) return fail(IMP_EUNSUPPORTED, "FFT length %d unsupported", L);
     if (!ctx->twiddle) {
-        std::vector<float2> h(4096, make_float2(0.f, 0.f));
-        for (int n = 2; n <= 2048; n *= 2)
+        std::vector<float2> h(8192, make_float2(0.f, 0.f));
+        auto fill = [&](int n, int off) {
             for (int m = 0; m < n; ++m) {
                 const double a = -2.0 * M_PI * (double)m / (double)n;
-                h[n + m] = make_float2((float)cos(a), (float)sin(a));
+                h[off + m] = make_float2((float)cos(a), (float)sin(a));
             }
-        IMP_HIP(hipMalloc(&ctx->twiddle, 4096 * sizeof(float2)));
-        IMP_HIP(hipMemcpy(ctx->twiddle, h.data(), 4096 * sizeof(float2), hipMemcpyHostToDevice));
+        };
+        for (int n = 2; n <= 2048; n *= 2) fill(n, n);
+        for (int n = 24; n <= 1536; n *= 2) fill(n, 4096 + n);
+        IMP_HIP(hipMalloc(&ctx->twiddle, 8192 * sizeof(float2)));
+        IMP_HIP(hipMemcpy(ctx->twiddle, h.data(), 8192 * sizeof(float2), hipMemcpyHostToDevice));
     }
-    *out = ctx->twiddle + L;
+    *out = ctx->twiddle + (L % 3 == 0 ? 4096 + L : L);
     return IMP_OK;
 }
 
@@ -474,9 +495,12 @@ int toep_apply_normal(imp_ctx *ctx, ToepPlan &P, const float2 *p, float2 *out) {
     return IMP_OK;
 }
 
+// The 2x grid (Toeplitz passes) and the image grid (preconditioner) both
+// need instantiated FFT lengths.
 bool toep_sizes_supported(int N1, int N2, int N3) {
     if (!fft_len_ok(2 * N1) || !fft_len_ok(2 * N2)) return false;
-    if (N3 > 1 && !fft_len_ok(2 * N3)) return false;
+    if (!fft_len_ok(N1) || !fft_len_ok(N2)) return false;
+    if (N3 > 1 && (!fft_len_ok(2 * N3) || !fft_len_ok(N3))) return false;
     return true;
 }
 

@@ -138,21 +138,83 @@ __device__ __forceinline__ void dft<16, false>(float2 *v) { dft16<false>(v); }
 template <>
 __device__ __forceinline__ void dft<16, true>(float2 *v) { dft16<true>(v); }
 
+template <bool INV>
+__device__ __forceinline__ void dft3(float2 *v) {
+    constexpr float r3 = 0.86602540378443865f;  // sqrt(3) / 2
+    const float2 s = cadd(v[1], v[2]);
+    const float2 d = rot90<INV>(csub(v[1], v[2]));
+    const float2 t = make_float2(v[0].x - 0.5f * s.x, v[0].y - 0.5f * s.y);
+    v[0] = cadd(v[0], s);
+    v[1] = make_float2(t.x + r3 * d.x, t.y + r3 * d.y);
+    v[2] = make_float2(t.x - r3 * d.x, t.y - r3 * d.y);
+}
+template <>
+__device__ __forceinline__ void dft<3, false>(float2 *v) { dft3<false>(v); }
+template <>
+__device__ __forceinline__ void dft<3, true>(float2 *v) { dft3<true>(v); }
+
+template <bool INV>
+__device__ __forceinline__ void dft12(float2 *v) {
+    // 12 = 4 x 3: DFT-3 over n = a + 4 c (c), twiddle w12^(a k1), DFT-4 over a;
+    // output k = k1 + 3 k2
+    float2 x[4][3];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) x[a][c] = v[a + 4 * c];
+        dft3<INV>(x[a]);
+    }
+    // w12^m = cos(pi m / 6) -+ i sin(pi m / 6), m = a k1 in {1, 2, 2, 4, 3, 6}
+    constexpr float r3 = 0.86602540378443865f;
+    constexpr float sg = INV ? 1.f : -1.f;
+    const float2 w1 = make_float2(r3, sg * 0.5f), w2 = make_float2(0.5f, sg * r3),
+                 w4 = make_float2(-0.5f, sg * r3);
+    x[1][1] = cmul(x[1][1], w1);
+    x[1][2] = cmul(x[1][2], w2);
+    x[2][1] = cmul(x[2][1], w2);
+    x[2][2] = cmul(x[2][2], w4);
+    x[3][1] = rot90<INV>(x[3][1]);
+    x[3][2] = make_float2(-x[3][2].x, -x[3][2].y);
+#pragma unroll
+    for (int k1 = 0; k1 < 3; ++k1) {
+        float2 y[4] = {x[0][k1], x[1][k1], x[2][k1], x[3][k1]};
+        dft4<INV>(y);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) v[k1 + 3 * k2] = y[k2];
+    }
+}
+template <>
+__device__ __forceinline__ void dft<12, false>(float2 *v) { dft12<false>(v); }
+template <>
+__device__ __forceinline__ void dft<12, true>(float2 *v) { dft12<true>(v); }
+
 // ------------------------------------------------------------------ plans
-// Values per thread and threads per line of a length-L transform.
-__host__ __device__ constexpr int vals_per_thread(int L) { return L < 16 ? L : 16; }
-__host__ __device__ constexpr int threads_per_line(int L) { return L / vals_per_thread(L); }
+// Supported lengths: 2^k (2..4096) and 3 * 2^k (12..3072).
+// Powers of two: E = 16 values per thread (E = L below 16), radix 16 while
+// at least 4 bits remain, then the rest (2, 4, 8 or 16).
+// Lengths 3 * 2^k: E = 12, a radix-12 stage first, then radix 4, and a
+// final radix 2 when an odd power of two remains; every radix divides 12,
+// so each stage gives each of the T = L / 12 threads whole butterflies.
 __host__ __device__ constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n / 2); }
-// Radix of stage s: 16 while at least 4 bits remain, then the rest (2, 4,
-// 8 or 16); lengths below 16 use a single stage.
+__host__ __device__ constexpr bool has3(int L) { return L % 3 == 0; }
+__host__ __device__ constexpr int vals_per_thread(int L) {
+    return has3(L) ? (L < 12 ? L : 12) : (L < 16 ? L : 16);
+}
+__host__ __device__ constexpr int threads_per_line(int L) { return L / vals_per_thread(L); }
 __host__ __device__ constexpr int num_stages(int L) {
-    return L <= 16 ? 1 : (ilog2c(L) + 3) / 4;
+    return has3(L) ? (L <= 12 ? 1 : 1 + (ilog2c(L / 12) + 1) / 2)
+                   : (L <= 16 ? 1 : (ilog2c(L) + 3) / 4);
 }
 __host__ __device__ constexpr int stage_radix(int L, int s) {
-    return L <= 16 ? L : (s + 1 < num_stages(L) ? 16 : (1 << (ilog2c(L) - 4 * s)));
+    return has3(L) ? (L <= 12 ? L : (s == 0 ? 12 : (ilog2c(L / 12) - 2 * (s - 1) >= 2 ? 4 : 2)))
+                   : (L <= 16 ? L : (s + 1 < num_stages(L) ? 16 : (1 << (ilog2c(L) - 4 * s))));
 }
 __host__ __device__ constexpr int stage_span(int L, int s) {  // NS: product of earlier radices
     return s == 0 ? 1 : stage_span(L, s - 1) * stage_radix(L, s - 1);
+}
+__host__ __device__ constexpr bool fft_length_supported(int L) {
+    return L >= 2 && (has3(L) ? (L >= 12 && L <= 3072 && ((L / 3) & (L / 3 - 1)) == 0)
+                              : (L <= 4096 && (L & (L - 1)) == 0));
 }
 
 // LDS index of element i of a line: one pad word every 16 elements keeps the

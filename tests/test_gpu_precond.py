@@ -21,7 +21,8 @@ def _dims(p):
     return p.shape if p.Nz > 1 else p.shape[:2]
 
 
-@pytest.mark.parametrize("N,Nz,lam", [(16, 1, 0.0), (32, 1, 1e-2), (8, 4, 1e-2)])
+@pytest.mark.parametrize("N,Nz,lam", [(16, 1, 0.0), (32, 1, 1e-2), (8, 4, 1e-2), (24, 1, 1e-2),
+                                      (48, 1, 0.0)])
 def test_chan_weights_match_oracle(ctx, N, Nz, lam):
     from impatient import ops
     from oracle import oracle as orc

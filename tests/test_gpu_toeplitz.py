@@ -17,7 +17,9 @@ def _setup(N=16, Nz=1, ncoils=3, L=3, seed=7, max_hz=60.0):
 
 
 @pytest.mark.parametrize("N,Nz,ncoils,L,lam", [(16, 1, 1, 0, 0.0), (16, 1, 3, 3, 1e-2),
-                                                (32, 1, 2, 5, 0.0), (8, 4, 2, 2, 1e-2)])
+                                                (32, 1, 2, 5, 0.0), (8, 4, 2, 2, 1e-2),
+                                                (24, 1, 2, 3, 1e-2), (48, 1, 1, 2, 0.0),
+                                                (8, 24, 2, 2, 1e-2)])
 def test_apply_matches_oracle(ctx, N, Nz, ncoils, L, lam):
     from impatient import ops
     from oracle import oracle as orc
