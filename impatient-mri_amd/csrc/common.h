@@ -67,6 +67,7 @@ enum WsSlot {
     WS_BFORCE,     // brute-force CG vectors
     WS_REG,        // regularizer / TV staging
     WS_QHALF,      // Q on the half grid (Hermitian fill)
+    WS_BATCH,      // batched NUDFT operands
     WS_COUNT
 };
 

@@ -167,22 +167,28 @@ __global__ __launch_bounds__(NU_THREADS) void k_nudft_valu(
 // compensation in LDS.  Output planes: partial[chunk][TILES*32][n_lane]
 // (plane = tile*32 + part*16 + rhs%16).  `one` is 1.0f, passed at run time
 // so the lo split compiles to v_fma_mix.
+template <int TILES>
+struct MfmaLds {
+    float cmp[NU_THREADS / 64][2 * TILES * 16][64];
+};
+
+// Workgroup (bx, chunk) of one NUDFT job; shared by the single-job and the
+// batched launch.
 template <int TILES, bool D3>
-__global__ __launch_bounds__(NU_THREADS) __attribute__((amdgpu_waves_per_eu(TILES == 1 ? 3 : 2)))
-void k_nudft_mfma(
-    const float4 *__restrict__ lanes, int n_lane, const double *__restrict__ spos,
-    const BPair *__restrict__ bpack, int n_steps, int steps_per_chunk,
-    float *__restrict__ partial, float one) {
+__device__ __forceinline__ void mfma_body(const float4 *__restrict__ lanes, int n_lane,
+                                          const double *__restrict__ spos,
+                                          const BPair *__restrict__ bpack, int n_steps,
+                                          int steps_per_chunk, float *__restrict__ partial,
+                                          float one, int bx, int chunk, MfmaLds<TILES> &lds) {
     constexpr int NA = 2 * TILES;  // accumulators [m tile][n tile]
     constexpr int WAVES = NU_THREADS / 64;
-    __shared__ float cmp_lds[WAVES][NA * 16][64];
+    auto &cmp_lds = lds.cmp;
     const int l = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int wt = blockIdx.x * WAVES + wave;
+    const int wt = bx * WAVES + wave;
     const int h = l >> 5;
     const float4 Pf = lanes[min(wt * MF_WAVE_PTS + l, n_lane - 1)];
     const double px = Pf.x, py = Pf.y, pz = Pf.z, pw = Pf.w;
-    const int chunk = blockIdx.y;
     const int s0 = chunk * steps_per_chunk;
     const int s1 = min(n_steps, s0 + steps_per_chunk);
 
@@ -289,6 +295,43 @@ void k_nudft_mfma(
             }
         }
     }
+}
+
+template <int TILES, bool D3>
+__global__ __launch_bounds__(NU_THREADS) __attribute__((amdgpu_waves_per_eu(TILES == 1 ? 3 : 2)))
+void k_nudft_mfma(const float4 *__restrict__ lanes, int n_lane, const double *__restrict__ spos,
+                  const BPair *__restrict__ bpack, int n_steps, int steps_per_chunk,
+                  float *__restrict__ partial, float one) {
+    __shared__ MfmaLds<TILES> lds;
+    mfma_body<TILES, D3>(lanes, n_lane, spos, bpack, n_steps, steps_per_chunk, partial, one,
+                         blockIdx.x, blockIdx.y, lds);
+}
+
+// One job of a batched launch (images of different sizes in one grid).
+struct MfmaJob {
+    const float4 *lanes;
+    const double *spos;
+    const BPair *bpack;
+    float *partial;
+    int n_lane, n_steps, steps_per_chunk;
+    int wg_begin;   // first flattened workgroup of this job
+    int wgs_x;      // workgroups along the points; chunks = count / wgs_x
+    int pad[3];
+};
+
+// Persistent-grid batch: flattened workgroup -> (job, bx, chunk); the job
+// table is scanned with wave-uniform (scalar) loads.
+template <int TILES, bool D3>
+__global__ __launch_bounds__(NU_THREADS) __attribute__((amdgpu_waves_per_eu(TILES == 1 ? 3 : 2)))
+void k_nudft_mfma_batch(const MfmaJob *__restrict__ jobs, int njobs, float one) {
+    __shared__ MfmaLds<TILES> lds;
+    const int g = blockIdx.x;
+    int j = 0;
+    while (j + 1 < njobs && jobs[j + 1].wg_begin <= g) ++j;
+    const MfmaJob J = jobs[j];
+    const int rem = g - J.wg_begin;
+    mfma_body<TILES, D3>(J.lanes, J.n_lane, J.spos, J.bpack, J.n_steps, J.steps_per_chunk,
+                         J.partial, one, rem % J.wgs_x, rem / J.wgs_x, lds);
 }
 
 // ---- packing ---------------------------------------------------------------
@@ -754,6 +797,123 @@ int nudft_adjoint(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *k
     return run_job(ctx, J);
 }
 
+// F^H d of several images (different sizes, trajectories, coil maps) with
+// one persistent MFMA launch over all of them.  Every job needs 3..32 coils
+// and the same dimensionality.  chunk_scale multiplies the per-job
+// workgroup target (the autotuned knob; 0 = default 8 per CU).
+int nudft_adjoint_batch(imp_ctx *ctx, int njobs, const imp_nudft_job *jobs, int chunk_scale) {
+    if (!ctx || njobs <= 0 || !jobs) return fail(IMP_EINVAL, "nudft_adjoint_batch: bad arguments");
+    const bool d3 = jobs[0].kz && jobs[0].z;
+    int tiles = 0;
+    for (int j = 0; j < njobs; ++j) {
+        const imp_nudft_job &b = jobs[j];
+        if (b.num_k <= 0 || b.num_x <= 0 || !b.kx || !b.ky || !b.t || !b.x || !b.y || !b.d_r ||
+            !b.d_i || !b.out_r || !b.out_i || (b.s_r && !b.s_i) || (b.phi_r && !b.phi_i))
+            return fail(IMP_EINVAL, "nudft_adjoint_batch: bad job %d", j);
+        if (b.ncoils < MFMA_MIN_RHS || b.ncoils > 2 * MF_COLS)
+            return fail(IMP_EUNSUPPORTED, "nudft_adjoint_batch: job %d has %d coils (3..32)", j,
+                        b.ncoils);
+        if ((b.kz && b.z) != d3)
+            return fail(IMP_EINVAL, "nudft_adjoint_batch: mixed 2D and 3D jobs");
+        const int t = ceil_div(b.ncoils, MF_COLS);
+        if (j == 0) tiles = t;
+        if (t != tiles)
+            return fail(IMP_EUNSUPPORTED, "nudft_adjoint_batch: jobs need the same coil tile count");
+    }
+    const long target = (long)(chunk_scale > 0 ? chunk_scale : 8) * ctx->num_cus;
+    // carve one workspace: per job spos, lanes, W, maxbits, bpack, partials
+    struct Lay {
+        size_t spos, lanes, w, mb, bp, part;
+        int n_steps, chunks, spc, wgs_x;
+    };
+    std::vector<Lay> lay(njobs);
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return o;
+    };
+    for (int j = 0; j < njobs; ++j) {
+        const imp_nudft_job &b = jobs[j];
+        Lay &L = lay[j];
+        L.n_steps = ceil_div(b.num_k, MF_K);
+        L.wgs_x = ceil_div(b.num_x, MF_WAVE_PTS * (NU_THREADS / 64));
+        L.chunks = 1;
+        while ((long)L.wgs_x * L.chunks < target && L.n_steps / (L.chunks * 2) >= 4 * MF_FOLD)
+            L.chunks *= 2;
+        L.spc = ceil_div(L.n_steps, L.chunks);
+        L.spos = take((size_t)L.n_steps * pos_stride(d3) * sizeof(double));
+        L.lanes = take((size_t)b.num_x * sizeof(float4));
+        L.w = take((size_t)b.num_k * b.ncoils * sizeof(float2));
+        L.mb = take(sizeof(unsigned int));
+        L.bp = take((size_t)(L.n_steps + 1) * tiles * 64 * sizeof(BPair));
+        L.part = take((size_t)L.chunks * tiles * 32 * b.num_x * sizeof(float));
+    }
+    const size_t table = take((size_t)njobs * sizeof(MfmaJob));
+    void *wsp = nullptr;
+    IMP_TRY(workspace(ctx, WS_BATCH, off, &wsp));
+    char *base = (char *)wsp;
+    const hipStream_t st = ctx->stream;
+    std::vector<MfmaJob> tab(njobs);
+    int wg = 0;
+    for (int j = 0; j < njobs; ++j) {
+        const imp_nudft_job &b = jobs[j];
+        const Lay &L = lay[j];
+        double *spos = (double *)(base + L.spos);
+        float4 *lanes = (float4 *)(base + L.lanes);
+        float2 *W = (float2 *)(base + L.w);
+        unsigned int *mb = (unsigned int *)(base + L.mb);
+        BPair *bp = (BPair *)(base + L.bp);
+        const int n_pad = L.n_steps * MF_K;
+        hipLaunchKernelGGL(k_spos, dim3(ceil_div(n_pad, 256)), dim3(256), 0, st, b.num_k, n_pad,
+                           b.kx, b.ky, d3 ? b.kz : nullptr, b.t, 1.0, spos);
+        hipLaunchKernelGGL(k_lanes_from_arrays, dim3(ceil_div(b.num_x, 256)), dim3(256), 0, st,
+                           b.num_x, b.x, b.y, d3 ? b.z : nullptr, b.fm, 1.0 / (2.0 * M_PI), lanes);
+        hipLaunchKernelGGL(k_w_adjoint, dim3(ceil_div((long)b.num_k * b.ncoils, 256)), dim3(256),
+                           0, st, b.num_k, b.ncoils, b.phi_r, b.phi_i, b.d_r, b.d_i, W);
+        IMP_HIP(hipMemsetAsync(mb, 0, sizeof(unsigned int), st));
+        hipLaunchKernelGGL(k_absmax, dim3(512), dim3(256), 0, st, (const float2 *)W,
+                           (size_t)b.ncoils * b.num_k, mb);
+        const size_t nb = (size_t)(L.n_steps + 1) * tiles * 64;
+        hipLaunchKernelGGL(k_pack_mfma, dim3(ceil_div((long)nb, 256)), dim3(256), 0, st, b.num_k,
+                           b.ncoils, tiles, L.n_steps + 1, +1.f, (const float2 *)W,
+                           (const unsigned int *)mb, bp);
+        IMP_CHECK_LAUNCH();
+        MfmaJob &J = tab[j];
+        J.lanes = lanes;
+        J.spos = spos;
+        J.bpack = bp;
+        J.partial = (float *)(base + L.part);
+        J.n_lane = b.num_x;
+        J.n_steps = L.n_steps;
+        J.steps_per_chunk = L.spc;
+        J.wg_begin = wg;
+        J.wgs_x = L.wgs_x;
+        wg += L.wgs_x * L.chunks;
+    }
+    MfmaJob *dtab = (MfmaJob *)(base + table);
+    IMP_HIP(hipMemcpyAsync(dtab, tab.data(), (size_t)njobs * sizeof(MfmaJob),
+                           hipMemcpyHostToDevice, st));
+    using BFn = void (*)(const MfmaJob *, int, float);
+    BFn fn = tiles == 1 ? (d3 ? (BFn)k_nudft_mfma_batch<1, true> : (BFn)k_nudft_mfma_batch<1, false>)
+                        : (d3 ? (BFn)k_nudft_mfma_batch<2, true> : (BFn)k_nudft_mfma_batch<2, false>);
+    const int rec = prof_begin(ctx, "k_nudft_mfma_batch");
+    hipLaunchKernelGGL(fn, dim3(wg), dim3(NU_THREADS), 0, st, (const MfmaJob *)dtab, njobs, 1.0f);
+    prof_end(ctx, rec);
+    IMP_CHECK_LAUNCH();
+    for (int j = 0; j < njobs; ++j) {
+        const imp_nudft_job &b = jobs[j];
+        const Lay &L = lay[j];
+        hipLaunchKernelGGL(k_reduce_mfma, dim3(ceil_div(b.num_x, 256)), dim3(256), 0, st,
+                           (const float *)(base + L.part), L.chunks, tiles, b.ncoils, b.num_x, 0,
+                           (const unsigned int *)(base + L.mb), b.s_r, b.s_i, b.out_r, b.out_i);
+    }
+    IMP_CHECK_LAUNCH();
+    // the host job table must outlive the copy
+    IMP_HIP(hipStreamSynchronize(st));
+    return IMP_OK;
+}
+
 int nudft_forward(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *kx,
                   const float *ky, const float *kz, const float *t, const float *x,
                   const float *y, const float *z, const float *fm,
@@ -851,3 +1011,8 @@ int toeplitz_q(imp_ctx *ctx, int num_k, const float *kx, const float *ky,
 }
 
 }  // namespace imp
+
+extern "C" int imp_nudft_adjoint_batch(imp_ctx *ctx, int njobs, const imp_nudft_job *jobs,
+                                       int chunk_scale) {
+    return imp::nudft_adjoint_batch(ctx, njobs, jobs, chunk_scale);
+}

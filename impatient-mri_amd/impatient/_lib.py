@@ -49,6 +49,13 @@ class CGStats(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class NudftJob(C.Structure):
+    """imp_nudft_job (device pointers as integers)."""
+    _fields_ = [("num_k", C.c_int), ("num_x", C.c_int), ("ncoils", C.c_int)] + [
+        (n, C.c_void_p) for n in ("kx", "ky", "kz", "t", "x", "y", "z", "fm", "phi_r", "phi_i",
+                                  "d_r", "d_i", "s_r", "s_i", "out_r", "out_i")]
+
+
 class FileInfo(C.Structure):
     _fields_ = [("version", C.c_float), ("per_coil", C.c_int), ("ncoils", C.c_int),
                 ("nslices", C.c_int), ("total", C.c_int), ("nx", C.c_int),

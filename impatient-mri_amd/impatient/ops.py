@@ -56,6 +56,38 @@ def nudft_adjoint(ctx: Context, kx, ky, kz, t, x, y, z, fm, d, sens=None, phi=No
     return _cplx(orr, oi)
 
 
+def nudft_adjoint_batch(ctx: Context, jobs, chunk_scale=0):
+    """F^H d of several 2D images in one persistent launch.  jobs: list of
+    dicts with kx, ky, t, x, y, fm, d [C, M], sens [C, N] (numpy).  Returns
+    the list of images."""
+    from ._lib import NudftJob
+    keep = []
+    arr = (NudftJob * len(jobs))()
+    outs = []
+    for j, jb in enumerate(jobs):
+        d = np.atleast_2d(np.asarray(jb["d"]))
+        ncoils, nk = d.shape
+        nx = len(jb["x"])
+        dev = {k: ctx.to_device(_f32(jb[k])) for k in ("kx", "ky", "t", "x", "y", "fm")}
+        dr, di = _split(ctx, d.reshape(-1))
+        sr = si = None
+        if jb.get("sens") is not None:
+            sr, si = _split(ctx, np.asarray(jb["sens"]).reshape(-1))
+        o = _pair(ctx, nx)
+        keep += list(dev.values()) + [dr, di, sr, si]
+        outs.append(o)
+        J = arr[j]
+        J.num_k, J.num_x, J.ncoils = nk, nx, ncoils
+        for k, v in dev.items():
+            setattr(J, k, v.ptr)
+        J.d_r, J.d_i = dr.ptr, di.ptr
+        J.s_r = sr.ptr if sr is not None else None
+        J.s_i = si.ptr if si is not None else None
+        J.out_r, J.out_i = o[0].ptr, o[1].ptr
+    call("imp_nudft_adjoint_batch", ctx.h, C.c_int(len(jobs)), arr, C.c_int(chunk_scale))
+    return [_cplx(*o) for o in outs]
+
+
 def nudft_forward(ctx: Context, kx, ky, kz, t, x, y, z, fm, rho, sens=None, ncoils=None):
     """d_c(k) = sum_x S_c(x) rho(x) e^{-i(2pi k.x + fm t)}; returns [C, M]."""
     nk, nx = len(kx), len(x)

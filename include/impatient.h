@@ -117,6 +117,21 @@ int imp_nudft_forward(imp_ctx *ctx, int num_k, int num_x, int ncoils,
                       const float *s_r, const float *s_i,
                       float *d_r, float *d_i);
 
+/* Batched F^H d: several images (different sizes, trajectories, coil maps;
+ * each 3..32 coils, all 2D or all 3D) in one persistent MFMA launch whose
+ * flattened grid covers every image's point tiles.  Arguments of each job
+ * as for imp_nudft_adjoint.  chunk_scale: workgroups per CU targeted per job
+ * (the schedule knob an autotuner sweeps; 0 = 8).  No counterpart in the
+ * reference, which loops slices on the host (main_mri.cu:771-904). */
+typedef struct {
+    int num_k, num_x, ncoils;
+    const float *kx, *ky, *kz, *t, *x, *y, *z, *fm;
+    const float *phi_r, *phi_i, *d_r, *d_i, *s_r, *s_i;
+    float *out_r, *out_i;
+} imp_nudft_job;
+int imp_nudft_adjoint_batch(imp_ctx *ctx, int njobs, const imp_nudft_job *jobs,
+                            int chunk_scale);
+
 /* Toeplitz kernels Q_l, l = 0..ntime_segs (ComputeQ_GPU / computeQ_GPU /
  * toeplitz_computeQ_GPU, toeplitz/computeQ.cmem.cu:169-208, 756-822, 826-1028):
  *   Q_l(r) = sum_k a_l(t_k) |phi(k)|^2 e^{+i 2pi k.r}

@@ -140,3 +140,26 @@ def test_q_explicit_grid_matches_standard(ctx):
     a = ops.toeplitz_q(ctx, p.kx, p.ky, None, p.t, 8, 8, 1, 2)
     b = ops.toeplitz_q(ctx, p.kx, p.ky, None, p.t, 8, 8, 1, 2, grid=(g[0], g[1], None))
     assert rel_l2(a, b) < 1e-6
+
+
+def test_adjoint_batch_matches_single_calls_and_oracle(ctx):
+    """Three images of different sizes, trajectories and coil counts in one
+    persistent launch: each equals its own imp_nudft_adjoint call (same
+    arithmetic, different chunking) and the FP64 oracle."""
+    from impatient import ops, synth
+    from oracle import oracle as orc
+    probs = [synth.tiny(N=16, ncoils=4, seed=1), synth.tiny(N=24, ncoils=6, seed=2, J=6),
+             synth.tiny(N=32, ncoils=8, seed=3, J=8, S=256)]
+    jobs = []
+    for p in probs:
+        y = orc.nudft_forward(p.kx, p.ky, None, p.t, p.x, p.y, None, p.fm, p.rho, p.sens)
+        jobs.append(dict(kx=p.kx, ky=p.ky, t=p.t, x=p.x, y=p.y, fm=p.fm, d=y, sens=p.sens))
+    for scale in (0, 2):
+        got = ops.nudft_adjoint_batch(ctx, jobs, chunk_scale=scale)
+        for p, jb, g in zip(probs, jobs, got):
+            one = ops.nudft_adjoint(ctx, p.kx, p.ky, None, p.t, p.x, p.y, None, p.fm, jb["d"],
+                                    p.sens)
+            ref = orc.nudft_adjoint(p.kx, p.ky, None, p.t, p.x, p.y, None, p.fm, jb["d"], p.sens)
+            e1, e2 = rel_l2(g, one), rel_l2(g, ref)
+            print(f"batch N={p.Nx} C={p.ncoils} scale={scale}: vs single {e1:.2e}, vs FP64 {e2:.2e}")
+            assert e1 < 1e-6 and e2 < OP_TOL
