@@ -395,6 +395,7 @@ extern "C" int imp_toep_create(imp_ctx *ctx, int N1, int N2, int N3, int ncoils,
         while (groups < ncoils && wgs_per_group * groups < 4L * ctx->num_cus) groups *= 2;
         if (groups > ncoils) groups = ncoils;
         P.coils_per_group = ceil_div(ncoils, groups);
+        P.coils_per_group_min = P.coils_per_group;
     }
     const int ngroups = ceil_div(ncoils, P.coils_per_group);
     const long A = N3 > 1 ? P.M2 : 1, Ml = N3 > 1 ? P.M3 : P.M2;
@@ -511,6 +512,23 @@ extern "C" int imp_toep_set_precond_radial(imp_toep *op, const float *gains, int
                        P.pw_chan, dg, nbins, P.N1, P.N2, P.N3, P.pw);
     IMP_CHECK_LAUNCH();
     IMP_HIP(hipStreamSynchronize(ctx->stream));
+    return IMP_OK;
+}
+
+extern "C" int imp_toep_get_schedule(imp_toep *op, int *coils_per_group) {
+    if (!op || !coils_per_group) return fail(IMP_EINVAL, "imp_toep_get_schedule: bad arguments");
+    *coils_per_group = op->P.coils_per_group;
+    return IMP_OK;
+}
+
+extern "C" int imp_toep_set_schedule(imp_toep *op, int coils_per_group) {
+    if (!op || !op->ctx) return fail(IMP_EINVAL, "imp_toep_set_schedule: bad operator");
+    ToepPlan &P = op->P;
+    // the partial buffer holds the groups of the schedule chosen at create time
+    if (coils_per_group < P.coils_per_group_min || coils_per_group > P.ncoils)
+        return fail(IMP_EINVAL, "imp_toep_set_schedule: coils per group %d outside [%d, %d]",
+                    coils_per_group, P.coils_per_group_min, P.ncoils);
+    P.coils_per_group = coils_per_group;
     return IMP_OK;
 }
 

@@ -16,6 +16,7 @@ struct ToepPlan {
     int nseg = 1;                 // L + 1 interpolators
     int ncoils = 1;
     int coils_per_group = 8;      // pass-E coil grouping
+    int coils_per_group_min = 8;  // the grouping the partial buffer was sized for
     float tau = 0.f, t0 = 0.f;
     float lambda = 0.f;
     float2 *qf = nullptr;         // [nseg][G]  FFT(ifftshift(Q_l))

@@ -188,6 +188,12 @@ int imp_toep_create(imp_ctx *ctx, int N1, int N2, int N3, int ncoils,
 int imp_toep_destroy(imp_toep *op);
 /* Replace the TV weight W of lambda R (device [N], NULL = identity). */
 int imp_toep_set_weight(imp_toep *op, const float *w);
+/* Pass-E schedule: coils summed per workgroup (more coils per group = fewer
+ * partial images for pass F, fewer workgroups).  imp_toep_create picks one
+ * that fills the GPU; set accepts [that value, ncoils] (the autotuner's
+ * knob, impatient/batch.py). */
+int imp_toep_get_schedule(imp_toep *op, int *coils_per_group);
+int imp_toep_set_schedule(imp_toep *op, int coils_per_group);
 /* Replace the field map (device [N], rad/s, NULL = 0); Q_l and the
  * preconditioner do not depend on it, so one plan serves every field map of
  * a trajectory (fMRI frames, training draws).  No counterpart in the
