@@ -161,7 +161,6 @@ def reconstruct(ctx: Context, slices, tol=1e-5, max_iter=500, model=None, tuner=
         call("imp_toep_set_schedule", h, C.c_int(cpg))
         flags = IMP_CG_BEST_ITERATE | IMP_CG_TRUE_RESIDUAL | IMP_CG_PRECOND | IMP_CG_GRAPH
         if cl["net"] is not None:
-            call("imp_toep_precond_apply", h, ptr(br), ptr(bi), ptr(xr), ptr(xi))
             zr, zi = ctx.alloc((p.nvox,)), ctx.alloc((p.nvox,))
             call("imp_toep_precond_apply", h, ptr(br), ptr(bi), ptr(zr), ptr(zi))
             cl["net"].forward_device(zr, zi, xr, xi)
