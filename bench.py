@@ -122,14 +122,17 @@ def learned_solve(args, ctx, h, prob, dbr, dbi, dxr, dxi, fhd_ms, t_q, t_plan, l
     flags = (impatient.IMP_CG_BEST_ITERATE | impatient.IMP_CG_TRUE_RESIDUAL |
              impatient.IMP_CG_PRECOND | impatient.IMP_CG_USE_X0 | impatient.IMP_CG_FALLBACK_ZERO)
     zr, zi = ctx.alloc((prob.nvox,)), ctx.alloc((prob.nvox,))
-    out = {"model": MODEL.name, "trained_on": f"{meta['draws']} draws, {meta['epochs']} epochs"}
+    out = {"model": MODEL.name, "trained_on": f"{meta['draws']} draws, {meta['epochs']} epochs",
+           "preconditioner": "learned radial gains" if meta.get("use_gains") else "Chan"}
     for warm in (True, False):   # first pass warms the kernels up, second is timed
         call("imp_toep_set_precond", h, None)
         ctx.synchronize()
         t0 = time.perf_counter()
         call("imp_toep_precond_apply", h, ptr(dbr), ptr(dbi), ptr(zr), ptr(zi))
         net.forward_device(zr, zi, dxr, dxi)
-        call("imp_toep_set_precond_radial", h, gh.ctypes.data_as(C.c_void_p), C.c_int(len(gh)))
+        if meta.get("use_gains", False):
+            call("imp_toep_set_precond_radial", h, gh.ctypes.data_as(C.c_void_p),
+                 C.c_int(len(gh)))
         ctx.synchronize()
         t_start = time.perf_counter() - t0
         st = _lib_stats()
@@ -137,7 +140,7 @@ def learned_solve(args, ctx, h, prob, dbr, dbi, dxr, dxi, fhd_ms, t_q, t_plan, l
              C.c_double(1e-5), C.c_int(flags), C.byref(st))
         t_total = time.perf_counter() - t0
     s = st.as_dict()
-    log(f"learned start + learned-gain PCG: start {t_start * 1e3:.1f} ms (residual "
+    log(f"learned start + PCG ({out['preconditioner']}): start {t_start * 1e3:.1f} ms (residual "
         f"{s['start_rel_residual']:.3e}, used {s['start_used']}), {s['iterations']} it, "
         f"{t_total:.3f} s total, true {s['true_rel_residual']:.2e}")
     call("imp_toep_set_precond", h, None)

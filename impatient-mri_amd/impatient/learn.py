@@ -10,8 +10,10 @@ Learned parts:
   * the residual CNN (imp_cnn_*) x0 = z0 + s * net(z0 / s), trained on the
     GPU with its own forward / backward / Adam kernels;
   * radial gains g(|k|) of the circulant preconditioner, w = w_chan * g,
-    the per-band least-squares fit of M^-1 b ~ x* over the training pairs
-    (g > 0, so M stays Hermitian positive definite and PCG stays valid).
+    the per-band least-squares fit of M^-1 A v ~ v over white probes v
+    under the training field maps (g > 0, so M stays Hermitian positive
+    definite and PCG stays valid); kept only when they beat the Chan weights
+    on held-out draws.
 
 At solve time the learned start is used only when its residual
 ||b - A x0|| is below ||b|| (the zero start's); otherwise CG starts from 0.
@@ -218,6 +220,29 @@ def fit_radial_gains(geo: Geometry, pairs, nbins=NBINS):
     return np.clip(g, 0.05, 20.0).astype(np.float32)
 
 
+def probe_pairs(geo: Geometry, n: int, seed0: int, log=print):
+    """Pairs (z = M_chan^-1 A v, v) for white complex Gaussian probes v under
+    the field maps of n draws: M^-1 A ~ I per radial band is what a
+    preconditioner needs, and unlike (M^-1 b, x*) these pairs carry no
+    unconverged-solve or noise bias."""
+    prob = geo.prob
+    X = prob.x.reshape(prob.Ny, prob.Nx).astype(np.float64)
+    Y = prob.y.reshape(prob.Ny, prob.Nx).astype(np.float64)
+    pairs = []
+    yr, yi = geo.ctx.alloc((geo.nv,)), geo.ctx.alloc((geo.nv,))
+    for k in range(n):
+        rng = np.random.default_rng(seed0 + k)
+        geo.set_fieldmap(synth.field_map(rng, X, Y, 4, prob.meta["max_hz"]))
+        p = Pair(*(geo.ctx.alloc((geo.nv,)) for _ in range(4)))
+        p.tr.upload(rng.standard_normal(geo.nv).astype(np.float32))
+        p.ti.upload(rng.standard_normal(geo.nv).astype(np.float32))
+        call("imp_toep_apply", geo.op, ptr(p.tr), ptr(p.ti), ptr(yr), ptr(yi))
+        geo.precond(yr, yi, p.zr, p.zi)
+        pairs.append(p)
+    log(f"{n} probe pairs for the preconditioner gains")
+    return pairs
+
+
 def learned_start(geo: Geometry, net: ops.CNN, xr, xi):
     """x0 = z0 + s * net(z0 / s) with z0 = M^-1 b (current preconditioner)."""
     geo.precond(geo.br, geo.bi, geo.rr, geo.ri)
@@ -307,12 +332,17 @@ def main(argv=None):
     pairs = make_pairs(geo, args.draws, seed0, log=log)
     net = ops.CNN(ctx, prob.Ny, prob.Nx, seed=seed0)
     hist = train(net, pairs, args.epochs, args.lr, log=log)
-    gains = fit_radial_gains(geo, pairs)
+    gains = fit_radial_gains(geo, probe_pairs(geo, 24, seed0 + 500000, log=log))
     log("radial gains: " + " ".join(f"{g:.3f}" for g in gains[::8]))
     rows = evaluate(geo, net, gains, args.val, seed0 + args.draws + 1000, log=log)
+    val = {m: [r[m]["iterations"] for r in rows] for m in rows[0]}
+    # keep the learned gains only if they beat the Chan weights on held-out draws
+    use_gains = float(np.mean(val["gains_learned"])) < float(np.mean(val["chan_learned"]))
+    log(f"validation mean iterations: " + ", ".join(f"{m} {np.mean(v):.1f}" for m, v in val.items())
+        + f"; learned gains {'kept' if use_gains else 'not used (Chan weights are better)'}")
     meta = dict(config=prob.name, draws=args.draws, epochs=args.epochs, lr=args.lr,
-                seed0=seed0, loss_history=hist, nbins=NBINS,
-                validation={m: [r[m]["iterations"] for r in rows] for m in rows[0]})
+                seed0=seed0, loss_history=hist, nbins=NBINS, use_gains=use_gains,
+                validation=val)
     save_model(args.out, net.get_params(), gains, meta)
     log(f"saved {args.out}")
     net.close()

@@ -97,8 +97,10 @@ def test_toeplitz_apply_at_config1_size(ctx):
 
 def test_config1_solves_to_tolerance(ctx):
     """Configuration 1's reconstruction (20 CG iterations from F^H d) and the
-    preconditioned solve to 1e-5: recomputed true residuals, and the PCG image
-    against the plain-CG image run to the same residual."""
+    solves to 1e-5 with and without the preconditioner: recomputed true
+    residuals and iteration counts.  (With lambda = 1e-3 this system needs
+    ~1000 plain-CG iterations; two FP32 solutions at residual 1e-5 agree only
+    to ~cond * 1e-5, so the images are not compared point by point.)"""
     from impatient import ops, synth, IMP_CG_BEST_ITERATE, IMP_CG_PRECOND, IMP_CG_TRUE_RESIDUAL
     from oracle import oracle as orc
     p = synth.config(1)
@@ -117,5 +119,6 @@ def test_config1_solves_to_tolerance(ctx):
           f" |xp - xc| / |xc| = {rel_l2(xp, xc):.2e}")
     assert s20["iterations"] == p.cg_iters and s20["true_rel_residual"] < 1.0
     assert sc["converged"] == 1 and sp["converged"] == 1
-    assert sp["true_rel_residual"] < 1e-5 and sp["iterations"] <= sc["iterations"]
-    assert rel_l2(xp, xc) < 1e-2
+    assert sp["true_rel_residual"] < 1e-5 and sc["true_rel_residual"] < 1e-5
+    assert sp["iterations"] < 0.5 * sc["iterations"]
+    assert rel_l2(xp, xc) < 0.5
