@@ -38,6 +38,11 @@ PEAK_HBM_GBS = 8000.0
 # fp16 MFMA flops per (coil, sample, voxel) pair: one complex MAC is 8 real
 # flops, done as 3 fp16 products (Eh Wh + Eh Wl + El Wh) for FP32 accuracy
 MFMA_FLOPS_PER_PAIR = 24.0
+# HBM bytes per F^H d launch of k_nudft_mfma<2,false> on this workload, from
+# rocprofv3 --pmc FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0), the factors a
+# 256 MiB device copy calibrates (profiles/traffic_512x512x1_32coil_pmc.md);
+# re-measure when the kernel changes
+NUDFT_TRAFFIC_BYTES = 2.0 * 3323479.7 * 1024 + 1.0 * 1046711.3 * 1024
 
 
 def hip_events():
@@ -93,6 +98,9 @@ def cpu_baseline(prob, d_host, seconds_target=15.0):
     a = max(t1 - b * m1, 0.0)
     m = int(min(prob.nk, max(m2, (seconds_target - a) / b)))
     dt = run(m)
+    if dt < 0.6 * seconds_target and m < prob.nk:   # small samples overstate b: refine once
+        m = int(min(prob.nk, m * seconds_target / dt))
+        dt = run(m)
     pairs = float(m) * nv
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": pairs / dt / 1e9, "unit": "GSample*vox/s", "cores": cores,
@@ -253,7 +261,7 @@ def main():
     achieved = flops / (k_ms * 1e-3) / 1e12
     roofline = {"bound": "mfma", "achieved": round(achieved, 1), "peak": PEAK_FP16_MFMA_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP16_MFMA_TFLOPS, 4),
-                "traffic": None,
+                "traffic": round(NUDFT_TRAFFIC_BYTES),
                 "kernel": "k_nudft_mfma<2,false> (F^H d, 32 coils in one pass)",
                 "kernel_ms": round(k_ms, 3),
                 "work": "24 fp16 MFMA flops per (coil, sample, voxel) pair"}
