@@ -512,29 +512,45 @@ extern "C" int imp_cnn_create(imp_ctx *ctx, int H, int W, unsigned long long see
         imp_cnn_destroy(net);
         return fail(IMP_EHIP, "imp_cnn_create: initialisation copy failed");
     }
+    ctx->cnns.insert(net);
     *out = net;
     return IMP_OK;
 }
 
 extern "C" int imp_cnn_destroy(imp_cnn *net) {
     if (!net) return IMP_OK;
-    if (net->ctx) (void)hipStreamSynchronize(net->ctx->stream);
-    for (void *p : net->owned) (void)hipFree(p);
+    // a network whose context was destroyed first has no device memory left
+    if (net->ctx) {
+        (void)hipStreamSynchronize(net->ctx->stream);
+        imp_ctx *ctx = net->ctx;
+        imp::detach_cnn(net);
+        ctx->cnns.erase(net);
+    }
     delete net;
     return IMP_OK;
 }
 
+namespace imp {
+void detach_cnn(imp_cnn *net) {
+    for (void *p : net->owned) (void)hipFree(p);
+    net->owned.clear();
+    net->ctx = nullptr;
+}
+}  // namespace imp
+
 extern "C" int imp_cnn_num_params(void) { return NPARAM; }
 
 extern "C" int imp_cnn_get_params(imp_cnn *net, float *host, int n) {
-    if (!net || !host || n != NPARAM) return fail(IMP_EINVAL, "imp_cnn_get_params: bad arguments");
+    if (!net || !net->ctx || !host || n != NPARAM)
+        return fail(IMP_EINVAL, "imp_cnn_get_params: bad arguments or destroyed context");
     IMP_HIP(hipStreamSynchronize(net->ctx->stream));
     IMP_HIP(hipMemcpy(host, net->params, NPARAM * sizeof(float), hipMemcpyDeviceToHost));
     return IMP_OK;
 }
 
 extern "C" int imp_cnn_set_params(imp_cnn *net, const float *host, int n) {
-    if (!net || !host || n != NPARAM) return fail(IMP_EINVAL, "imp_cnn_set_params: bad arguments");
+    if (!net || !net->ctx || !host || n != NPARAM)
+        return fail(IMP_EINVAL, "imp_cnn_set_params: bad arguments or destroyed context");
     for (int i = 0; i < NPARAM; ++i)
         if (!std::isfinite(host[i])) return fail(IMP_EINVAL, "imp_cnn_set_params: non-finite value");
     IMP_HIP(hipStreamSynchronize(net->ctx->stream));
@@ -548,7 +564,7 @@ extern "C" int imp_cnn_set_params(imp_cnn *net, const float *host, int n) {
 
 extern "C" int imp_cnn_forward(imp_cnn *net, const float *z_r, const float *z_i, float *x_r,
                                float *x_i) {
-    if (!net || !z_r || !z_i || !x_r || !x_i)
+    if (!net || !net->ctx || !z_r || !z_i || !x_r || !x_i)
         return fail(IMP_EINVAL, "imp_cnn_forward: bad arguments");
     IMP_TRY(forward(net, z_r, z_i));
     hipLaunchKernelGGL(k_output, dim3(ceil_div(net->n, 256)), dim3(256), 0, net->ctx->stream, z_r,
@@ -561,7 +577,7 @@ extern "C" int imp_cnn_forward(imp_cnn *net, const float *z_r, const float *z_i,
 extern "C" int imp_cnn_gradient(imp_cnn *net, const float *z_r, const float *z_i,
                                 const float *t_r, const float *t_i, float *grad_host,
                                 double *loss) {
-    if (!net || !z_r || !z_i || !t_r || !t_i)
+    if (!net || !net->ctx || !z_r || !z_i || !t_r || !t_i)
         return fail(IMP_EINVAL, "imp_cnn_gradient: bad arguments");
     IMP_TRY(backward(net, z_r, z_i, t_r, t_i, loss));
     if (grad_host)

@@ -68,6 +68,9 @@ enum WsSlot {
     WS_REG,        // regularizer / TV staging
     WS_QHALF,      // Q on the half grid (Hermitian fill)
     WS_BATCH,      // batched NUDFT operands
+    WS_QS_SORT,    // separable Q: sample order, bucket counts, weight scale
+    WS_QS_TAB,     // separable Q: phasor and weight tables
+    WS_QS_PART,    // separable Q: K-block partial sums and job table
     WS_COUNT
 };
 
@@ -88,6 +91,8 @@ struct imp_ctx {
     float2 *twiddle = nullptr;
     // live Toeplitz operators created on this context
     std::set<struct imp_toep *> toeps;
+    // live networks created on this context
+    std::set<struct imp_cnn *> cnns;
     // kernel timer (imp_profile_*): off by default
     bool prof_on = false;
     std::vector<imp::ProfRec> prof_recs;   // recorded this session
@@ -99,6 +104,10 @@ namespace imp {
 // Returns a device scratch region of at least `bytes` (grows, never shrinks).
 int workspace(imp_ctx *ctx, int slot, size_t bytes, void **out);
 int ensure_dscratch(imp_ctx *ctx, size_t n);
+
+// Frees a network's device memory and detaches it from its context (for
+// imp_ctx_destroy; a later imp_cnn_destroy then frees only the host object).
+void detach_cnn(struct imp_cnn *net);
 
 // Kernel timer: prof_begin records an event before a launch and returns a
 // record index (-1 when the timer is off); prof_end records the closing event.

@@ -166,6 +166,8 @@ extern "C" int imp_ctx_destroy(imp_ctx *ctx) {
         op->ctx = nullptr;
     }
     ctx->toeps.clear();
+    for (imp_cnn *net : ctx->cnns) detach_cnn(net);
+    ctx->cnns.clear();
     prof_release(ctx);
     for (hipEvent_t e : ctx->prof_free) (void)hipEventDestroy(e);
     for (auto &w : ctx->ws)

@@ -36,6 +36,7 @@
 // chunk partials are reduced in FP64.
 #include "common.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <vector>
@@ -939,6 +940,643 @@ int nudft_forward(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *k
     return run_job(ctx, J);
 }
 
+namespace {
+
+// ---- Q on the standard grid as a separable GEMM ------------------------------
+//
+// On the standard grid e^{i 2pi k.r} = ex(k, x) ey(k, y) ez(k, z), so
+//     Q_l[y][x][z] = sum_k (a_l(t_k) |phi_k|^2 ey(k, y) ez(k, z)) ex(k, x),
+// a complex GEMM with rows (y, z), columns x and the samples as K.  Its
+// operands come from per-sample phasor tables: one FP64 sincos per sample
+// and grid line instead of one per sample and grid point.  a_l(t) is
+// nonzero only for (t - t0)/tau in (l - 1, l + 1), so a stable counting
+// sort puts the samples in buckets b = floor((t - t0)/tau) + 1 (each padded
+// to a whole step of 8 samples) and segment l runs over buckets l and l + 1
+// only: about 2/L of the samples instead of all of them.
+// Rows y < 0 (iy < Ny) are computed.  Rows y > 0 are their mirrors
+// Q(-r) = conj(Q(r)) (the weights are real), except where the mirror leaves
+// the grid (x = -Nx, z = -Nz); those points and the row y = 0 go through the
+// direct per-point kernel.
+// GEMM layout: one wave owns 64 rows, two 32-row M tiles whose A operands
+// are built in registers from the ey/ez tables and the wave-uniform weights
+// of its segment, split into fp16 hi/lo and swapped into [Re | Im] K halves
+// as in k_nudft_mfma.  The wave also owns QS_TILES 16-column N tiles whose
+// B operands (ex, split, arranged per lane) the WM waves of a workgroup share
+// through LDS.  A workgroup covers one K block of one segment; the K-block
+// partials are reduced in FP64.
+
+constexpr int QS_TILES = 4;     // N tiles (16 complex columns each) per wave
+constexpr int QS_BLK = 1024;    // samples per counting-sort block
+constexpr int QS_MAXBK = 33;    // buckets: nseg + 1 <= 33
+constexpr int QS_BSHIFT = 10;   // ex is stored scaled by 2^10 (keeps lo parts normal)
+
+struct QsJob {
+    int l;       // segment
+    int s0, s1;  // steps [s0, s1) of the bucketed sample order
+    int pad;
+};
+
+// bucket of a sample; -1 when a_l(t) = 0 for every l
+__device__ __forceinline__ int q_bucket(float t, float tau, float t0, int nseg) {
+    const double a = ((double)t - (double)t0) / (double)tau;
+    if (!(a > -1.0) || !(a < (double)nseg)) return -1;
+    return (int)floor(a) + 1;
+}
+
+__global__ void k_fill_i32(int *p, long n, int v) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// per block of QS_BLK samples: cnt[b][blk]
+__global__ __launch_bounds__(256) void k_qb_count(int n, const float *t, float tau, float t0,
+                                                  int nseg, int nblk, int *cnt) {
+    __shared__ int h[QS_MAXBK];
+    const int nbk = nseg + 1;
+    for (int i = threadIdx.x; i < nbk; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < QS_BLK; i += blockDim.x) {
+        const long k = (long)blockIdx.x * QS_BLK + i;
+        if (k < n) {
+            const int b = q_bucket(t[k], tau, t0, nseg);
+            if (b >= 0) atomicAdd(&h[b], 1);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nbk; i += blockDim.x) cnt[(size_t)i * nblk + blockIdx.x] = h[i];
+}
+
+// counts -> positions (bucket-major; every bucket padded to a whole step):
+// cnt[b][blk] becomes the position of block blk's first sample of bucket b;
+// bstart[b] = first position of bucket b, bstart[nbk] = padded total.
+// One workgroup of 1024 threads scans each bucket's block counts.
+__global__ __launch_bounds__(1024) void k_qb_scan(int nblk, int nbk, int *cnt, int *bstart) {
+    __shared__ int wsum[16];
+    __shared__ int carry;
+    __shared__ int start[QS_MAXBK + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) start[0] = 0;
+    for (int b = 0; b < nbk; ++b) {
+        if (tid == 0) carry = 0;
+        __syncthreads();
+        for (int c0 = 0; c0 < nblk; c0 += 1024) {
+            const int k = c0 + tid;
+            const int v = k < nblk ? cnt[(size_t)b * nblk + k] : 0;
+            int x = v;
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const int y = __shfl_up(x, d, 64);
+                if (lane >= d) x += y;
+            }
+            if (lane == 63) wsum[wave] = x;
+            __syncthreads();
+            if (wave == 0) {
+                int w = lane < 16 ? wsum[lane] : 0;
+#pragma unroll
+                for (int d = 1; d < 16; d <<= 1) {
+                    const int y = __shfl_up(w, d, 64);
+                    if (lane >= d) w += y;
+                }
+                if (lane < 16) wsum[lane] = w;
+            }
+            __syncthreads();
+            if (k < nblk)
+                cnt[(size_t)b * nblk + k] = start[b] + carry + (wave > 0 ? wsum[wave - 1] : 0) + x - v;
+            __syncthreads();
+            if (tid == 0) carry += wsum[15];
+            __syncthreads();
+        }
+        if (tid == 0) start[b + 1] = start[b] + (carry + MF_K - 1) / MF_K * MF_K;
+        __syncthreads();
+    }
+    if (tid <= nbk) bstart[tid] = start[tid];
+}
+
+// stable scatter: perm[position] = sample index (padding keeps -1)
+__global__ __launch_bounds__(256) void k_qb_scatter(int n, const float *t, float tau, float t0,
+                                                    int nseg, int nblk, const int *off,
+                                                    int *perm) {
+    __shared__ int base[QS_MAXBK];
+    __shared__ int wcnt[4][QS_MAXBK];
+    const int nbk = nseg + 1;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int i = threadIdx.x; i < nbk; i += 256) base[i] = off[(size_t)i * nblk + blockIdx.x];
+    __syncthreads();
+    for (int it = 0; it < QS_BLK / 256; ++it) {
+        const long k = (long)blockIdx.x * QS_BLK + it * 256 + threadIdx.x;
+        const int b = k < n ? q_bucket(t[k], tau, t0, nseg) : -1;
+        int rank = 0;
+        for (int c = 0; c < nbk; ++c) {
+            const unsigned long long m = __ballot(b == c);
+            if (b == c) rank = __popcll(m & ((1ull << lane) - 1ull));
+            if (lane == 0) wcnt[wave][c] = __popcll(m);
+        }
+        __syncthreads();
+        if (b >= 0) {
+            int p = base[b] + rank;
+            for (int w = 0; w < wave; ++w) p += wcnt[w][b];
+            perm[p] = (int)k;
+        }
+        __syncthreads();
+        if (threadIdx.x < nbk)
+            base[threadIdx.x] += wcnt[0][threadIdx.x] + wcnt[1][threadIdx.x] +
+                                 wcnt[2][threadIdx.x] + wcnt[3][threadIdx.x];
+        __syncthreads();
+    }
+}
+
+// wq[l][p] = a_l(t) |phi|^2 2^shift of the sample at position p (0 on padding)
+__global__ void k_qt_w(int kp, int nseg, int num_k, const int *perm, const float2 *W,
+                       const unsigned int *maxbits, float *wq) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)kp * nseg) return;
+    const int l = (int)(i / kp), p = (int)(i % kp);
+    const int k = perm[p];
+    const float scale = ldexpf(1.f, w_shift(*maxbits));
+    wq[i] = k >= 0 ? W[(size_t)l * num_k + k].x * scale : 0.f;
+}
+
+// revolutions kc (i - Nc)/Nc in FP64, reduced to [-1/2, 1/2]
+__device__ __forceinline__ double q_rev(float kc, int i, int Nc) {
+    const double u = (double)kc * (double)(i - Nc) / (double)Nc;
+    return u - rint(u);
+}
+
+// tab[step][line][j] = e^{i 2pi kc (line - Nc)/Nc} of the sample at
+// position 8 step + j, lines [0, nl); 0 on padding
+__global__ void k_qt_line(int kp, int nl, int Nc, const int *perm, const float *kc,
+                          float2 *tab) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)kp * nl) return;
+    const int j = (int)(i & (MF_K - 1));
+    const long sl = i / MF_K;  // step * nl + line
+    const int line = (int)(sl % nl);
+    const long step = sl / nl;
+    const int k = perm[step * MF_K + j];
+    float2 v = make_float2(0.f, 0.f);
+    if (k >= 0) {
+        double s, c;
+        sincospi(2.0 * q_rev(kc[k], line, Nc), &s, &c);
+        v = make_float2((float)c, (float)s);
+    }
+    tab[i] = v;
+}
+
+__device__ __forceinline__ void split_h(double v, _Float16 &hi, _Float16 &lo) {
+    hi = (_Float16)(float)v;
+    lo = (_Float16)(float)(v - (double)(float)hi);
+}
+
+// B operands of the Q GEMM, [step][tile][hi, lo][lane] half8, ntp tiles of
+// 16 x values.  Lane l: column n = l%32 -> part n/16 of the product (Re, Im)
+// and x = 16 tile + n%16; K rows (h = l/32, sample j): h = 0 multiplies
+// Re A, h = 1 multiplies Im A, so Re = Ar Br - Ai Bi, Im = Ar Bi + Ai Br.
+// One thread per (step, x) evaluates its 8 phasors and writes the 4 lanes.
+__global__ void k_qt_bx(int n_steps, int ntp, int C2, int Nx, const int *perm, const float *kx,
+                        half8 *bx) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)n_steps * ntp * 16) return;
+    const int xl = (int)(i % (ntp * 16));
+    const long step = i / (ntp * 16);
+    const int tile = xl >> 4, xx = xl & 15;
+    const double sc = (double)(1 << QS_BSHIFT);
+    half8 ch, cl, sh, sl, nsh, nsl;
+#pragma unroll
+    for (int j = 0; j < MF_K; ++j) {
+        const int k = perm[step * MF_K + j];
+        double s = 0.0, c = 0.0;
+        if (k >= 0 && xl < C2) sincospi(2.0 * q_rev(kx[k], xl, Nx), &s, &c);
+        _Float16 a, b;
+        split_h(c * sc, a, b);
+        ch[j] = a;
+        cl[j] = b;
+        split_h(s * sc, a, b);
+        sh[j] = a;
+        sl[j] = b;
+        nsh[j] = -a;
+        nsl[j] = -b;
+    }
+    half8 *hi = bx + ((size_t)step * ntp + tile) * 2 * 64;
+    half8 *lo = hi + 64;
+    hi[xx] = ch;       lo[xx] = cl;        // h = 0, Re column: Br
+    hi[16 + xx] = sh;  lo[16 + xx] = sl;   // h = 0, Im column: Bi
+    hi[32 + xx] = nsh; lo[32 + xx] = nsl;  // h = 1, Re column: -Bi
+    hi[48 + xx] = ch;  lo[48 + xx] = cl;   // h = 1, Im column: Br
+}
+
+template <int WM>
+struct QsLds {
+    float cmp[WM][2 * QS_TILES * 16][64];
+    half8 b[2][QS_TILES][2][64];
+};
+
+// One workgroup = WM waves x 64 rows, QS_TILES N tiles, one K block (job).
+// B operands of the next step are staged into the other LDS buffer while
+// the current one is used (one barrier per step); A data of the next step
+// are prefetched into registers.  part[job][Re|Im][R][C2].
+template <int WM, bool D3>
+__global__ __launch_bounds__(64 * WM) __attribute__((amdgpu_waves_per_eu(1, 1)))
+void k_q_sep(const float4 *__restrict__ ey, const float4 *__restrict__ ez,
+             const float *__restrict__ wq, const half8 *__restrict__ bx,
+             const QsJob *__restrict__ jobs, int R, int Ny, int M3, int ntp, int kp, int C2,
+             float *__restrict__ part, float one) {
+    constexpr int NA = 2 * QS_TILES;
+    constexpr int NT = 64 * WM;
+    constexpr int NV = QS_TILES * 2 * 64;  // half8 of B per step and workgroup
+    constexpr int NB = (NV + NT - 1) / NT;
+    __shared__ QsLds<WM> lds;
+    auto &cmp = lds.cmp;
+    const int l = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int h = l >> 5;
+    const QsJob J = jobs[blockIdx.z];
+    const int row0 = (blockIdx.y * WM + wave) * 64;
+    const int row = row0 + l;
+    const bool rv = row < R;
+    const int iy = rv ? row / M3 : 0, iz = rv ? row % M3 : 0;
+    const int tile0 = blockIdx.x * QS_TILES;
+    const float *__restrict__ wl = wq + (size_t)J.l * kp;
+    auto bsrc = [&](int s) { return bx + ((size_t)s * ntp + tile0) * 2 * 64; };
+
+    floatx16 acc[NA];
+    float tot[NA][16];
+#pragma unroll
+    for (int q = 0; q < NA; ++q) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            acc[q][i] = 0.f;
+            tot[q][i] = 0.f;
+            cmp[wave][q * 16 + i][l] = 0.f;
+        }
+    }
+    {
+        const half8 *src = bsrc(J.s0);
+        half8 *dst = &lds.b[0][0][0][0];
+        for (int v = threadIdx.x; v < NV; v += NT) dst[v] = src[v];
+    }
+    float4 an[4], zn[4];
+    auto load_a = [&](int s, float4 (&e)[4], float4 (&z)[4]) {
+        const float4 *pe = ey + ((size_t)s * Ny + iy) * 4;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) e[q] = rv ? pe[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+        if constexpr (D3) {
+            const float4 *pz = ez + ((size_t)s * M3 + iz) * 4;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) z[q] = rv ? pz[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    };
+    load_a(J.s0, an, zn);
+    __syncthreads();
+
+    for (int f0 = J.s0; f0 < J.s1; f0 += MF_FOLD) {
+        const int f1 = min(J.s1, f0 + MF_FOLD);
+        for (int s = f0; s < f1; ++s) {
+            const int buf = (s - J.s0) & 1;
+            const bool more = s + 1 < J.s1;
+            half8 bn[NB];
+            if (more) {
+                const half8 *src = bsrc(s + 1);
+#pragma unroll
+                for (int u = 0; u < NB; ++u) {
+                    const int v = threadIdx.x + u * NT;
+                    if (v < NV) bn[u] = src[v];
+                }
+            }
+            float4 ae[4], ze[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                ae[q] = an[q];
+                if constexpr (D3) ze[q] = zn[q];
+            }
+            if (more) load_a(s + 1, an, zn);
+            const float *__restrict__ w8 = wl + (size_t)s * MF_K;
+            half8 rh, ih, rl, il;
+#pragma unroll
+            for (int j = 0; j < MF_K; ++j) {
+                const float4 e4 = ae[j >> 1];
+                float pr = (j & 1) ? e4.z : e4.x;
+                float pi = (j & 1) ? e4.w : e4.y;
+                if constexpr (D3) {
+                    const float4 z4 = ze[j >> 1];
+                    const float zr = (j & 1) ? z4.z : z4.x;
+                    const float zi = (j & 1) ? z4.w : z4.y;
+                    const float tr = pr * zr - pi * zi;
+                    pi = pr * zi + pi * zr;
+                    pr = tr;
+                }
+                const float w = w8[j];
+                float ar = w * pr, ai = w * pi;
+                // Opaque to the optimizer: otherwise (_Float16)(w * pr) may be
+                // re-derived as one fused multiply-convert (v_fma_mixlo_f16,
+                // single rounding) for the lo part while the packed hi part
+                // rounds the FP32 product (double rounding).  Where the two
+                // differ, hi + lo misses w * pr by one fp16 ulp.
+                asm("" : "+v"(ar), "+v"(ai));
+                const _Float16 ar16 = (_Float16)ar, ai16 = (_Float16)ai;
+                rh[j] = ar16;
+                ih[j] = ai16;
+                rl[j] = (_Float16)__builtin_fmaf(ar, one, -(float)ar16);
+                il[j] = (_Float16)__builtin_fmaf(ai, one, -(float)ai16);
+            }
+            const u32x4 rhu = __builtin_bit_cast(u32x4, rh), ihu = __builtin_bit_cast(u32x4, ih);
+            const u32x4 rlu = __builtin_bit_cast(u32x4, rl), ilu = __builtin_bit_cast(u32x4, il);
+            u32x4 hA, hB, lA, lB;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const auto ph = __builtin_amdgcn_permlane32_swap(rhu[i], ihu[i], false, false);
+                hA[i] = ph[0];
+                hB[i] = ph[1];
+                const auto pl = __builtin_amdgcn_permlane32_swap(rlu[i], ilu[i], false, false);
+                lA[i] = pl[0];
+                lB[i] = pl[1];
+            }
+            const half8 AhA = __builtin_bit_cast(half8, hA), AhB = __builtin_bit_cast(half8, hB);
+            const half8 AlA = __builtin_bit_cast(half8, lA), AlB = __builtin_bit_cast(half8, lB);
+            half8 bh[QS_TILES], bl[QS_TILES];
+#pragma unroll
+            for (int t = 0; t < QS_TILES; ++t) {
+                bh[t] = lds.b[buf][t][0][l];
+                bl[t] = lds.b[buf][t][1][l];
+            }
+#pragma unroll
+            for (int t = 0; t < QS_TILES; ++t) {
+                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(AhA, bh[t], acc[t], 0, 0, 0);
+                acc[QS_TILES + t] =
+                    __builtin_amdgcn_mfma_f32_32x32x16_f16(AhB, bh[t], acc[QS_TILES + t], 0, 0, 0);
+            }
+#pragma unroll
+            for (int t = 0; t < QS_TILES; ++t) {
+                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(AhA, bl[t], acc[t], 0, 0, 0);
+                acc[QS_TILES + t] =
+                    __builtin_amdgcn_mfma_f32_32x32x16_f16(AhB, bl[t], acc[QS_TILES + t], 0, 0, 0);
+            }
+#pragma unroll
+            for (int t = 0; t < QS_TILES; ++t) {
+                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(AlA, bh[t], acc[t], 0, 0, 0);
+                acc[QS_TILES + t] =
+                    __builtin_amdgcn_mfma_f32_32x32x16_f16(AlB, bh[t], acc[QS_TILES + t], 0, 0, 0);
+            }
+            if (more) {
+                half8 *dst = &lds.b[buf ^ 1][0][0][0];
+#pragma unroll
+                for (int u = 0; u < NB; ++u) {
+                    const int v = threadIdx.x + u * NT;
+                    if (v < NV) dst[v] = bn[u];
+                }
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int q = 0; q < NA; ++q) {
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                float sum, err;
+                two_sum(tot[q][i], acc[q][i], sum, err);
+                tot[q][i] = sum;
+                cmp[wave][q * 16 + i][l] += err;
+                acc[q][i] = 0.f;
+            }
+        }
+    }
+    // D layout of 32x32 MFMA: lane l, register i -> column l%32,
+    // row 8*(i/4) + 4*(l/32) + i%4
+    const int n = l & 31;
+    const size_t plane = (size_t)R * C2;
+    float *__restrict__ out = part + (size_t)blockIdx.z * 2 * plane + (size_t)(n >> 4) * plane;
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+#pragma unroll
+        for (int t = 0; t < QS_TILES; ++t) {
+            const int q = m * QS_TILES + t;
+            const int x = (tile0 + t) * 16 + (n & 15);
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int r = row0 + m * 32 + 8 * (i >> 2) + 4 * h + (i & 3);
+                if (r < R && x < C2) out[(size_t)r * C2 + x] = tot[q][i] + cmp[wave][q * 16 + i][l];
+            }
+        }
+    }
+}
+
+// K-block partials -> Q on the rows iy < Ny and their mirrors; FP64 sums,
+// scales undone; cumulative: prefix sums over l.
+__global__ void k_qs_reduce(const float *part, const int *jb, int nseg, int R, int C2, int Nx,
+                            int Ny, int Nz, int M3, int cumulative, const unsigned int *maxbits,
+                            float *q_r, float *q_i) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long)R * C2) return;
+    const int x = (int)(i % C2), r = (int)(i / C2);
+    const int iy = r / M3, iz = r % M3;
+    const long nq = 2L * Ny * C2 * M3;
+    const long g = ((long)iy * C2 + x) * M3 + iz;
+    const bool mir = iy > 0 && x > 0 && (M3 == 1 || iz > 0);
+    const long gm = mir ? ((long)(2 * Ny - iy) * C2 + (2 * Nx - x)) * M3 + (M3 > 1 ? 2 * Nz - iz : 0)
+                        : 0;
+    const double unscale = ldexp(1.0, -w_shift(*maxbits) - QS_BSHIFT);
+    const size_t plane = (size_t)R * C2;
+    double cr = 0.0, ci = 0.0;
+    for (int l = 0; l < nseg; ++l) {
+        double sr = 0.0, si = 0.0;
+        for (int j = jb[l]; j < jb[l + 1]; ++j) {
+            sr += part[(size_t)j * 2 * plane + i];
+            si += part[((size_t)j * 2 + 1) * plane + i];
+        }
+        sr *= unscale;
+        si *= unscale;
+        if (cumulative) {
+            cr += sr;
+            ci += si;
+            sr = cr;
+            si = ci;
+        }
+        q_r[(size_t)l * nq + g] = (float)sr;
+        q_i[(size_t)l * nq + g] = (float)si;
+        if (mir) {
+            q_r[(size_t)l * nq + gm] = (float)sr;
+            q_i[(size_t)l * nq + gm] = (float)-si;
+        }
+    }
+}
+
+// Points the GEMM path evaluates directly: the row y = 0 (iy = Ny), then the
+// points of rows y > 0 whose mirror leaves the grid (qhalf_edge order).
+__device__ __forceinline__ void qedge_point(int j, int Nx, int Ny, int Nz, int ne_row, int &iy,
+                                            int &ix, int &iz) {
+    const int M2 = 2 * Nx, M3 = Nz > 1 ? 2 * Nz : 1;
+    if (j < M2 * M3) {
+        iy = Ny;
+        ix = j / M3;
+        iz = j % M3;
+    } else {
+        qhalf_edge(j - M2 * M3, Ny, M2, M3, ne_row, iy, ix, iz);
+    }
+}
+
+__global__ void k_lanes_qedge(int Nx, int Ny, int Nz, int ne_row, int n_edge, float4 *lanes) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_edge) return;
+    int iy, ix, iz;
+    qedge_point(j, Nx, Ny, Nz, ne_row, iy, ix, iz);
+    lanes[j] = make_float4((float)(ix - Nx) / (float)Nx, (float)(iy - Ny) / (float)Ny,
+                           Nz > 1 ? (float)(iz - Nz) / (float)Nz : 0.f, 0.f);
+}
+
+__global__ void k_q_edge_fill(int Nx, int Ny, int Nz, int ne_row, int n_edge, int nseg,
+                              const float *hr, const float *hi, float *q_r, float *q_i) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n_edge) return;
+    int iy, ix, iz;
+    qedge_point(j, Nx, Ny, Nz, ne_row, iy, ix, iz);
+    const int M2 = 2 * Nx, M3 = Nz > 1 ? 2 * Nz : 1;
+    const long nq = 2L * Ny * M2 * M3;
+    const long g = ((long)iy * M2 + ix) * M3 + iz;
+    for (int l = 0; l < nseg; ++l) {
+        q_r[(size_t)l * nq + g] = hr[(size_t)l * n_edge + j];
+        q_i[(size_t)l * nq + g] = hi[(size_t)l * n_edge + j];
+    }
+}
+
+using QsFn = void (*)(const float4 *, const float4 *, const float *, const half8 *,
+                      const QsJob *, int, int, int, int, int, int, float *, float);
+
+QsFn pick_qsep(int wm, bool d3) {
+    switch (wm) {
+        case 1: return d3 ? (QsFn)k_q_sep<1, true> : (QsFn)k_q_sep<1, false>;
+        case 2: return d3 ? (QsFn)k_q_sep<2, true> : (QsFn)k_q_sep<2, false>;
+        case 3: return d3 ? (QsFn)k_q_sep<3, true> : (QsFn)k_q_sep<3, false>;
+        default: return d3 ? (QsFn)k_q_sep<4, true> : (QsFn)k_q_sep<4, false>;
+    }
+}
+
+// Standard-grid Q by the separable GEMM.  W = the k_w_q weights
+// [nseg][num_k]; spos = the direct kernel's sample stream (edge points).
+int toeplitz_q_sep(imp_ctx *ctx, int num_k, const float *kx, const float *ky, const float *kz,
+                   const float *t, const double *spos, const float2 *W, bool d3, int Nx, int Ny,
+                   int Nz, int nseg, float tau, float t0, bool cumulative, float *q_r,
+                   float *q_i) {
+    const hipStream_t st = ctx->stream;
+    const int M2 = 2 * Nx, M3 = d3 ? 2 * Nz : 1;
+    const int nbk = nseg + 1;
+    const int nblk = ceil_div(num_k, QS_BLK);
+    const long kp_max = (long)num_k + (long)nbk * MF_K;
+    // 1. bucket the samples by segment (stable counting sort)
+    void *sw = nullptr;
+    const size_t n_cnt = (size_t)nbk * nblk, n_bs = 64;
+    IMP_TRY(workspace(ctx, WS_QS_SORT, (n_cnt + n_bs + 64 + (size_t)kp_max) * sizeof(int), &sw));
+    int *cnt = (int *)sw, *bstart = cnt + n_cnt;
+    unsigned int *maxbits = (unsigned int *)(bstart + n_bs);
+    int *perm = bstart + n_bs + 64;
+    hipLaunchKernelGGL(k_fill_i32, dim3(ceil_div(kp_max, 256)), dim3(256), 0, st, perm, kp_max,
+                       -1);
+    hipLaunchKernelGGL(k_qb_count, dim3(nblk), dim3(256), 0, st, num_k, t, tau, t0, nseg, nblk,
+                       cnt);
+    hipLaunchKernelGGL(k_qb_scan, dim3(1), dim3(1024), 0, st, nblk, nbk, cnt, bstart);
+    hipLaunchKernelGGL(k_qb_scatter, dim3(nblk), dim3(256), 0, st, num_k, t, tau, t0, nseg, nblk,
+                       (const int *)cnt, perm);
+    IMP_HIP(hipMemsetAsync(maxbits, 0, sizeof(unsigned int), st));
+    hipLaunchKernelGGL(k_absmax, dim3(512), dim3(256), 0, st, W, (size_t)nseg * num_k, maxbits);
+    IMP_CHECK_LAUNCH();
+    std::vector<int> bs(nbk + 1);
+    IMP_HIP(hipMemcpyAsync(bs.data(), bstart, (nbk + 1) * sizeof(int), hipMemcpyDeviceToHost, st));
+    IMP_HIP(hipStreamSynchronize(st));
+    const int kp = bs[nbk];
+    const int n_steps = kp / MF_K;
+    // 2. jobs: segment l runs over buckets l and l + 1, split into K blocks
+    //    so the launch holds ~16 workgroups per CU
+    const int R = Ny * M3;
+    const int wm = R >= 256 ? 4 : ceil_div(R, 64);
+    const int ntx = ceil_div(M2, 16);
+    const int gx = ceil_div(ntx, QS_TILES), gy = ceil_div(R, 64 * wm);
+    const int ntp = gx * QS_TILES;
+    long tot_steps = 0;
+    for (int l = 0; l < nseg; ++l) tot_steps += (bs[l + 2] - bs[l]) / MF_K;
+    const long target = 16L * ctx->num_cus;
+    long kb = tot_steps * gx * gy / std::max(1L, target);
+    kb = std::min(2048L, std::max((long)MF_FOLD * 2, (kb + MF_FOLD - 1) / MF_FOLD * MF_FOLD));
+    std::vector<QsJob> jobs;
+    std::vector<int> jb(nseg + 1, 0);
+    for (int l = 0; l < nseg; ++l) {
+        jb[l] = (int)jobs.size();
+        const int a = bs[l] / MF_K, b = bs[l + 2] / MF_K;
+        for (int s = a; s < b; s += (int)kb) jobs.push_back(QsJob{l, s, std::min(b, s + (int)kb), 0});
+    }
+    jb[nseg] = (int)jobs.size();
+    const int njobs = (int)jobs.size();
+    if (njobs > 65535) return fail(IMP_EUNSUPPORTED, "toeplitz_q: %d K blocks", njobs);
+    // 3. tables: wq [nseg][kp], ey [step][Ny][8], ez [step][M3][8], bx
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t b_wq = al((size_t)nseg * kp * sizeof(float));
+    const size_t b_ey = al((size_t)kp * Ny * sizeof(float2));
+    const size_t b_ez = d3 ? al((size_t)kp * M3 * sizeof(float2)) : 0;
+    const size_t b_bx = al((size_t)n_steps * ntp * 2 * 64 * sizeof(half8));
+    void *tw = nullptr;
+    IMP_TRY(workspace(ctx, WS_QS_TAB, b_wq + b_ey + b_ez + b_bx + 256, &tw));
+    float *wq = (float *)tw;
+    float2 *ey = (float2 *)((char *)tw + b_wq);
+    float2 *ez = (float2 *)((char *)tw + b_wq + b_ey);
+    half8 *bx = (half8 *)((char *)tw + b_wq + b_ey + b_ez);
+    if (kp > 0) {
+        hipLaunchKernelGGL(k_qt_w, dim3(ceil_div((long)kp * nseg, 256)), dim3(256), 0, st, kp,
+                           nseg, num_k, (const int *)perm, W, (const unsigned int *)maxbits, wq);
+        hipLaunchKernelGGL(k_qt_line, dim3(ceil_div((long)kp * Ny, 256)), dim3(256), 0, st, kp,
+                           Ny, Ny, (const int *)perm, ky, ey);
+        if (d3)
+            hipLaunchKernelGGL(k_qt_line, dim3(ceil_div((long)kp * M3, 256)), dim3(256), 0, st,
+                               kp, M3, Nz, (const int *)perm, kz, ez);
+        hipLaunchKernelGGL(k_qt_bx, dim3(ceil_div((long)n_steps * ntp * 16, 256)), dim3(256), 0,
+                           st, n_steps, ntp, M2, Nx, (const int *)perm, kx, bx);
+        IMP_CHECK_LAUNCH();
+    }
+    // 4. GEMM over K blocks, FP64 reduce with the mirror fill
+    const size_t plane = (size_t)R * M2;
+    const size_t b_part = al((size_t)std::max(1, njobs) * 2 * plane * sizeof(float));
+    void *pw = nullptr;
+    IMP_TRY(workspace(ctx, WS_QS_PART, b_part + al(njobs * sizeof(QsJob)) + al(jb.size() * 4),
+                      &pw));
+    float *part = (float *)pw;
+    QsJob *djobs = (QsJob *)((char *)pw + b_part);
+    int *djb = (int *)((char *)pw + b_part + al(njobs * sizeof(QsJob)));
+    if (njobs > 0)
+        IMP_HIP(hipMemcpyAsync(djobs, jobs.data(), njobs * sizeof(QsJob), hipMemcpyHostToDevice,
+                               st));
+    IMP_HIP(hipMemcpyAsync(djb, jb.data(), jb.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    if (njobs > 0) {
+        const int rec = prof_begin(ctx, "k_q_sep");
+        hipLaunchKernelGGL(pick_qsep(wm, d3), dim3(gx, gy, njobs), dim3(64 * wm), 0, st,
+                           (const float4 *)ey, (const float4 *)ez, (const float *)wq,
+                           (const half8 *)bx, (const QsJob *)djobs, R, Ny, M3, ntp, kp, M2, part,
+                           1.0f);
+        prof_end(ctx, rec);
+        IMP_CHECK_LAUNCH();
+    }
+    hipLaunchKernelGGL(k_qs_reduce, dim3(ceil_div((long)plane, 256)), dim3(256), 0, st,
+                       (const float *)part, (const int *)djb, nseg, R, M2, Nx, Ny, Nz, M3,
+                       cumulative ? 1 : 0, (const unsigned int *)maxbits, q_r, q_i);
+    IMP_CHECK_LAUNCH();
+    // 5. the row y = 0 and the mirror-less points of rows y > 0, directly
+    const int ne_row = M3 > 1 ? M2 + M3 - 1 : 1;
+    const int n_edge = M2 * M3 + (Ny - 1) * ne_row;
+    void *lanes = nullptr, *half = nullptr;
+    IMP_TRY(workspace(ctx, WS_LANES, (size_t)n_edge * sizeof(float4), &lanes));
+    hipLaunchKernelGGL(k_lanes_qedge, dim3(ceil_div(n_edge, 256)), dim3(256), 0, st, Nx, Ny,
+                       d3 ? Nz : 1, ne_row, n_edge, (float4 *)lanes);
+    IMP_CHECK_LAUNCH();
+    IMP_TRY(workspace(ctx, WS_QHALF, 2 * (size_t)nseg * n_edge * sizeof(float), &half));
+    float *h_r = (float *)half, *h_i = h_r + (size_t)nseg * n_edge;
+    Job J{(const float4 *)lanes, n_edge, spos, num_k, W, nseg, d3, +1.f,
+          cumulative ? Out::PerRhsCumulative : Out::PerRhs, nullptr, nullptr, h_r, h_i};
+    IMP_TRY(run_job(ctx, J));
+    hipLaunchKernelGGL(k_q_edge_fill, dim3(ceil_div(n_edge, 256)), dim3(256), 0, st, Nx, Ny,
+                       d3 ? Nz : 1, ne_row, n_edge, nseg, (const float *)h_r, (const float *)h_i,
+                       q_r, q_i);
+    IMP_CHECK_LAUNCH();
+    // the host job table must outlive its copy
+    IMP_HIP(hipStreamSynchronize(st));
+    return IMP_OK;
+}
+
+}  // namespace
+
 int toeplitz_q(imp_ctx *ctx, int num_k, const float *kx, const float *ky,
                const float *kz, const float *t, const float *phi_r,
                const float *phi_i, int Nx, int Ny, int Nz, int num_q,
@@ -970,13 +1608,25 @@ int toeplitz_q(imp_ctx *ctx, int num_k, const float *kx, const float *ky,
     const float tau = ntime_segs > 0 ? (tlast - t0) / (float)ntime_segs : (tlast - t0);
     const double *spos = nullptr;
     IMP_TRY(build_spos(ctx, num_k, kx, ky, d3 ? kz : nullptr, nullptr, 0.0, &spos));
+    void *lanes = nullptr, *W = nullptr;
+    IMP_TRY(workspace(ctx, WS_W, (size_t)num_k * nseg * sizeof(float2), &W));
+    hipLaunchKernelGGL(k_w_q, dim3(ceil_div((long)num_k * nseg, 256)), dim3(256), 0,
+                       ctx->stream, num_k, nseg, t, phi_r, phi_i, tau, t0, (float2 *)W);
+    IMP_CHECK_LAUNCH();
+    const Out mode = (flags & IMP_Q_CUMULATIVE) ? Out::PerRhsCumulative : Out::PerRhs;
     // Standard grid (main_mri.cu:461-500; y outer, x, z inner, r = (x/Nx,
-    // y/Ny, z/Nz)): evaluate the half grid and fill by Hermitian symmetry.
+    // y/Ny, z/Nz)): the separable GEMM unless the direct kernel is asked for
+    // (or tau is degenerate); the direct kernel evaluates the half grid and
+    // fills the rest by Hermitian symmetry.
+    if (std_grid && !(flags & IMP_Q_DIRECT) && tau > 0.f && std::isfinite(tau) &&
+        std::isfinite(t0))
+        return toeplitz_q_sep(ctx, num_k, kx, ky, d3 ? kz : nullptr, t, spos, (const float2 *)W,
+                              d3, Nx, Ny, Nz, nseg, tau, t0, mode == Out::PerRhsCumulative, q_r,
+                              q_i);
     const int M2 = 2 * Nx, M3 = Nz > 1 ? 2 * Nz : 1;
     const int n1 = std_grid ? (Ny + 1) * M2 * M3 : 0;
     const int ne_row = M3 > 1 ? M2 + M3 - 1 : 1;
     const int n_comp = std_grid ? n1 + (Ny - 1) * ne_row : num_q;
-    void *lanes = nullptr, *W = nullptr;
     IMP_TRY(workspace(ctx, WS_LANES, (size_t)n_comp * sizeof(float4), &lanes));
     if (std_grid) {
         hipLaunchKernelGGL(k_lanes_qhalf, dim3(ceil_div(n_comp, 256)), dim3(256), 0, ctx->stream,
@@ -987,11 +1637,6 @@ int toeplitz_q(imp_ctx *ctx, int num_k, const float *kx, const float *ky,
                            (float4 *)lanes);
     }
     IMP_CHECK_LAUNCH();
-    IMP_TRY(workspace(ctx, WS_W, (size_t)num_k * nseg * sizeof(float2), &W));
-    hipLaunchKernelGGL(k_w_q, dim3(ceil_div((long)num_k * nseg, 256)), dim3(256), 0,
-                       ctx->stream, num_k, nseg, t, phi_r, phi_i, tau, t0, (float2 *)W);
-    IMP_CHECK_LAUNCH();
-    const Out mode = (flags & IMP_Q_CUMULATIVE) ? Out::PerRhsCumulative : Out::PerRhs;
     if (!std_grid) {
         Job J{(const float4 *)lanes, num_q, spos, num_k, (const float2 *)W, nseg, d3, +1.f,
               mode, nullptr, nullptr, q_r, q_i};

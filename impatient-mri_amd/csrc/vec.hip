@@ -202,7 +202,11 @@ __global__ __launch_bounds__(VT) void k_cg_alpha(const double *__restrict__ part
     s = block_sum(s);
     if (threadIdx.x == 0) {
         sc[SC_PQ] = s;
-        if (!(s > 0.0) || !isfinite(s)) {
+        // As the reference (recon.cu:2071-2077), alpha = rho / p^H A p of either
+        // sign: the time-segmented operator is Hermitian but only approximately
+        // positive, and the best-iterate tracking keeps the result.  Only a zero
+        // or non-finite p^H A p stops the iteration.
+        if (s == 0.0 || !isfinite(s)) {
             sc[SC_BREAK] = 1.0;
             sc[SC_ALPHA] = 0.0;
         } else {

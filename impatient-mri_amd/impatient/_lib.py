@@ -23,6 +23,7 @@ STATUS_NAMES = {0: "IMP_OK", -1: "IMP_EINVAL", -2: "IMP_EHIP", -3: "IMP_EIO",
                 -7: "IMP_EBREAKDOWN", -8: "IMP_EUNSUPPORTED"}
 
 IMP_Q_CUMULATIVE = 0x1
+IMP_Q_DIRECT = 0x2
 IMP_CG_BEST_ITERATE = 0x1
 IMP_CG_USE_X0 = 0x2
 IMP_CG_TRUE_RESIDUAL = 0x4

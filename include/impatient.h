@@ -38,7 +38,7 @@ extern "C" {
 #define IMP_ENOMEM       -4   /* allocation failed                            */
 #define IMP_EFORMAT      -5   /* malformed data file header                   */
 #define IMP_ENODEV       -6   /* no usable GPU                                */
-#define IMP_EBREAKDOWN   -7   /* CG breakdown: p^H A p <= 0 or not finite     */
+#define IMP_EBREAKDOWN   -7   /* CG breakdown: p^H A p zero or not finite      */
 #define IMP_EUNSUPPORTED -8   /* size the kernels do not support (yet)        */
 
 const char *imp_version_string(void);
@@ -144,8 +144,12 @@ int imp_nudft_adjoint_batch(imp_ctx *ctx, int njobs, const imp_nudft_job *jobs,
  * phi_r/phi_i [num_k] or NULL (phi = 1).  q_r/q_i [(ntime_segs+1)*num_q].
  * flags: IMP_Q_CUMULATIVE reproduces the reference direct path, whose device
  * accumulator is not cleared between segments (computeQ.cmem.cu:190-191,
- * 951-963), i.e. returns prefix sums over l. */
+ * 951-963), i.e. returns prefix sums over l.  On the standard grid the
+ * sums are evaluated as a separable GEMM (e^{i 2pi k.r} = e_x e_y e_z per
+ * sample, each segment over the samples where a_l != 0); IMP_Q_DIRECT
+ * evaluates every grid point with the per-point kernel instead. */
 #define IMP_Q_CUMULATIVE 0x1
+#define IMP_Q_DIRECT 0x2
 int imp_toeplitz_q(imp_ctx *ctx, int num_k,
                    const float *kx, const float *ky, const float *kz,
                    const float *t, const float *phi_r, const float *phi_i,

@@ -90,3 +90,16 @@ def test_cnn_params_roundtrip_and_errors(ctx):
         net.set_params(bad)
     with pytest.raises(ImpError):
         ops.CNN(ctx, 4, 48)
+
+
+def test_cnn_outliving_its_context_is_detached():
+    """A network whose context is destroyed first loses its device memory
+    with the context; calls on it then fail with IMP_EINVAL and destroying
+    it afterwards frees only the host object (no use of the freed context)."""
+    from impatient import ops, Context, ImpError
+    c2 = Context(0)
+    net = ops.CNN(c2, 8, 64, seed=2)
+    c2.close()
+    with pytest.raises(ImpError):
+        net.get_params()
+    net.close()

@@ -163,3 +163,73 @@ def test_adjoint_batch_matches_single_calls_and_oracle(ctx):
             e1, e2 = rel_l2(g, one), rel_l2(g, ref)
             print(f"batch N={p.Nx} C={p.ncoils} scale={scale}: vs single {e1:.2e}, vs FP64 {e2:.2e}")
             assert e1 < 1e-6 and e2 < OP_TOL
+
+
+@pytest.mark.parametrize("Nx,Ny,Nz,L", [(8, 8, 1, 3), (5, 3, 1, 0), (12, 12, 1, 12), (4, 4, 2, 2),
+                                        (3, 5, 3, 1)])
+def test_q_separable_and_direct_paths_match_oracle(ctx, Nx, Ny, Nz, L):
+    """The standard grid's separable GEMM path (default) and the per-point
+    path (IMP_Q_DIRECT) against the FP64 oracle, with |phi|^2 weights and a
+    readout whose times are not monotone and partly outside [t0, t_last]:
+    samples before t0 - tau or after t0 + (L+1) tau have zero weight in every
+    segment and samples in (t0 - tau, t0) feed segment 0 only."""
+    from impatient import ops, IMP_Q_DIRECT
+    from oracle import oracle as orc
+    rng = np.random.default_rng(40 + Nx + L)
+    M = 700
+    kx = rng.uniform(-Nx / 2, Nx / 2, M).astype(np.float32)
+    ky = rng.uniform(-Ny / 2, Ny / 2, M).astype(np.float32)
+    kz = rng.uniform(-Nz / 2, Nz / 2, M).astype(np.float32) if Nz > 1 else None
+    t = rng.uniform(-2e-3, 6e-3, M).astype(np.float32)
+    t[0], t[-1] = 0.0, 4e-3
+    phi = (1 + 0.3 * rng.standard_normal(M)) + 0.2j * rng.standard_normal(M)
+    ref = orc.toeplitz_q(kx, ky, kz, t, Nx, Ny, Nz, L, phi=phi)
+    for flags in (0, IMP_Q_DIRECT):
+        got = ops.toeplitz_q(ctx, kx, ky, kz, t, Nx, Ny, Nz, L, phi=phi, flags=flags)
+        err = rel_l2(got, ref)
+        print(f"Q {Nx}x{Ny}x{Nz} L={L} flags={flags}: rel_l2={err:.2e}")
+        assert err < 2e-6
+
+
+def test_q_separable_matches_direct_at_larger_sizes(ctx):
+    """GEMM path against the per-point path where the FP64 oracle is too slow:
+    a 128x128 2D grid (rows span two workgroups of four waves, columns 16
+    tiles) with L = 10 and a 3D 24x16x6 grid (rows not a multiple of 64)."""
+    from impatient import ops, synth, IMP_Q_DIRECT
+    cases = [(synth.make_problem("q128", 128, 1, 1, 8, 2048, 8e-3, 100.0, 3, 10, 5), 10),
+             (synth.make_problem("q3d", 16, 6, 1, 4, 512, 5e-3, 100.0, 2, 4, 6), 4)]
+    for p, L in cases:
+        kz = p.kz if p.Nz > 1 else None
+        # non-square 3D grid: stretch x
+        Nx = p.Nx if p.Nz == 1 else 24
+        a = ops.toeplitz_q(ctx, p.kx, p.ky, kz, p.t, Nx, p.Ny, p.Nz, L)
+        b = ops.toeplitz_q(ctx, p.kx, p.ky, kz, p.t, Nx, p.Ny, p.Nz, L, flags=IMP_Q_DIRECT)
+        err = rel_l2(a, b)
+        print(f"Q {Nx}x{p.Ny}x{p.Nz} L={L}: separable vs direct rel_l2={err:.2e}")
+        assert err < 2e-6
+
+
+def test_q_separable_cumulative_matches_direct(ctx):
+    from impatient import ops, IMP_Q_CUMULATIVE, IMP_Q_DIRECT
+    p = _prob(N=16, ncoils=1, ntime_segs=5)
+    a = ops.toeplitz_q(ctx, p.kx, p.ky, None, p.t, 16, 16, 1, 5, flags=IMP_Q_CUMULATIVE)
+    b = ops.toeplitz_q(ctx, p.kx, p.ky, None, p.t, 16, 16, 1, 5,
+                       flags=IMP_Q_CUMULATIVE | IMP_Q_DIRECT)
+    assert rel_l2(a, b) < 2e-6
+
+
+@pytest.mark.parametrize("N", [8, 16])
+def test_q_separable_split_is_consistent_per_segment(ctx, N):
+    """Every segment of the GEMM path within 1e-6 of the per-point path.  With
+    L = 12 these readouts contain products w * ey whose FP32 rounding and
+    direct fp16 rounding differ (segment 6 row y = -1/8 at N = 8, segment 10
+    row y = -1/16 at N = 16): the fp16 hi/lo split must take both parts from
+    the same rounded product, else that sample is off by one fp16 ulp
+    (~1e-5 of the segment)."""
+    from impatient import ops, synth, IMP_Q_DIRECT
+    p = synth.tiny(N=N, ncoils=1, ntime_segs=12)
+    a = ops.toeplitz_q(ctx, p.kx, p.ky, None, p.t, N, N, 1, 12).reshape(13, -1)
+    b = ops.toeplitz_q(ctx, p.kx, p.ky, None, p.t, N, N, 1, 12, flags=IMP_Q_DIRECT).reshape(13, -1)
+    per = [rel_l2(a[l], b[l]) for l in range(13)]
+    print(f"N={N} per-segment separable vs direct: " + " ".join(f"{e:.1e}" for e in per))
+    assert max(per) < 1e-6

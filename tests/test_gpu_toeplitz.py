@@ -134,3 +134,32 @@ def test_cg_converges_to_dense_solution(ctx):
     assert st["true_rel_residual"] < 1e-5
     assert r64 < 2e-6
     assert err <= kappa * r64 * 1.01
+
+
+def test_cg_continues_through_negative_curvature_like_the_reference(ctx):
+    """An indefinite Hermitian operator (lambda = -1000 makes the regularizer
+    dominate with the wrong sign): p^H A p < 0 at the first and fourth
+    iteration.  The CG takes alpha = rho / p^H A p of either sign as the
+    reference loop does (recon.cu:2071-2077) and matches the oracle CG
+    iterate by iterate instead of stopping."""
+    from impatient import ops, synth
+    from oracle import oracle as orc
+    p = synth.tiny(N=8, ncoils=2, ntime_segs=2, J=8, S=256)
+    q = orc.toeplitz_q(p.kx, p.ky, None, p.t, 8, 8, 1, 2)
+    tau, t0 = orc.segment_timing(p.t, 2)
+    lam = -1000.0
+    op = ops.Toeplitz(ctx, p.shape, q.astype(np.complex64), tau, t0, p.fm, p.sens, lam)
+    A = lambda v: orc.toeplitz_apply(v.reshape(8, 8), q, tau, t0, p.fm, p.sens, lam).reshape(-1)
+    rng = np.random.default_rng(12)
+    b = rng.standard_normal(p.nvox) + 1j * rng.standard_normal(p.nvox)
+    assert np.vdot(b, A(b)).real < 0   # first search direction p = b
+    x, st = op.cg(b.astype(np.complex64), 4, 0.0, 0)
+    x32, _, h32 = orc.toeplitz_cg(A, b, 4, best_iterate=False, vec_dtype=np.complex64)
+    x64, _, h64 = orc.toeplitz_cg(A, b, 4, best_iterate=False)
+    e32, e64 = rel_l2(x, x32), rel_l2(x, x64)
+    print(f"indefinite CG 4 iters: vs FP32-storage CG {e32:.2e}, vs FP64 CG {e64:.2e}, "
+          f"res {st['rel_residual']:.3e} vs {h32[-1]:.3e} / {h64[-1]:.3e}")
+    assert st["iterations"] == 4
+    assert abs(st["rel_residual"] - h32[-1]) / h32[-1] < 2e-2
+    assert e32 < 5e-3
+    assert e64 < 5e-2
