@@ -624,6 +624,39 @@ __global__ void k_reduce_mfma(const float *partial, int nchunks, int tiles, int 
     }
 }
 
+// Per-rhs output (mode 1) when there are few points and many chunks: one
+// workgroup per (64 points, rhs), its four waves split the chunks and meet
+// in LDS (FP64).
+__global__ __launch_bounds__(256) void k_reduce_mfma_rhs(const float *partial, int nchunks,
+                                                         int tiles, int n,
+                                                         const unsigned int *maxbits,
+                                                         float *out_r, float *out_i) {
+    __shared__ double red[2][4][64];
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int a = blockIdx.x * 64 + l;
+    const int c = blockIdx.y;
+    const int t = c / MF_COLS, cc = c % MF_COLS;
+    const int nplanes = tiles * 32;
+    double vr = 0.0, vi = 0.0;
+    if (a < n) {
+        for (int p = w; p < nchunks; p += 4) {
+            const size_t base = (size_t)p * nplanes + t * 32;
+            vr += partial[(base + cc) * n + a];
+            vi += partial[(base + MF_COLS + cc) * n + a];
+        }
+    }
+    red[0][w][l] = vr;
+    red[1][w][l] = vi;
+    __syncthreads();
+    if (w == 0 && a < n) {
+        const double unscale = ldexp(1.0, -w_shift(*maxbits));
+        const double sr = red[0][0][l] + red[0][1][l] + red[0][2][l] + red[0][3][l];
+        const double si = red[1][0][l] + red[1][1][l] + red[1][2][l] + red[1][3][l];
+        out_r[(size_t)c * n + a] = (float)(sr * unscale);
+        out_i[(size_t)c * n + a] = (float)(si * unscale);
+    }
+}
+
 // ---- dispatch ----------------------------------------------------------------
 
 using VFn = void (*)(const float4 *, int, const double *, const float *, int, int, float, int,
@@ -714,9 +747,13 @@ int run_job(imp_ctx *ctx, const Job &J) {
         prof_end(ctx, rec);
         IMP_CHECK_LAUNCH();
         const int mode = J.out == Out::Combine ? 0 : (J.out == Out::PerRhs ? 1 : 2);
-        hipLaunchKernelGGL(k_reduce_mfma, dim3(ceil_div(n, 256)), dim3(256), 0, st,
-                           (const float *)part, chunks, tiles, J.nrhs, n, mode, maxbits, J.s_r,
-                           J.s_i, J.out_r, J.out_i);
+        if (mode == 1 && (long)n * J.nrhs < 64L * ctx->num_cus * 16)
+            hipLaunchKernelGGL(k_reduce_mfma_rhs, dim3(ceil_div(n, 64), J.nrhs), dim3(256), 0, st,
+                               (const float *)part, chunks, tiles, n, maxbits, J.out_r, J.out_i);
+        else
+            hipLaunchKernelGGL(k_reduce_mfma, dim3(ceil_div(n, 256)), dim3(256), 0, st,
+                               (const float *)part, chunks, tiles, J.nrhs, n, mode, maxbits,
+                               J.s_r, J.s_i, J.out_r, J.out_i);
         IMP_CHECK_LAUNCH();
         return IMP_OK;
     }
@@ -1131,37 +1168,57 @@ __device__ __forceinline__ void split_h(double v, _Float16 &hi, _Float16 &lo) {
 // 16 x values.  Lane l: column n = l%32 -> part n/16 of the product (Re, Im)
 // and x = 16 tile + n%16; K rows (h = l/32, sample j): h = 0 multiplies
 // Re A, h = 1 multiplies Im A, so Re = Ar Br - Ai Bi, Im = Ar Bi + Ai Br.
-// One thread per (step, x) evaluates its 8 phasors and writes the 4 lanes.
-__global__ void k_qt_bx(int n_steps, int ntp, int C2, int Nx, const int *perm, const float *kx,
-                        half8 *bx) {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (long)n_steps * ntp * 16) return;
-    const int xl = (int)(i % (ntp * 16));
-    const long step = i / (ntp * 16);
-    const int tile = xl >> 4, xx = xl & 15;
+// A workgroup covers one step and 16 tiles (256 x values, one per thread).
+// e^{i 2pi kx (x - Nx)/Nx} = e^{i 2pi kx xl/Nx} e^{i 2pi kx (16 tile - Nx)/Nx}
+// (x = 16 tile + xl): the workgroup evaluates both factors of its 8 samples
+// in FP64 (one sincos per thread) into LDS, and each thread forms its x by
+// one complex FP64 product per sample, then writes its 4 lanes' half8.
+__global__ __launch_bounds__(256) void k_qt_bx(int ntp, int C2, int Nx, const int *perm,
+                                               const float *kx, half8 *bx) {
+    __shared__ double2 f_lo[MF_K][16];  // e^{i 2pi kx xl / Nx}
+    __shared__ double2 f_hi[MF_K][16];  // e^{i 2pi kx (16 tile - Nx) / Nx}
+    const long step = blockIdx.x;
+    const int tb = blockIdx.y * 16;
+    const int tid = threadIdx.x;
+    {
+        const int j = tid >> 5, hi = (tid >> 4) & 1, idx = tid & 15;
+        const int k = perm[step * MF_K + j];
+        double sn = 0.0, cs = 0.0;
+        if (k >= 0) {
+            const double kk = (double)kx[k];
+            double u = hi ? kk * (double)(16 * (tb + idx) - Nx) / (double)Nx
+                          : kk * (double)idx / (double)Nx;
+            sincospi(2.0 * (u - rint(u)), &sn, &cs);
+        }
+        (hi ? f_hi : f_lo)[j][idx] = make_double2(cs, sn);
+    }
+    __syncthreads();
+    const int tile = tb + (tid >> 4), xl = tid & 15;
+    if (tile >= ntp) return;
+    const bool in = tile * 16 + xl < C2;
     const double sc = (double)(1 << QS_BSHIFT);
     half8 ch, cl, sh, sl, nsh, nsl;
 #pragma unroll
     for (int j = 0; j < MF_K; ++j) {
-        const int k = perm[step * MF_K + j];
-        double s = 0.0, c = 0.0;
-        if (k >= 0 && xl < C2) sincospi(2.0 * q_rev(kx[k], xl, Nx), &s, &c);
-        _Float16 a, b;
-        split_h(c * sc, a, b);
-        ch[j] = a;
-        cl[j] = b;
-        split_h(s * sc, a, b);
-        sh[j] = a;
-        sl[j] = b;
-        nsh[j] = -a;
-        nsl[j] = -b;
+        const double2 a = f_lo[j][xl], b = f_hi[j][tid >> 4];
+        const double c = in ? a.x * b.x - a.y * b.y : 0.0;
+        const double s = in ? a.x * b.y + a.y * b.x : 0.0;
+        _Float16 p, q;
+        split_h(c * sc, p, q);
+        ch[j] = p;
+        cl[j] = q;
+        split_h(s * sc, p, q);
+        sh[j] = p;
+        sl[j] = q;
+        nsh[j] = -p;
+        nsl[j] = -q;
     }
-    half8 *hi = bx + ((size_t)step * ntp + tile) * 2 * 64;
-    half8 *lo = hi + 64;
-    hi[xx] = ch;       lo[xx] = cl;        // h = 0, Re column: Br
-    hi[16 + xx] = sh;  lo[16 + xx] = sl;   // h = 0, Im column: Bi
-    hi[32 + xx] = nsh; lo[32 + xx] = nsl;  // h = 1, Re column: -Bi
-    hi[48 + xx] = ch;  lo[48 + xx] = cl;   // h = 1, Im column: Br
+    half8 *ho = bx + ((size_t)step * ntp + tile) * 2 * 64;
+    half8 *lo = ho + 64;
+    ho[xl] = ch;       lo[xl] = cl;        // h = 0, Re column: Br
+    ho[16 + xl] = sh;  lo[16 + xl] = sl;   // h = 0, Im column: Bi
+    ho[32 + xl] = nsh; lo[32 + xl] = nsl;  // h = 1, Re column: -Bi
+    ho[48 + xl] = ch;  lo[48 + xl] = cl;   // h = 1, Im column: Br
 }
 
 template <int WM>
@@ -1172,8 +1229,8 @@ struct QsLds {
 
 // One workgroup = WM waves x 64 rows, QS_TILES N tiles, one K block (job).
 // B operands of the next step are staged into the other LDS buffer while
-// the current one is used (one barrier per step); A data of the next step
-// are prefetched into registers.  part[job][Re|Im][R][C2].
+// the current one is used (one barrier per step); global loads run two
+// steps ahead of their use.  part[job][Re|Im][R][C2].
 template <int WM, bool D3>
 __global__ __launch_bounds__(64 * WM) __attribute__((amdgpu_waves_per_eu(1, 1)))
 void k_q_sep(const float4 *__restrict__ ey, const float4 *__restrict__ ez,
@@ -1214,117 +1271,142 @@ void k_q_sep(const float4 *__restrict__ ey, const float4 *__restrict__ ez,
         half8 *dst = &lds.b[0][0][0][0];
         for (int v = threadIdx.x; v < NV; v += NT) dst[v] = src[v];
     }
-    float4 an[4], zn[4];
+    // Global loads run two steps ahead of their use.  Two register sets
+    // alternate by step parity (the loop is unrolled by two), so no register
+    // holding a load in flight is copied (a copy would wait for the load).
+    // Set e serves step s: its A data, and with B2 the B operands of step
+    // s + 1, staged into LDS at the end of step s; then it is refilled for
+    // step s + 2 (A) and s + 3 (B).  Without B2 (a thread's share of B above
+    // 4 vectors, or uneven) B is fetched one step ahead into a single set.
+    constexpr bool B2 = NB <= 4 && NV % NT == 0;
+    // Loads are unconditional (a branch around them makes the wait-count
+    // insertion wait for every load in flight).  Lanes of rows past R read
+    // row 0's data: their output rows are never written and rows of the
+    // GEMM are independent.
+    constexpr bool NV_EVEN = NV % NT == 0;
+    auto load_b = [&](int s, half8 (&r)[NB]) {
+        const half8 *src = bsrc(s);
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            const int v = threadIdx.x + u * NT;
+            if (NV_EVEN || v < NV) r[u] = src[v];
+        }
+    };
     auto load_a = [&](int s, float4 (&e)[4], float4 (&z)[4]) {
         const float4 *pe = ey + ((size_t)s * Ny + iy) * 4;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) e[q] = rv ? pe[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int q = 0; q < 4; ++q) e[q] = pe[q];
         if constexpr (D3) {
             const float4 *pz = ez + ((size_t)s * M3 + iz) * 4;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) z[q] = rv ? pz[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+            for (int q = 0; q < 4; ++q) z[q] = pz[q];
         }
     };
-    load_a(J.s0, an, zn);
+    // Prefetch indices are clamped to the last step (a harmless reload at
+    // the tail) and the B staging is unconditional (the buffer written at
+    // the last step is never read): the step body has no branches.  Jobs
+    // have an even number of steps, so the pair loop needs no test either.
+    const int s_last = J.s1 - 1;
+    float4 a0[4], z0[4], a1[4], z1[4];
+    half8 b0[NB], b1[NB];
+    load_a(J.s0, a0, z0);
+    load_a(min(J.s0 + 1, s_last), a1, z1);
+    if constexpr (B2) {
+        load_b(min(J.s0 + 1, s_last), b0);
+        load_b(min(J.s0 + 2, s_last), b1);
+    }
     __syncthreads();
+
+    auto step = [&](int s, float4 (&ae)[4], float4 (&ze)[4], half8 (&bq)[NB]) {
+        const int buf = (s - J.s0) & 1;
+        if (!B2) load_b(min(s + 1, s_last), bq);
+        const float *__restrict__ w8 = wl + (size_t)s * MF_K;
+        half8 rh, ih, rl, il;
+#pragma unroll
+        for (int j = 0; j < MF_K; ++j) {
+            const float4 e4 = ae[j >> 1];
+            float pr = (j & 1) ? e4.z : e4.x;
+            float pi = (j & 1) ? e4.w : e4.y;
+            if constexpr (D3) {
+                const float4 z4 = ze[j >> 1];
+                const float zr = (j & 1) ? z4.z : z4.x;
+                const float zi = (j & 1) ? z4.w : z4.y;
+                const float tr = pr * zr - pi * zi;
+                pi = pr * zi + pi * zr;
+                pr = tr;
+            }
+            const float w = w8[j];
+            float ar = w * pr, ai = w * pi;
+            // Opaque to the optimizer: otherwise (_Float16)(w * pr) may be
+            // re-derived as one fused multiply-convert (v_fma_mixlo_f16,
+            // single rounding) for the lo part while the packed hi part
+            // rounds the FP32 product (double rounding).  Where the two
+            // differ, hi + lo misses w * pr by one fp16 ulp.
+            asm("" : "+v"(ar), "+v"(ai));
+            const _Float16 ar16 = (_Float16)ar, ai16 = (_Float16)ai;
+            rh[j] = ar16;
+            ih[j] = ai16;
+            rl[j] = (_Float16)__builtin_fmaf(ar, one, -(float)ar16);
+            il[j] = (_Float16)__builtin_fmaf(ai, one, -(float)ai16);
+        }
+        // ae/ze consumed: refill them for step s + 2
+        load_a(min(s + 2, s_last), ae, ze);
+        const u32x4 rhu = __builtin_bit_cast(u32x4, rh), ihu = __builtin_bit_cast(u32x4, ih);
+        const u32x4 rlu = __builtin_bit_cast(u32x4, rl), ilu = __builtin_bit_cast(u32x4, il);
+        u32x4 hA, hB, lA, lB;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const auto ph = __builtin_amdgcn_permlane32_swap(rhu[i], ihu[i], false, false);
+            hA[i] = ph[0];
+            hB[i] = ph[1];
+            const auto pl = __builtin_amdgcn_permlane32_swap(rlu[i], ilu[i], false, false);
+            lA[i] = pl[0];
+            lB[i] = pl[1];
+        }
+        const half8 AhA = __builtin_bit_cast(half8, hA), AhB = __builtin_bit_cast(half8, hB);
+        const half8 AlA = __builtin_bit_cast(half8, lA), AlB = __builtin_bit_cast(half8, lB);
+        half8 bh[QS_TILES], bl[QS_TILES];
+#pragma unroll
+        for (int t = 0; t < QS_TILES; ++t) {
+            bh[t] = lds.b[buf][t][0][l];
+            bl[t] = lds.b[buf][t][1][l];
+        }
+#pragma unroll
+        for (int t = 0; t < QS_TILES; ++t) {
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(AhA, bh[t], acc[t], 0, 0, 0);
+            acc[QS_TILES + t] =
+                __builtin_amdgcn_mfma_f32_32x32x16_f16(AhB, bh[t], acc[QS_TILES + t], 0, 0, 0);
+        }
+#pragma unroll
+        for (int t = 0; t < QS_TILES; ++t) {
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(AhA, bl[t], acc[t], 0, 0, 0);
+            acc[QS_TILES + t] =
+                __builtin_amdgcn_mfma_f32_32x32x16_f16(AhB, bl[t], acc[QS_TILES + t], 0, 0, 0);
+        }
+#pragma unroll
+        for (int t = 0; t < QS_TILES; ++t) {
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(AlA, bh[t], acc[t], 0, 0, 0);
+            acc[QS_TILES + t] =
+                __builtin_amdgcn_mfma_f32_32x32x16_f16(AlB, bh[t], acc[QS_TILES + t], 0, 0, 0);
+        }
+        {
+            half8 *dst = &lds.b[buf ^ 1][0][0][0];
+#pragma unroll
+            for (int u = 0; u < NB; ++u) {
+                const int v = threadIdx.x + u * NT;
+                if (NV_EVEN || v < NV) dst[v] = bq[u];
+            }
+        }
+        if constexpr (B2) load_b(min(s + 3, s_last), bq);
+        __syncthreads();
+    };
 
     for (int f0 = J.s0; f0 < J.s1; f0 += MF_FOLD) {
         const int f1 = min(J.s1, f0 + MF_FOLD);
-        for (int s = f0; s < f1; ++s) {
-            const int buf = (s - J.s0) & 1;
-            const bool more = s + 1 < J.s1;
-            half8 bn[NB];
-            if (more) {
-                const half8 *src = bsrc(s + 1);
-#pragma unroll
-                for (int u = 0; u < NB; ++u) {
-                    const int v = threadIdx.x + u * NT;
-                    if (v < NV) bn[u] = src[v];
-                }
-            }
-            float4 ae[4], ze[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                ae[q] = an[q];
-                if constexpr (D3) ze[q] = zn[q];
-            }
-            if (more) load_a(s + 1, an, zn);
-            const float *__restrict__ w8 = wl + (size_t)s * MF_K;
-            half8 rh, ih, rl, il;
-#pragma unroll
-            for (int j = 0; j < MF_K; ++j) {
-                const float4 e4 = ae[j >> 1];
-                float pr = (j & 1) ? e4.z : e4.x;
-                float pi = (j & 1) ? e4.w : e4.y;
-                if constexpr (D3) {
-                    const float4 z4 = ze[j >> 1];
-                    const float zr = (j & 1) ? z4.z : z4.x;
-                    const float zi = (j & 1) ? z4.w : z4.y;
-                    const float tr = pr * zr - pi * zi;
-                    pi = pr * zi + pi * zr;
-                    pr = tr;
-                }
-                const float w = w8[j];
-                float ar = w * pr, ai = w * pi;
-                // Opaque to the optimizer: otherwise (_Float16)(w * pr) may be
-                // re-derived as one fused multiply-convert (v_fma_mixlo_f16,
-                // single rounding) for the lo part while the packed hi part
-                // rounds the FP32 product (double rounding).  Where the two
-                // differ, hi + lo misses w * pr by one fp16 ulp.
-                asm("" : "+v"(ar), "+v"(ai));
-                const _Float16 ar16 = (_Float16)ar, ai16 = (_Float16)ai;
-                rh[j] = ar16;
-                ih[j] = ai16;
-                rl[j] = (_Float16)__builtin_fmaf(ar, one, -(float)ar16);
-                il[j] = (_Float16)__builtin_fmaf(ai, one, -(float)ai16);
-            }
-            const u32x4 rhu = __builtin_bit_cast(u32x4, rh), ihu = __builtin_bit_cast(u32x4, ih);
-            const u32x4 rlu = __builtin_bit_cast(u32x4, rl), ilu = __builtin_bit_cast(u32x4, il);
-            u32x4 hA, hB, lA, lB;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const auto ph = __builtin_amdgcn_permlane32_swap(rhu[i], ihu[i], false, false);
-                hA[i] = ph[0];
-                hB[i] = ph[1];
-                const auto pl = __builtin_amdgcn_permlane32_swap(rlu[i], ilu[i], false, false);
-                lA[i] = pl[0];
-                lB[i] = pl[1];
-            }
-            const half8 AhA = __builtin_bit_cast(half8, hA), AhB = __builtin_bit_cast(half8, hB);
-            const half8 AlA = __builtin_bit_cast(half8, lA), AlB = __builtin_bit_cast(half8, lB);
-            half8 bh[QS_TILES], bl[QS_TILES];
-#pragma unroll
-            for (int t = 0; t < QS_TILES; ++t) {
-                bh[t] = lds.b[buf][t][0][l];
-                bl[t] = lds.b[buf][t][1][l];
-            }
-#pragma unroll
-            for (int t = 0; t < QS_TILES; ++t) {
-                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(AhA, bh[t], acc[t], 0, 0, 0);
-                acc[QS_TILES + t] =
-                    __builtin_amdgcn_mfma_f32_32x32x16_f16(AhB, bh[t], acc[QS_TILES + t], 0, 0, 0);
-            }
-#pragma unroll
-            for (int t = 0; t < QS_TILES; ++t) {
-                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(AhA, bl[t], acc[t], 0, 0, 0);
-                acc[QS_TILES + t] =
-                    __builtin_amdgcn_mfma_f32_32x32x16_f16(AhB, bl[t], acc[QS_TILES + t], 0, 0, 0);
-            }
-#pragma unroll
-            for (int t = 0; t < QS_TILES; ++t) {
-                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(AlA, bh[t], acc[t], 0, 0, 0);
-                acc[QS_TILES + t] =
-                    __builtin_amdgcn_mfma_f32_32x32x16_f16(AlB, bh[t], acc[QS_TILES + t], 0, 0, 0);
-            }
-            if (more) {
-                half8 *dst = &lds.b[buf ^ 1][0][0][0];
-#pragma unroll
-                for (int u = 0; u < NB; ++u) {
-                    const int v = threadIdx.x + u * NT;
-                    if (v < NV) dst[v] = bn[u];
-                }
-            }
-            __syncthreads();
+        for (int s = f0; s < f1; s += 2) {
+            step(s, a0, z0, b0);
+            if constexpr (B2) step(s + 1, a1, z1, b1);
+            else step(s + 1, a1, z1, b0);
         }
 #pragma unroll
         for (int q = 0; q < NA; ++q) {
@@ -1458,7 +1540,8 @@ int toeplitz_q_sep(imp_ctx *ctx, int num_k, const float *kx, const float *ky, co
     const int M2 = 2 * Nx, M3 = d3 ? 2 * Nz : 1;
     const int nbk = nseg + 1;
     const int nblk = ceil_div(num_k, QS_BLK);
-    const long kp_max = (long)num_k + (long)nbk * MF_K;
+    // bucketed positions: at most num_k + nbk * 7, plus up to two zero steps
+    const long kp_max = (long)num_k + (long)(nbk + 2) * MF_K;
     // 1. bucket the samples by segment (stable counting sort)
     void *sw = nullptr;
     const size_t n_cnt = (size_t)nbk * nblk, n_bs = 64;
@@ -1479,8 +1562,10 @@ int toeplitz_q_sep(imp_ctx *ctx, int num_k, const float *kx, const float *ky, co
     std::vector<int> bs(nbk + 1);
     IMP_HIP(hipMemcpyAsync(bs.data(), bstart, (nbk + 1) * sizeof(int), hipMemcpyDeviceToHost, st));
     IMP_HIP(hipStreamSynchronize(st));
-    const int kp = bs[nbk];
-    const int n_steps = kp / MF_K;
+    // tables hold an even number of steps with at least one zero step after
+    // the last bucket: a job extended to even length (below) stays inside
+    const int n_steps = (bs[nbk] / MF_K + 2) & ~1;
+    const int kp = n_steps * MF_K;
     // 2. jobs: segment l runs over buckets l and l + 1, split into K blocks
     //    so the launch holds ~16 workgroups per CU
     const int R = Ny * M3;
@@ -1498,10 +1583,21 @@ int toeplitz_q_sep(imp_ctx *ctx, int num_k, const float *kx, const float *ky, co
     for (int l = 0; l < nseg; ++l) {
         jb[l] = (int)jobs.size();
         const int a = bs[l] / MF_K, b = bs[l + 2] / MF_K;
-        for (int s = a; s < b; s += (int)kb) jobs.push_back(QsJob{l, s, std::min(b, s + (int)kb), 0});
+        // every job an even number of steps (kb is): a last odd one takes
+        // one more step, whose samples enter with their own weights a_l
+        // (zero beyond the two buckets) or are zero padding
+        for (int s = a; s < b; s += (int)kb) {
+            int e = std::min(b, s + (int)kb);
+            if ((e - s) & 1) ++e;
+            jobs.push_back(QsJob{l, s, e, 0});
+        }
     }
     jb[nseg] = (int)jobs.size();
     const int njobs = (int)jobs.size();
+    for (const QsJob &j : jobs)
+        if (j.s0 < 0 || j.s1 > n_steps || j.s1 - j.s0 < 2 || ((j.s1 - j.s0) & 1))
+            return fail(IMP_EINVAL, "toeplitz_q: K block [%d, %d) outside %d steps", j.s0, j.s1,
+                        n_steps);
     if (njobs > 65535) return fail(IMP_EUNSUPPORTED, "toeplitz_q: %d K blocks", njobs);
     // 3. tables: wq [nseg][kp], ey [step][Ny][8], ez [step][M3][8], bx
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
@@ -1523,8 +1619,8 @@ int toeplitz_q_sep(imp_ctx *ctx, int num_k, const float *kx, const float *ky, co
         if (d3)
             hipLaunchKernelGGL(k_qt_line, dim3(ceil_div((long)kp * M3, 256)), dim3(256), 0, st,
                                kp, M3, Nz, (const int *)perm, kz, ez);
-        hipLaunchKernelGGL(k_qt_bx, dim3(ceil_div((long)n_steps * ntp * 16, 256)), dim3(256), 0,
-                           st, n_steps, ntp, M2, Nx, (const int *)perm, kx, bx);
+        hipLaunchKernelGGL(k_qt_bx, dim3(n_steps, ceil_div(ntp, 16)), dim3(256), 0, st, ntp, M2,
+                           Nx, (const int *)perm, kx, bx);
         IMP_CHECK_LAUNCH();
     }
     // 4. GEMM over K blocks, FP64 reduce with the mirror fill
