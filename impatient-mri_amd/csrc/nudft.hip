@@ -68,6 +68,25 @@ struct alignas(16) BPair {
 
 __device__ __forceinline__ float rev_fract(double u) { return (float)__builtin_amdgcn_fract(u); }
 
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+
+// fp16 hi/lo split of 8 FP32 values, packed as 4 dwords each: the hi parts
+// of a pair come from one v_cvt_pkrtz (any rounding of hi works) and each
+// lo part is v - hi of that same packed hi, read back through op_sel, so
+// hi + lo stays within a lo ulp of v.
+__device__ __forceinline__ void split8(const float (&v)[8], float one, u32x4 &hi, u32x4 &lo) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const half2v h =
+            __builtin_bit_cast(half2v, __builtin_amdgcn_cvt_pkrtz(v[2 * i], v[2 * i + 1]));
+        half2v l;
+        l[0] = (_Float16)__builtin_fmaf(v[2 * i], one, -(float)h[0]);
+        l[1] = (_Float16)__builtin_fmaf(v[2 * i + 1], one, -(float)h[1]);
+        hi[i] = __builtin_bit_cast(unsigned int, h);
+        lo[i] = __builtin_bit_cast(unsigned int, l);
+    }
+}
+
 // ---------------------------------------------------------------- VALU kernel
 // Weights stream: [group][n_stream][2*G] floats (Re, Im per rhs); positions
 // in the per-step layout.  EPI_COMBINE sums conj(S_c) * acc_c over the rhs.
@@ -218,7 +237,7 @@ __device__ __forceinline__ void mfma_body(const float4 *__restrict__ lanes, int 
                 nxt[t] = bpack[((size_t)(st + 1) * TILES + t) * 64 + l];
             }
             const double *B = spos + (size_t)st * pos_stride(D3);
-            half8 ch, cl, sh, sl;
+            float cv[MF_K], sv[MF_K];
 #pragma unroll
             for (int j = 0; j < MF_K; ++j) {
                 double u = B[2 * MF_K + j] * pw;
@@ -226,16 +245,12 @@ __device__ __forceinline__ void mfma_body(const float4 *__restrict__ lanes, int 
                 u = __builtin_fma(B[MF_K + j], py, u);
                 u = __builtin_fma(B[j], px, u);
                 const float r = rev_fract(u);
-                const float c = __builtin_amdgcn_cosf(r);
-                const float sn = __builtin_amdgcn_sinf(r);
-                const _Float16 c16 = (_Float16)c, s16 = (_Float16)sn;
-                ch[j] = c16;
-                sh[j] = s16;
-                cl[j] = (_Float16)__builtin_fmaf(c, one, -(float)c16);
-                sl[j] = (_Float16)__builtin_fmaf(sn, one, -(float)s16);
+                cv[j] = __builtin_amdgcn_cosf(r);
+                sv[j] = __builtin_amdgcn_sinf(r);
             }
-            const u32x4 chu = __builtin_bit_cast(u32x4, ch), shu = __builtin_bit_cast(u32x4, sh);
-            const u32x4 clu = __builtin_bit_cast(u32x4, cl), slu = __builtin_bit_cast(u32x4, sl);
+            u32x4 chu, clu, shu, slu;
+            split8(cv, one, chu, clu);
+            split8(sv, one, shu, slu);
             u32x4 hA, hB, lA, lB;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
@@ -1235,8 +1250,8 @@ template <int WM, bool D3>
 __global__ __launch_bounds__(64 * WM) __attribute__((amdgpu_waves_per_eu(1, 1)))
 void k_q_sep(const float4 *__restrict__ ey, const float4 *__restrict__ ez,
              const float *__restrict__ wq, const half8 *__restrict__ bx,
-             const QsJob *__restrict__ jobs, int R, int Ny, int M3, int ntp, int kp, int C2,
-             float *__restrict__ part, float one) {
+             const QsJob *__restrict__ jobs, int njobs, int gx, int gy, int R, int Ny, int M3,
+             int ntp, int kp, int C2, float *__restrict__ part, float one) {
     constexpr int NA = 2 * QS_TILES;
     constexpr int NT = 64 * WM;
     constexpr int NV = QS_TILES * 2 * 64;  // half8 of B per step and workgroup
@@ -1246,12 +1261,21 @@ void k_q_sep(const float4 *__restrict__ ey, const float4 *__restrict__ ez,
     const int l = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int h = l >> 5;
-    const QsJob J = jobs[blockIdx.z];
-    const int row0 = (blockIdx.y * WM + wave) * 64;
+    // XCD-aware order: workgroups go to the 8 XCDs round-robin, so slot
+    // b / 8 of XCD b % 8 takes job 8 * (slot / (gx gy)) + b % 8: every
+    // workgroup of a job (all its row and column blocks, which share the A
+    // rows and the B tiles) runs on one XCD's L2, column blocks first.
+    const int per = gx * gy;
+    const int slot = (int)(blockIdx.x >> 3);
+    const int job = 8 * (slot / per) + (int)(blockIdx.x & 7);
+    if (job >= njobs) return;
+    const int bxi = slot % per % gx, byi = slot % per / gx;
+    const QsJob J = jobs[job];
+    const int row0 = (byi * WM + wave) * 64;
     const int row = row0 + l;
     const bool rv = row < R;
     const int iy = rv ? row / M3 : 0, iz = rv ? row % M3 : 0;
-    const int tile0 = blockIdx.x * QS_TILES;
+    const int tile0 = bxi * QS_TILES;
     const float *__restrict__ wl = wq + (size_t)J.l * kp;
     auto bsrc = [&](int s) { return bx + ((size_t)s * ntp + tile0) * 2 * 64; };
 
@@ -1321,7 +1345,7 @@ void k_q_sep(const float4 *__restrict__ ey, const float4 *__restrict__ ez,
         const int buf = (s - J.s0) & 1;
         if (!B2) load_b(min(s + 1, s_last), bq);
         const float *__restrict__ w8 = wl + (size_t)s * MF_K;
-        half8 rh, ih, rl, il;
+        float arv[MF_K], aiv[MF_K];
 #pragma unroll
         for (int j = 0; j < MF_K; ++j) {
             const float4 e4 = ae[j >> 1];
@@ -1337,22 +1361,20 @@ void k_q_sep(const float4 *__restrict__ ey, const float4 *__restrict__ ez,
             }
             const float w = w8[j];
             float ar = w * pr, ai = w * pi;
-            // Opaque to the optimizer: otherwise (_Float16)(w * pr) may be
-            // re-derived as one fused multiply-convert (v_fma_mixlo_f16,
-            // single rounding) for the lo part while the packed hi part
-            // rounds the FP32 product (double rounding).  Where the two
-            // differ, hi + lo misses w * pr by one fp16 ulp.
+            // Opaque to the optimizer, so that the split below sees one
+            // FP32 value: a conversion of w * pr re-derived as a fused
+            // multiply-convert (v_fma_mixlo_f16, single rounding) next to one
+            // of the rounded product gives two different hi parts, and hi + lo
+            // then misses w * pr by one fp16 ulp.
             asm("" : "+v"(ar), "+v"(ai));
-            const _Float16 ar16 = (_Float16)ar, ai16 = (_Float16)ai;
-            rh[j] = ar16;
-            ih[j] = ai16;
-            rl[j] = (_Float16)__builtin_fmaf(ar, one, -(float)ar16);
-            il[j] = (_Float16)__builtin_fmaf(ai, one, -(float)ai16);
+            arv[j] = ar;
+            aiv[j] = ai;
         }
         // ae/ze consumed: refill them for step s + 2
         load_a(min(s + 2, s_last), ae, ze);
-        const u32x4 rhu = __builtin_bit_cast(u32x4, rh), ihu = __builtin_bit_cast(u32x4, ih);
-        const u32x4 rlu = __builtin_bit_cast(u32x4, rl), ilu = __builtin_bit_cast(u32x4, il);
+        u32x4 rhu, rlu, ihu, ilu;
+        split8(arv, one, rhu, rlu);
+        split8(aiv, one, ihu, ilu);
         u32x4 hA, hB, lA, lB;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -1424,7 +1446,7 @@ void k_q_sep(const float4 *__restrict__ ey, const float4 *__restrict__ ez,
     // row 8*(i/4) + 4*(l/32) + i%4
     const int n = l & 31;
     const size_t plane = (size_t)R * C2;
-    float *__restrict__ out = part + (size_t)blockIdx.z * 2 * plane + (size_t)(n >> 4) * plane;
+    float *__restrict__ out = part + (size_t)job * 2 * plane + (size_t)(n >> 4) * plane;
 #pragma unroll
     for (int m = 0; m < 2; ++m) {
 #pragma unroll
@@ -1519,7 +1541,7 @@ __global__ void k_q_edge_fill(int Nx, int Ny, int Nz, int ne_row, int n_edge, in
 }
 
 using QsFn = void (*)(const float4 *, const float4 *, const float *, const half8 *,
-                      const QsJob *, int, int, int, int, int, int, float *, float);
+                      const QsJob *, int, int, int, int, int, int, int, int, int, float *, float);
 
 QsFn pick_qsep(int wm, bool d3) {
     switch (wm) {
@@ -1598,7 +1620,8 @@ int toeplitz_q_sep(imp_ctx *ctx, int num_k, const float *kx, const float *ky, co
         if (j.s0 < 0 || j.s1 > n_steps || j.s1 - j.s0 < 2 || ((j.s1 - j.s0) & 1))
             return fail(IMP_EINVAL, "toeplitz_q: K block [%d, %d) outside %d steps", j.s0, j.s1,
                         n_steps);
-    if (njobs > 65535) return fail(IMP_EUNSUPPORTED, "toeplitz_q: %d K blocks", njobs);
+    if ((long)njobs * gx * gy > 0x7fffffffL)
+        return fail(IMP_EUNSUPPORTED, "toeplitz_q: %d K blocks", njobs);
     // 3. tables: wq [nseg][kp], ey [step][Ny][8], ez [step][M3][8], bx
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
     const size_t b_wq = al((size_t)nseg * kp * sizeof(float));
@@ -1638,10 +1661,11 @@ int toeplitz_q_sep(imp_ctx *ctx, int num_k, const float *kx, const float *ky, co
     IMP_HIP(hipMemcpyAsync(djb, jb.data(), jb.size() * sizeof(int), hipMemcpyHostToDevice, st));
     if (njobs > 0) {
         const int rec = prof_begin(ctx, "k_q_sep");
-        hipLaunchKernelGGL(pick_qsep(wm, d3), dim3(gx, gy, njobs), dim3(64 * wm), 0, st,
+        const long nwg = (long)ceil_div(njobs, 8) * 8 * gx * gy;
+        hipLaunchKernelGGL(pick_qsep(wm, d3), dim3((unsigned)nwg), dim3(64 * wm), 0, st,
                            (const float4 *)ey, (const float4 *)ez, (const float *)wq,
-                           (const half8 *)bx, (const QsJob *)djobs, R, Ny, M3, ntp, kp, M2, part,
-                           1.0f);
+                           (const half8 *)bx, (const QsJob *)djobs, njobs, gx, gy, R, Ny, M3,
+                           ntp, kp, M2, part, 1.0f);
         prof_end(ctx, rec);
         IMP_CHECK_LAUNCH();
     }
