@@ -1295,18 +1295,21 @@ void k_q_sep(const float4 *__restrict__ ey, const float4 *__restrict__ ez,
         half8 *dst = &lds.b[0][0][0][0];
         for (int v = threadIdx.x; v < NV; v += NT) dst[v] = src[v];
     }
-    // Global loads run two steps ahead of their use.  Two register sets
-    // alternate by step parity (the loop is unrolled by two), so no register
-    // holding a load in flight is copied (a copy would wait for the load).
-    // Set e serves step s: its A data, and with B2 the B operands of step
-    // s + 1, staged into LDS at the end of step s; then it is refilled for
-    // step s + 2 (A) and s + 3 (B).  Without B2 (a thread's share of B above
-    // 4 vectors, or uneven) B is fetched one step ahead into a single set.
-    constexpr bool B2 = NB <= 4 && NV % NT == 0;
-    // Loads are unconditional (a branch around them makes the wait-count
-    // insertion wait for every load in flight).  Lanes of rows past R read
-    // row 0's data: their output rows are never written and rows of the
-    // GEMM are independent.
+    // Software pipeline.  Step s issues its MFMAs with A operands built
+    // during step s - 1 and, independent of them, builds the A operands of
+    // step s + 1 (VALU), so the two interleave.  A data are fetched two steps
+    // before they are built on; B operands two steps before their LDS stage
+    // (B2) or one (a thread's share of B above 4 vectors, 2 in 3D, or uneven).
+    // Register sets alternate by step parity (the loop is unrolled by two),
+    // so no register holding a load in flight is copied (a copy waits for
+    // the load).  Loads are unconditional, prefetch indices clamped to the
+    // last step (a harmless reload at the tail), the B staging unconditional
+    // (the buffer written at the last step is never read) and every job has
+    // an even number of steps: the step body has no branches, and the
+    // wait-count insertion waits only for the loads each use needs.  Lanes
+    // of rows past R read row 0's data: their output rows are never written
+    // and rows of the GEMM are independent.
+    constexpr bool B2 = NB <= (D3 ? 2 : 4) && NV % NT == 0;
     constexpr bool NV_EVEN = NV % NT == 0;
     auto load_b = [&](int s, half8 (&r)[NB]) {
         const half8 *src = bsrc(s);
@@ -1326,24 +1329,12 @@ void k_q_sep(const float4 *__restrict__ ey, const float4 *__restrict__ ez,
             for (int q = 0; q < 4; ++q) z[q] = pz[q];
         }
     };
-    // Prefetch indices are clamped to the last step (a harmless reload at
-    // the tail) and the B staging is unconditional (the buffer written at
-    // the last step is never read): the step body has no branches.  Jobs
-    // have an even number of steps, so the pair loop needs no test either.
-    const int s_last = J.s1 - 1;
-    float4 a0[4], z0[4], a1[4], z1[4];
-    half8 b0[NB], b1[NB];
-    load_a(J.s0, a0, z0);
-    load_a(min(J.s0 + 1, s_last), a1, z1);
-    if constexpr (B2) {
-        load_b(min(J.s0 + 1, s_last), b0);
-        load_b(min(J.s0 + 2, s_last), b1);
-    }
-    __syncthreads();
-
-    auto step = [&](int s, float4 (&ae)[4], float4 (&ze)[4], half8 (&bq)[NB]) {
-        const int buf = (s - J.s0) & 1;
-        if (!B2) load_b(min(s + 1, s_last), bq);
+    // A operands of one step: [Re | Im] K halves of M tiles A (rows 0-31)
+    // and B (rows 32-63) of this wave, fp16 hi and lo
+    struct AOps {
+        half8 hA, hB, lA, lB;
+    };
+    auto build = [&](int s, const float4 (&ae)[4], const float4 (&ze)[4], AOps &o) {
         const float *__restrict__ w8 = wl + (size_t)s * MF_K;
         float arv[MF_K], aiv[MF_K];
 #pragma unroll
@@ -1370,8 +1361,6 @@ void k_q_sep(const float4 *__restrict__ ey, const float4 *__restrict__ ez,
             arv[j] = ar;
             aiv[j] = ai;
         }
-        // ae/ze consumed: refill them for step s + 2
-        load_a(min(s + 2, s_last), ae, ze);
         u32x4 rhu, rlu, ihu, ilu;
         split8(arv, one, rhu, rlu);
         split8(aiv, one, ihu, ilu);
@@ -1385,31 +1374,58 @@ void k_q_sep(const float4 *__restrict__ ey, const float4 *__restrict__ ez,
             lA[i] = pl[0];
             lB[i] = pl[1];
         }
-        const half8 AhA = __builtin_bit_cast(half8, hA), AhB = __builtin_bit_cast(half8, hB);
-        const half8 AlA = __builtin_bit_cast(half8, lA), AlB = __builtin_bit_cast(half8, lB);
+        o.hA = __builtin_bit_cast(half8, hA);
+        o.hB = __builtin_bit_cast(half8, hB);
+        o.lA = __builtin_bit_cast(half8, lA);
+        o.lB = __builtin_bit_cast(half8, lB);
+    };
+
+    const int s_last = J.s1 - 1;
+    float4 d0[4], z0[4], d1[4], z1[4];
+    half8 b0[NB], b1[NB];
+    AOps op0, op1;
+    load_a(J.s0, d0, z0);
+    load_a(min(J.s0 + 1, s_last), d1, z1);
+    if constexpr (B2) {
+        load_b(min(J.s0 + 1, s_last), b0);
+        load_b(min(J.s0 + 2, s_last), b1);
+    }
+    build(J.s0, d0, z0, op0);
+    load_a(min(J.s0 + 2, s_last), d0, z0);
+    __syncthreads();
+
+    // step s: MFMAs with `cur`; `nxt` built from dn (data of step s + 1),
+    // dn refilled with step s + 3; bq holds B of step s + 1 (B2) and is
+    // refilled with step s + 3
+    auto step = [&](int s, const AOps &cur, AOps &nxt, float4 (&dn)[4], float4 (&zn)[4],
+                    half8 (&bq)[NB]) {
+        const int buf = (s - J.s0) & 1;
+        if (!B2) load_b(min(s + 1, s_last), bq);
         half8 bh[QS_TILES], bl[QS_TILES];
 #pragma unroll
         for (int t = 0; t < QS_TILES; ++t) {
             bh[t] = lds.b[buf][t][0][l];
             bl[t] = lds.b[buf][t][1][l];
         }
+        build(min(s + 1, s_last), dn, zn, nxt);
+        load_a(min(s + 3, s_last), dn, zn);
 #pragma unroll
         for (int t = 0; t < QS_TILES; ++t) {
-            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(AhA, bh[t], acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(cur.hA, bh[t], acc[t], 0, 0, 0);
             acc[QS_TILES + t] =
-                __builtin_amdgcn_mfma_f32_32x32x16_f16(AhB, bh[t], acc[QS_TILES + t], 0, 0, 0);
+                __builtin_amdgcn_mfma_f32_32x32x16_f16(cur.hB, bh[t], acc[QS_TILES + t], 0, 0, 0);
         }
 #pragma unroll
         for (int t = 0; t < QS_TILES; ++t) {
-            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(AhA, bl[t], acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(cur.hA, bl[t], acc[t], 0, 0, 0);
             acc[QS_TILES + t] =
-                __builtin_amdgcn_mfma_f32_32x32x16_f16(AhB, bl[t], acc[QS_TILES + t], 0, 0, 0);
+                __builtin_amdgcn_mfma_f32_32x32x16_f16(cur.hB, bl[t], acc[QS_TILES + t], 0, 0, 0);
         }
 #pragma unroll
         for (int t = 0; t < QS_TILES; ++t) {
-            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(AlA, bh[t], acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(cur.lA, bh[t], acc[t], 0, 0, 0);
             acc[QS_TILES + t] =
-                __builtin_amdgcn_mfma_f32_32x32x16_f16(AlB, bh[t], acc[QS_TILES + t], 0, 0, 0);
+                __builtin_amdgcn_mfma_f32_32x32x16_f16(cur.lB, bh[t], acc[QS_TILES + t], 0, 0, 0);
         }
         {
             half8 *dst = &lds.b[buf ^ 1][0][0][0];
@@ -1426,9 +1442,9 @@ void k_q_sep(const float4 *__restrict__ ey, const float4 *__restrict__ ez,
     for (int f0 = J.s0; f0 < J.s1; f0 += MF_FOLD) {
         const int f1 = min(J.s1, f0 + MF_FOLD);
         for (int s = f0; s < f1; s += 2) {
-            step(s, a0, z0, b0);
-            if constexpr (B2) step(s + 1, a1, z1, b1);
-            else step(s + 1, a1, z1, b0);
+            step(s, op0, op1, d1, z1, b0);
+            if constexpr (B2) step(s + 1, op1, op0, d0, z0, b1);
+            else step(s + 1, op1, op0, d0, z0, b0);
         }
 #pragma unroll
         for (int q = 0; q < NA; ++q) {
