@@ -101,8 +101,12 @@ struct imp_ctx {
 
 namespace imp {
 
-// Returns a device scratch region of at least `bytes` (grows, never shrinks).
+// Returns a device scratch region of at least `bytes` (grows; shrinks only
+// through workspace_release).
 int workspace(imp_ctx *ctx, int slot, size_t bytes, void **out);
+// Frees a slot's region if it holds more than `keep` bytes (synchronises the
+// context stream first).
+int workspace_release(imp_ctx *ctx, int slot, size_t keep);
 int ensure_dscratch(imp_ctx *ctx, size_t n);
 
 // Frees a network's device memory and detaches it from its context (for

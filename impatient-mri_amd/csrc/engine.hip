@@ -51,6 +51,17 @@ int workspace(imp_ctx *ctx, int slot, size_t bytes, void **out) {
     return IMP_OK;
 }
 
+int workspace_release(imp_ctx *ctx, int slot, size_t keep) {
+    Workspace &w = ctx->ws[slot];
+    if (w.ptr && w.bytes > keep) {
+        IMP_HIP(hipStreamSynchronize(ctx->stream));
+        IMP_HIP(hipFree(w.ptr));
+        w.ptr = nullptr;
+        w.bytes = 0;
+    }
+    return IMP_OK;
+}
+
 int ensure_dscratch(imp_ctx *ctx, size_t n) {
     if (ctx->dscratch_n >= n) return IMP_OK;
     if (ctx->dscratch) {

@@ -1021,6 +1021,10 @@ constexpr int QS_TILES = 4;     // N tiles (16 complex columns each) per wave
 constexpr int QS_BLK = 1024;    // samples per counting-sort block
 constexpr int QS_MAXBK = 33;    // buckets: nseg + 1 <= 33
 constexpr int QS_BSHIFT = 10;   // ex is stored scaled by 2^10 (keeps lo parts normal)
+constexpr size_t QS_PART_CAP = (size_t)4 << 30;   // K-block partials, bytes
+constexpr size_t QS_TAB_CAP = (size_t)64 << 30;   // phasor tables; above: direct kernel
+constexpr size_t QS_KEEP = (size_t)1 << 30;       // workspaces above this are freed after use
+constexpr int QS_FALLBACK = 1;                    // toeplitz_q_sep: use the direct kernel
 
 struct QsJob {
     int l;       // segment
@@ -1616,6 +1620,14 @@ int toeplitz_q_sep(imp_ctx *ctx, int num_k, const float *kx, const float *ky, co
     const long target = 16L * ctx->num_cus;
     long kb = tot_steps * gx * gy / std::max(1L, target);
     kb = std::min(2048L, std::max((long)MF_FOLD * 2, (kb + MF_FOLD - 1) / MF_FOLD * MF_FOLD));
+    // K-block partials stay within QS_PART_CAP: on large grids fewer, longer
+    // blocks (the two-sum folds keep their accuracy over any length)
+    const size_t part_per_job = (size_t)2 * R * M2 * sizeof(float);
+    const long max_jobs = std::max<long>(2L * nseg, (long)(QS_PART_CAP / part_per_job));
+    if (tot_steps / kb + nseg > max_jobs) {
+        kb = (tot_steps + (max_jobs - nseg) - 1) / (max_jobs - nseg);
+        kb = (kb + MF_FOLD - 1) / MF_FOLD * MF_FOLD;
+    }
     std::vector<QsJob> jobs;
     std::vector<int> jb(nseg + 1, 0);
     for (int l = 0; l < nseg; ++l) {
@@ -1644,6 +1656,7 @@ int toeplitz_q_sep(imp_ctx *ctx, int num_k, const float *kx, const float *ky, co
     const size_t b_ey = al((size_t)kp * Ny * sizeof(float2));
     const size_t b_ez = d3 ? al((size_t)kp * M3 * sizeof(float2)) : 0;
     const size_t b_bx = al((size_t)n_steps * ntp * 2 * 64 * sizeof(half8));
+    if (b_wq + b_ey + b_ez + b_bx > QS_TAB_CAP) return QS_FALLBACK;
     void *tw = nullptr;
     IMP_TRY(workspace(ctx, WS_QS_TAB, b_wq + b_ey + b_ez + b_bx + 256, &tw));
     float *wq = (float *)tw;
@@ -1706,8 +1719,10 @@ int toeplitz_q_sep(imp_ctx *ctx, int num_k, const float *kx, const float *ky, co
                        d3 ? Nz : 1, ne_row, n_edge, nseg, (const float *)h_r, (const float *)h_i,
                        q_r, q_i);
     IMP_CHECK_LAUNCH();
-    // the host job table must outlive its copy
+    // the host job table must outlive its copy; large tables are not kept
     IMP_HIP(hipStreamSynchronize(st));
+    IMP_TRY(workspace_release(ctx, WS_QS_TAB, QS_KEEP));
+    IMP_TRY(workspace_release(ctx, WS_QS_PART, QS_KEEP));
     return IMP_OK;
 }
 
@@ -1755,10 +1770,12 @@ int toeplitz_q(imp_ctx *ctx, int num_k, const float *kx, const float *ky,
     // (or tau is degenerate); the direct kernel evaluates the half grid and
     // fills the rest by Hermitian symmetry.
     if (std_grid && !(flags & IMP_Q_DIRECT) && tau > 0.f && std::isfinite(tau) &&
-        std::isfinite(t0))
-        return toeplitz_q_sep(ctx, num_k, kx, ky, d3 ? kz : nullptr, t, spos, (const float2 *)W,
-                              d3, Nx, Ny, Nz, nseg, tau, t0, mode == Out::PerRhsCumulative, q_r,
-                              q_i);
+        std::isfinite(t0)) {
+        const int rc = toeplitz_q_sep(ctx, num_k, kx, ky, d3 ? kz : nullptr, t, spos,
+                                      (const float2 *)W, d3, Nx, Ny, Nz, nseg, tau, t0,
+                                      mode == Out::PerRhsCumulative, q_r, q_i);
+        if (rc != QS_FALLBACK) return rc;
+    }
     const int M2 = 2 * Nx, M3 = Nz > 1 ? 2 * Nz : 1;
     const int n1 = std_grid ? (Ny + 1) * M2 * M3 : 0;
     const int ne_row = M3 > 1 ? M2 + M3 - 1 : 1;

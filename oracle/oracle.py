@@ -172,6 +172,45 @@ def toeplitz_apply(p, q, tau, t0, fm=None, sens=None, lam=0.0, w=None):
     return out.reshape(shp)
 
 
+def toeplitz_operator(q, tau, t0, shape, fm=None, sens=None, lam=0.0, w=None, workers=-1):
+    """toeplitz_apply as a callable v -> A v for iterated FP64 solves at
+    configuration sizes: Qf_l computed once, all coils in one batched
+    transform through scipy.fft (the same pocketfft transforms as numpy.fft,
+    threaded), and the low-corner embedding.  The centre padding with
+    ifftshift places v at offset -N/2 (mod 2N) and the crop after fftshift
+    reads at the same offset; a circular convolution commutes with shifts,
+    so placing v at 0 and reading at 0 is the same map.
+    v and the result are flat [N] complex128."""
+    import scipy.fft as sfft
+    N = tuple(shape) if len(shape) == 3 and shape[2] > 1 else tuple(shape[:2])
+    M = tuple(2 * n for n in N)
+    axes = tuple(range(1, len(N) + 1))
+    q = np.asarray(q, np.complex128).reshape(q.shape[0], *M)
+    Qf = [sfft.fftn(_shift(q[l], True), workers=workers) for l in range(q.shape[0])]
+    f = np.zeros(N) if fm is None else np.asarray(fm, np.float64).reshape(N)
+    S = (np.ones((1,) + N, np.complex128) if sens is None
+         else np.asarray(sens, np.complex128).reshape((-1,) + N))
+    ph = [np.exp(1j * f * (tau * l + t0)) for l in range(len(Qf))]
+    corner = (slice(None),) + tuple(slice(0, n) for n in N)
+    big = np.zeros((S.shape[0],) + M, dtype=np.complex128)
+
+    def apply(v):
+        pv = np.asarray(v, np.complex128).reshape(N)
+        sp = S * pv
+        acc = np.zeros_like(sp)
+        for l in range(len(Qf)):
+            big[corner] = np.conj(ph[l]) * sp   # the rest of big stays zero
+            X = sfft.fftn(big, axes=axes, workers=workers)
+            X *= Qf[l]
+            r = sfft.ifftn(X, axes=axes, workers=workers, overwrite_x=True)
+            acc += ph[l] * r[corner]
+        out = (np.conj(S) * acc).sum(axis=0)
+        if lam:
+            out += fd_reg(pv, lam, w)
+        return out.reshape(-1)
+    return apply
+
+
 def fd_reg(p, lam, w=None):
     """s = lam * sum_d D_d^T W D_d p, periodic: (D_d p)(i) = p(i) - p(i+e_d),
     (D_d^T t)(i) = t(i) - t(i-e_d).  DHWD2dGpu / DHWD3dGpu
@@ -281,15 +320,17 @@ def periodic_laplacian_eigs(dims):
     return out
 
 
-def chan_precond_weights(q, dims, lam):
+def chan_precond_weights(q, dims, lam, floor=1e-3):
     """Weights of M^-1 = F^H diag(w) F (unnormalised DFT F, so w includes the
     1/prod(dims) of the inverse): eigenvalues of chan_circulant(sum_l Q_l)
-    plus lam times the Laplacian spectrum, floored at 1e-3 of the DC one."""
+    plus lam times the Laplacian spectrum, floored at `floor` times the DC
+    one (1e-3 on the GPU; the Chan eigenvalues are Rayleigh quotients of the
+    Toeplitz matrix at Fourier vectors, so a PSD kernel needs no floor)."""
     t = np.asarray(q).reshape(np.asarray(q).shape[0], -1).sum(axis=0)
     c = chan_circulant(t, dims)
     eig = np.fft.fftn(c).real + lam * periodic_laplacian_eigs(dims)
     dc = np.fft.fftn(c).real.flat[0]
-    eig = np.maximum(eig, 1e-3 * abs(dc))
+    eig = np.maximum(eig, floor * abs(dc))
     return 1.0 / (eig * np.prod(dims))
 
 
@@ -299,10 +340,11 @@ def circulant_apply(w, r):
     return np.fft.ifftn(w * np.fft.fftn(r)) * w.size
 
 
-def pcg(apply, precond, b, max_iter, tol=1e-16, best_iterate=True):
+def pcg(apply, precond, b, max_iter, tol=1e-16, best_iterate=True, on_iter=None):
     """Textbook preconditioned CG (the IMP_CG_PRECOND path of imp_toep_cg):
     rho = Re(r^H z), alpha = rho / Re(p^H A p), p = z + (rho'/rho) p; the stop
-    test and best-iterate tracking use the unpreconditioned |r|."""
+    test and best-iterate tracking use the unpreconditioned |r|.
+    on_iter(it, x, rel_residual) is called after every iteration."""
     b = np.asarray(b, np.complex128).reshape(-1)
     x = np.zeros_like(b)
     r = b.copy()
@@ -325,6 +367,8 @@ def pcg(apply, precond, b, max_iter, tol=1e-16, best_iterate=True):
         rho = rho_new
         hist.append(np.sqrt(rr / normb))
         it += 1
+        if on_iter is not None:
+            on_iter(it, x, hist[-1])
         if rr < best:
             best, xbest = rr, x.copy()
         if np.sqrt(rr / normb) < tol:

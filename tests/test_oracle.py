@@ -470,3 +470,29 @@ def test_bf16_rounding_is_nearest_even():
     assert cnn.bf16(np.float32(1.0 + 3 * ulp / 2)) == 1.0 + 2 * ulp
     assert cnn.bf16(np.float32(1.0 + ulp / 2 + 2 ** -20)) == 1.0 + ulp
     assert cnn.bf16(one) == 1.0 and cnn.bf16(np.float32(-3.0)) == -3.0
+
+
+@pytest.mark.parametrize("shape", [(8, 8, 1), (4, 6, 3)])
+def test_toeplitz_operator_equals_apply(shape):
+    """The cached operator (Qf once, scipy.fft) is the same map as
+    toeplitz_apply, in 2D and 3D, with coils, field map and regularizer."""
+    orc = _orc()
+    rng = np.random.default_rng(21)
+    N1, N2, N3 = shape
+    d3 = N3 > 1
+    M = (2 * N1, 2 * N2) + ((2 * N3,) if d3 else ())
+    L = 2
+    q = rng.standard_normal((L + 1, int(np.prod(M)))) + 1j * rng.standard_normal((L + 1, int(np.prod(M))))
+    img = shape if d3 else shape[:2]
+    fm = rng.uniform(-300, 300, img)
+    sens = rng.standard_normal((2,) + img) + 1j * rng.standard_normal((2,) + img)
+    v = rng.standard_normal(img) + 1j * rng.standard_normal(img)
+    ref = orc.toeplitz_apply(v, q, 1e-3, 2e-4, fm, sens, 0.3)
+    A = orc.toeplitz_operator(q, 1e-3, 2e-4, shape, fm, sens, 0.3)
+    got = A(v.reshape(-1)).reshape(img)
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-12
+    # a second call on another vector: the operator keeps no state between calls
+    v2 = rng.standard_normal(img) + 1j * rng.standard_normal(img)
+    ref2 = orc.toeplitz_apply(v2, q, 1e-3, 2e-4, fm, sens, 0.3)
+    got2 = A(v2.reshape(-1)).reshape(img)
+    assert np.linalg.norm(got2 - ref2) / np.linalg.norm(ref2) < 1e-12
