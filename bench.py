@@ -35,6 +35,8 @@ sys.path.insert(0, str(REPO / "impatient-mri_amd"))
 
 PEAK_FP16_MFMA_TFLOPS = 2500.0   # MI355X dense fp16 MFMA
 PEAK_HBM_GBS = 8000.0
+PEAK_FP32_VALU_TFLOPS = 157.3    # MI355X FP32 vector
+VALU_FLOPS_PER_PAIR = 8.0        # one complex MAC in FP32
 # fp16 MFMA flops per (coil, sample, voxel) pair: one complex MAC is 8 real
 # flops, done as 3 fp16 products (Eh Wh + Eh Wl + El Wh) for FP32 accuracy
 MFMA_FLOPS_PER_PAIR = 24.0
@@ -42,7 +44,7 @@ MFMA_FLOPS_PER_PAIR = 24.0
 # rocprofv3 --pmc FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0), the factors a
 # 256 MiB device copy calibrates (profiles/traffic_512x512x1_32coil_pmc.md);
 # re-measure when the kernel changes
-NUDFT_TRAFFIC_BYTES = 2.0 * 3323479.7 * 1024 + 1.0 * 1046711.3 * 1024
+NUDFT_TRAFFIC_BYTES = 2.0 * 4059573.6 * 1024 + 1.0 * 967454.9 * 1024
 
 
 def hip_events():
@@ -265,6 +267,13 @@ def main():
                 "kernel": "k_nudft_mfma<2,false> (F^H d, 32 coils in one pass)",
                 "kernel_ms": round(k_ms, 3),
                 "work": "24 fp16 MFMA flops per (coil, sample, voxel) pair"}
+    # the same launch priced as an FP32 VALU NUDFT: one complex MAC (8 FP32
+    # flops) per (coil, sample, voxel) pair, against the 157.3 TFLOP/s FP32
+    # vector peak (the phase and sincos, shared by the coils, not counted)
+    valu_eq = VALU_FLOPS_PER_PAIR * pairs / (k_ms * 1e-3) / 1e12
+    roofline["fp32_valu_equivalent"] = {
+        "flops_per_pair": VALU_FLOPS_PER_PAIR, "tflops": round(valu_eq, 1),
+        "peak": PEAK_FP32_VALU_TFLOPS, "frac": round(valu_eq / PEAK_FP32_VALU_TFLOPS, 3)}
 
     result = {
         "metric": "NUDFT GSample*vox/s (direct field-corrected F^H d); time to rel-residual 1e-5 "

@@ -190,6 +190,8 @@ def main(argv=None):
     ap.add_argument("--slices", type=int, default=48)
     ap.add_argument("--model", default=None, help="trained warm-start model (.npz)")
     ap.add_argument("--tol", type=float, default=1e-5)
+    ap.add_argument("--repeat", type=int, default=2,
+                    help="passes over the batch with one autotuner; the first includes its trials")
     args = ap.parse_args(argv)
 
     def log(m):
@@ -203,13 +205,18 @@ def main(argv=None):
     if args.model:
         from .learn import load_model
         params, _, _ = load_model(args.model)
-    res = reconstruct(ctx, sl, tol=args.tol, model=params, log=log)
+    tuner = Autotuner(ctx)
+    passes = [reconstruct(ctx, sl, tol=args.tol, model=params, tuner=tuner, log=log)
+              for _ in range(max(1, args.repeat))]
+    res = passes[-1]
     recs = res["slices"]
     by = {}
     for r in recs:
         by.setdefault(r["N"], []).append(r)
     summary = {
-        "slices": len(recs), "total_s": round(res["total_s"], 4),
+        "slices": len(recs), "passes": len(passes),
+        "first_pass_s": round(passes[0]["total_s"], 4),
+        "total_s": round(res["total_s"], 4),
         "slices_per_s": round(len(recs) / res["total_s"], 2),
         "fhd_batch_s": round(res["fhd_s"], 4), "q_s": round(res["q_s"], 4),
         "converged": int(sum(r["converged"] for r in recs)),
@@ -220,6 +227,8 @@ def main(argv=None):
                          "start_used": int(sum(r["start_used"] for r in v))}
                      for N, v in sorted(by.items())},
         "autotune": {"nudft_chunk_scale": res["nudft_chunk_scale"], "trials_ms": res["tuner"]},
+        "note": "total_s and per-size figures are the last pass (schedules already tuned); "
+                "first_pass_s includes the autotuner's trial launches",
     }
     print(json.dumps(summary), flush=True)
     ctx.close()

@@ -106,8 +106,10 @@ class Geometry:
         self.fm.upload(np.ascontiguousarray(fm, dtype=np.float32).reshape(-1))
         call("imp_toep_set_fieldmap", self.op, ptr(self.fm))
 
-    def synth_data(self, rho, snr_db, rng):
-        """d = F rho (current field map) + noise at snr_db; leaves d on the device."""
+    def synth_data(self, rho, snr_db, rng=None, noise=None):
+        """d = F rho (current field map) + noise at snr_db; leaves d on the device.
+        The unit-variance noise is drawn from rng (real part, then imaginary
+        part), or given as noise = (real, imaginary) drawn the same way."""
         rho = np.asarray(rho).reshape(-1)
         pr = self.ctx.to_device(np.ascontiguousarray(rho.real, np.float32))
         pi = self.ctx.to_device(np.ascontiguousarray(rho.imag, np.float32))
@@ -119,8 +121,11 @@ class Geometry:
             dr, di = self.dr.get(), self.di.get()
             rms = np.sqrt(np.mean(dr.astype(np.float64) ** 2 + di.astype(np.float64) ** 2))
             sd = rms * 10 ** (-snr_db / 20) / np.sqrt(2)
-            dr += (sd * rng.standard_normal(dr.size, dtype=np.float32)).astype(np.float32)
-            di += (sd * rng.standard_normal(di.size, dtype=np.float32)).astype(np.float32)
+            if noise is None:
+                noise = (rng.standard_normal(dr.size, dtype=np.float32),
+                         rng.standard_normal(di.size, dtype=np.float32))
+            dr += (sd * noise[0]).astype(np.float32)
+            di += (sd * noise[1]).astype(np.float32)
             self.dr.upload(dr)
             self.di.upload(di)
 
@@ -152,20 +157,38 @@ class Pair:
     ti: object
 
 
-def make_pairs(geo: Geometry, n: int, seed0: int, snr_db=40.0, tol=1e-6, log=print):
-    """n training pairs (z0 = M_chan^-1 b, x*) on the device."""
+def _draw_inputs(seed, X, Y, max_hz, n_noise):
+    """Host side of one draw: phantom, field map and unit noise, in the order
+    the seeded generator gives them."""
+    rng = np.random.default_rng(seed)
+    rho = random_phantom(rng, X, Y)
+    fm = synth.field_map(rng, X, Y, 4, max_hz)
+    noise = (rng.standard_normal(n_noise, dtype=np.float32),
+             rng.standard_normal(n_noise, dtype=np.float32))
+    return rho, fm, noise
+
+
+def make_pairs(geo: Geometry, n: int, seed0: int, snr_db=40.0, tol=1e-6, log=print, ahead=3):
+    """n training pairs (z0 = M_chan^-1 b, x*) on the device.  Worker threads
+    draw the next `ahead` draws' phantoms, field maps and noise while the GPU
+    solves the current one (the draws are the same as drawn inline)."""
+    from concurrent.futures import ThreadPoolExecutor
     prob = geo.prob
     X = prob.x.reshape(prob.Ny, prob.Nx).astype(np.float64)
     Y = prob.y.reshape(prob.Ny, prob.Nx).astype(np.float64)
     pairs = []
     t0 = time.perf_counter()
     flags = IMP_CG_PRECOND | IMP_CG_BEST_ITERATE
+    pool = ThreadPoolExecutor(max_workers=2)
+    submit = lambda k: pool.submit(_draw_inputs, seed0 + k, X, Y, prob.meta["max_hz"],
+                                   geo.C * geo.nk)
+    pending = [submit(k) for k in range(min(ahead, n))]
     for k in range(n):
-        rng = np.random.default_rng(seed0 + k)
-        rho = random_phantom(rng, X, Y)
-        fm = synth.field_map(rng, X, Y, 4, prob.meta["max_hz"])
+        rho, fm, noise = pending.pop(0).result()
+        if k + ahead < n:
+            pending.append(submit(k + ahead))
         geo.set_fieldmap(fm)
-        geo.synth_data(rho, snr_db, rng)
+        geo.synth_data(rho, snr_db, noise=noise)
         geo.fhd()
         p = Pair(*(geo.ctx.alloc((geo.nv,)) for _ in range(4)))
         st = geo.solve(p.tr, p.ti, tol, flags=flags)
@@ -174,6 +197,7 @@ def make_pairs(geo: Geometry, n: int, seed0: int, snr_db=40.0, tol=1e-6, log=pri
         if (k + 1) % max(1, n // 10) == 0 or k == n - 1:
             log(f"pairs {k + 1}/{n}: last PCG {st['iterations']} it, "
                 f"{time.perf_counter() - t0:.1f} s")
+    pool.shutdown()
     return pairs
 
 
@@ -318,6 +342,8 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=None, help="base seed of the training draws")
+    ap.add_argument("--pair-tol", type=float, default=1e-6,
+                    help="relative residual the training solutions x* are solved to")
     ap.add_argument("--out", required=True)
     args = ap.parse_args(argv)
 
@@ -329,7 +355,7 @@ def main(argv=None):
     ctx = Context(0)
     geo = Geometry(ctx, prob)
     log(f"{prob.name}: {args.draws} training draws from seed {seed0}")
-    pairs = make_pairs(geo, args.draws, seed0, log=log)
+    pairs = make_pairs(geo, args.draws, seed0, tol=args.pair_tol, log=log)
     net = ops.CNN(ctx, prob.Ny, prob.Nx, seed=seed0)
     hist = train(net, pairs, args.epochs, args.lr, log=log)
     gains = fit_radial_gains(geo, probe_pairs(geo, 24, seed0 + 500000, log=log))
@@ -341,6 +367,7 @@ def main(argv=None):
     log(f"validation mean iterations: " + ", ".join(f"{m} {np.mean(v):.1f}" for m, v in val.items())
         + f"; learned gains {'kept' if use_gains else 'not used (Chan weights are better)'}")
     meta = dict(config=prob.name, draws=args.draws, epochs=args.epochs, lr=args.lr,
+                pair_tol=args.pair_tol,
                 seed0=seed0, loss_history=hist, nbins=NBINS, use_gains=use_gains,
                 validation=val)
     save_model(args.out, net.get_params(), gains, meta)
