@@ -155,12 +155,19 @@ class Toeplitz:
     """Resident Toeplitz normal operator (imp_toep)."""
 
     def __init__(self, ctx: Context, shape, q, tau, t0, fm=None, sens=None, lam=0.0):
+        """q: the L+1 planes on the host ([L+1, grid] complex), or already on
+        the device as a (real, imaginary) pair of DeviceArrays."""
         shape = tuple(shape) + (1,) * (3 - len(shape))
         self.ctx, self.shape = ctx, shape
         self.nvox = int(np.prod(shape))
-        q = np.asarray(q)
-        nseg = q.shape[0]
-        self._qr, self._qi = _split(ctx, q.reshape(-1))
+        if isinstance(q, tuple):
+            self._qr, self._qi = q
+            grid = self.nvox * (8 if shape[2] > 1 else 4)
+            nseg = int(np.prod(self._qr.shape)) // grid
+        else:
+            q = np.asarray(q)
+            nseg = q.shape[0]
+            self._qr, self._qi = _split(ctx, q.reshape(-1))
         self._fm = _opt(ctx, None if fm is None else np.asarray(fm).reshape(-1))
         ncoils = 1
         self._sr = self._si = None
