@@ -71,6 +71,8 @@ enum WsSlot {
     WS_QS_SORT,    // separable Q: sample order, bucket counts, weight scale
     WS_QS_TAB,     // separable Q: phasor and weight tables
     WS_QS_PART,    // separable Q: K-block partial sums and job table
+    WS_STACK,      // stacked-trajectory detection flags
+    WS_QSTACK,     // stacked Q: one partition's 2D planes and the z factor
     WS_COUNT
 };
 

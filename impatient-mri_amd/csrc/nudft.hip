@@ -818,6 +818,172 @@ int build_spos(imp_ctx *ctx, int n, const float *x, const float *y, const float 
     return IMP_OK;
 }
 
+// ---- stacks of 2D trajectories (F^H d separable in z) -----------------------
+// A stack: M = P K samples where partition p (samples [pK, pK + K)) repeats
+// the kx, ky and t of partition 0 and holds one kz (stack-of-spirals and
+// stack-of-stars acquisitions).  Then
+//   F^H d(r) = sum_c conj(S_c(r)) sum_p e^{+i 2pi kz_p z} G_{c,p}(r),
+//   G_{c,p}(r) = sum_k conj(phi) d_{c,p,k} e^{+i (2pi (kx_k x + ky_k y) + w(r) t_k)},
+// a 2D NUDFT with C P right-hand sides, so the phase and sin/cos of each
+// (sample, voxel) pair serve C P columns instead of C.  W = conj(phi) d is
+// [C][P][K], i.e. already [(c P + p)][k].  The kz factor is applied in FP64
+// in the reduction.
+
+// smallest i with kz[i] != kz[0] (first initialised to n)
+__global__ void k_first_change(int n, const float *kz, int *first) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && kz[i] != kz[0]) atomicMin(first, i);
+}
+
+// bad = 1 unless every partition repeats partition 0's kx, ky, t (and phi,
+// when given) and holds one kz
+__global__ void k_stack_check(int n, int K, const float *kx, const float *ky, const float *kz,
+                              const float *t, const float *phi_r, const float *phi_i,
+                              int *bad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int j = i % K;
+    bool same = kx[i] == kx[j] && ky[i] == ky[j] && t[i] == t[j] && kz[i] == kz[i - j];
+    if (phi_r) same = same && phi_r[i] == phi_r[j] && phi_i[i] == phi_i[j];
+    if (!same) atomicOr(bad, 1);
+}
+
+// Partials of the group launches [group][chunk][64][n] (32 rhs per group,
+// rhs q = c P + p) -> out = sum_p e^{+i 2pi kz_p z} sum_c conj(S_c) G_{c,p}, in FP64.
+__global__ void k_reduce_stack(const float *partial, int nchunks, int ncoils, int P, int K,
+                               int n, const unsigned int *maxbits, const float *kz,
+                               const float *z, const float *s_r, const float *s_i, float *out_r,
+                               float *out_i) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    const double unscale = ldexp(1.0, -w_shift(*maxbits));
+    const double za = z[a];
+    double orr = 0.0, oi = 0.0;
+    for (int p = 0; p < P; ++p) {
+        double sp, cp;
+        sincospi(2.0 * ((double)kz[(size_t)p * K] * za), &sp, &cp);
+        double gr = 0.0, gi = 0.0;
+        for (int c = 0; c < ncoils; ++c) {
+            const int q = c * P + p, g = q / (2 * MF_COLS), col = q % (2 * MF_COLS);
+            const int tt = col / MF_COLS, cc = col % MF_COLS;
+            double vr = 0.0, vi = 0.0;
+            for (int ch = 0; ch < nchunks; ++ch) {
+                const size_t base = ((size_t)g * nchunks + ch) * 64 + tt * 32;
+                vr += partial[(base + cc) * n + a];
+                vi += partial[(base + MF_COLS + cc) * n + a];
+            }
+            double sr = 1.0, si = 0.0;
+            if (s_r) {
+                sr = s_r[(size_t)c * n + a];
+                si = s_i[(size_t)c * n + a];
+            }
+            gr += sr * vr + si * vi;
+            gi += sr * vi - si * vr;
+        }
+        orr += cp * gr - sp * gi;
+        oi += cp * gi + sp * gr;
+    }
+    out_r[a] = (float)(orr * unscale);
+    out_i[a] = (float)(oi * unscale);
+}
+
+// Partition length K when the samples form a stack (phi, if given, repeats
+// too), else 0.
+int find_stack(imp_ctx *ctx, int num_k, const float *kx, const float *ky, const float *kz,
+               const float *t, const float *phi_r, const float *phi_i, int *K_out) {
+    *K_out = 0;
+    if (!kz || num_k < 2 * MF_K) return IMP_OK;
+    const hipStream_t st = ctx->stream;
+    void *w = nullptr;
+    IMP_TRY(workspace(ctx, WS_STACK, 2 * sizeof(int), &w));
+    int *dev = (int *)w;
+    int h[2] = {num_k, 0};
+    IMP_HIP(hipMemcpyAsync(dev, h, sizeof(h), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_first_change, dim3(ceil_div(num_k, 256)), dim3(256), 0, st, num_k, kz,
+                       dev);
+    IMP_CHECK_LAUNCH();
+    IMP_HIP(hipMemcpyAsync(h, dev, sizeof(int), hipMemcpyDeviceToHost, st));
+    IMP_HIP(hipStreamSynchronize(st));
+    const int K = h[0];
+    if (K >= num_k || K < MF_K || num_k % K) return IMP_OK;
+    hipLaunchKernelGGL(k_stack_check, dim3(ceil_div(num_k, 256)), dim3(256), 0, st, num_k, K,
+                       kx, ky, kz, t, phi_r, phi_i, dev + 1);
+    IMP_CHECK_LAUNCH();
+    IMP_HIP(hipMemcpyAsync(&h[1], dev + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+    IMP_HIP(hipStreamSynchronize(st));
+    if (!h[1]) *K_out = K;
+    return IMP_OK;
+}
+
+// Partition length for the separable F^H d, or 0 when the samples are not a
+// stack, the separable form would not save at least half of the phase
+// evaluations, or its partials would exceed STACK_PART_CAP.
+constexpr size_t STACK_PART_CAP = (size_t)8 << 30;
+
+int detect_stack(imp_ctx *ctx, int num_k, int ncoils, int num_x, const float *kx,
+                 const float *ky, const float *kz, const float *t, int *K_out) {
+    *K_out = 0;
+    int K = 0;
+    IMP_TRY(find_stack(ctx, num_k, kx, ky, kz, t, nullptr, nullptr, &K));
+    if (K == 0) return IMP_OK;
+    const int P = num_k / K;
+    const long groups = ceil_div((long)ncoils * P, 2 * MF_COLS);
+    if (2 * groups > P) return IMP_OK;
+    const long wgs = ceil_div(num_x, MF_WAVE_PTS * (NU_THREADS / 64));
+    const int chunks = pick_chunks(ctx, wgs, ceil_div(K, MF_K), 4 * MF_FOLD);
+    if ((size_t)groups * chunks * 64 * num_x * sizeof(float) > STACK_PART_CAP) return IMP_OK;
+    *K_out = K;
+    return IMP_OK;
+}
+
+// F^H d of a stack (W = conj(phi) d built, lanes built with the field map).
+int run_stack_adjoint(imp_ctx *ctx, int num_k, int K, int ncoils, const float4 *lanes, int n,
+                      const float *kx, const float *ky, const float *kz, const float *t,
+                      const float *z, const float2 *W, const float *s_r, const float *s_i,
+                      float *out_r, float *out_i) {
+    const hipStream_t st = ctx->stream;
+    const int P = num_k / K;
+    const int nrhs = ncoils * P;
+    const int groups = ceil_div(nrhs, 2 * MF_COLS);
+    constexpr int tiles = 2;
+    const double *spos = nullptr;
+    IMP_TRY(build_spos(ctx, K, kx, ky, nullptr, t, 1.0, &spos));
+    void *mb = nullptr, *bp = nullptr, *part = nullptr;
+    IMP_TRY(workspace(ctx, WS_MAXBITS, 256, &mb));
+    unsigned int *maxbits = (unsigned int *)mb;
+    IMP_HIP(hipMemsetAsync(maxbits, 0, sizeof(unsigned int), st));
+    hipLaunchKernelGGL(k_absmax, dim3(512), dim3(256), 0, st, W, (size_t)nrhs * K, maxbits);
+    IMP_CHECK_LAUNCH();
+    const int n_steps = ceil_div(K, MF_K);
+    const size_t nb = (size_t)(n_steps + 1) * tiles * 64;
+    IMP_TRY(workspace(ctx, WS_BPACK, nb * sizeof(BPair), &bp));
+    const long wgs = ceil_div(n, MF_WAVE_PTS * (NU_THREADS / 64));
+    const int chunks = pick_chunks(ctx, wgs, n_steps, 4 * MF_FOLD);
+    const int spc = ceil_div(n_steps, chunks);
+    const size_t per_group = (size_t)chunks * tiles * 32 * n;
+    IMP_TRY(workspace(ctx, WS_PARTIAL, groups * per_group * sizeof(float), &part));
+    for (int g = 0; g < groups; ++g) {
+        const int nrhs_g = std::min(2 * MF_COLS, nrhs - g * 2 * MF_COLS);
+        hipLaunchKernelGGL(k_pack_mfma, dim3(ceil_div((long)nb, 256)), dim3(256), 0, st, K,
+                           nrhs_g, tiles, n_steps + 1, 1.f, W + (size_t)g * 2 * MF_COLS * K,
+                           maxbits, (BPair *)bp);
+        IMP_CHECK_LAUNCH();
+        const int rec = prof_begin(ctx, "k_nudft_stack");
+        hipLaunchKernelGGL(pick_mfma(tiles, false), dim3((unsigned)wgs, chunks), dim3(NU_THREADS),
+                           0, st, lanes, n, spos, (const BPair *)bp, n_steps, spc,
+                           (float *)part + g * per_group, 1.0f);
+        prof_end(ctx, rec);
+        IMP_CHECK_LAUNCH();
+    }
+    const int rec = prof_begin(ctx, "k_reduce_stack");
+    hipLaunchKernelGGL(k_reduce_stack, dim3(ceil_div(n, 256)), dim3(256), 0, st,
+                       (const float *)part, chunks, ncoils, P, K, n, maxbits, kz, z, s_r, s_i,
+                       out_r, out_i);
+    prof_end(ctx, rec);
+    IMP_CHECK_LAUNCH();
+    return IMP_OK;
+}
+
 }  // namespace
 
 // ---- operator entry points (device pointers) --------------------------------
@@ -833,8 +999,10 @@ int nudft_adjoint(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *k
         (phi_r && !phi_i))
         return fail(IMP_EINVAL, "nudft_adjoint: bad arguments");
     const bool d3 = kz && z;
+    int K = 0;
+    if (d3) IMP_TRY(detect_stack(ctx, num_k, ncoils, num_x, kx, ky, kz, t, &K));
     const double *spos = nullptr;
-    IMP_TRY(build_spos(ctx, num_k, kx, ky, d3 ? kz : nullptr, t, 1.0, &spos));
+    if (K == 0) IMP_TRY(build_spos(ctx, num_k, kx, ky, d3 ? kz : nullptr, t, 1.0, &spos));
     void *lanes = nullptr, *W = nullptr;
     IMP_TRY(workspace(ctx, WS_LANES, (size_t)num_x * sizeof(float4), &lanes));
     hipLaunchKernelGGL(k_lanes_from_arrays, dim3(ceil_div(num_x, 256)), dim3(256), 0,
@@ -845,6 +1013,9 @@ int nudft_adjoint(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *k
     hipLaunchKernelGGL(k_w_adjoint, dim3(ceil_div((long)num_k * ncoils, 256)), dim3(256), 0,
                        ctx->stream, num_k, ncoils, phi_r, phi_i, d_r, d_i, (float2 *)W);
     IMP_CHECK_LAUNCH();
+    if (K > 0)
+        return run_stack_adjoint(ctx, num_k, K, ncoils, (const float4 *)lanes, num_x, kx, ky,
+                                 kz, t, z, (const float2 *)W, s_r, s_i, out_r, out_i);
     Job J{(const float4 *)lanes, num_x, spos, num_k, (const float2 *)W, ncoils, d3, +1.f,
           Out::Combine, s_r, s_i, out_r, out_i};
     return run_job(ctx, J);
@@ -1728,6 +1899,45 @@ int toeplitz_q_sep(imp_ctx *ctx, int num_k, const float *kx, const float *ky, co
 
 }  // namespace
 
+namespace {
+
+// Q_l of a stack on the standard 3D grid: the Q weights a_l(t) |phi|^2 and
+// kx, ky repeat in every partition, so
+//   Q_l(x, y, z) = Q2_l(x, y) Z(z),  Z(z) = sum_p e^{+i 2pi kz_p z},
+// with Q2_l the 2D kernel of one partition.  Z is evaluated in FP64 at the
+// grid's z = (iz - Nz) / Nz (the FP32 value the direct kernel uses).
+__global__ void k_qz_factor(int P, int K, const float *kz, int Nz, double2 *Z) {
+    const int iz = blockIdx.x * blockDim.x + threadIdx.x;
+    if (iz >= 2 * Nz) return;
+    const double z = (double)((float)(iz - Nz) / (float)Nz);
+    double zr = 0.0, zi = 0.0;
+    for (int p = 0; p < P; ++p) {
+        double s, c;
+        sincospi(2.0 * ((double)kz[(size_t)p * K] * z), &s, &c);
+        zr += c;
+        zi += s;
+    }
+    Z[iz] = make_double2(zr, zi);
+}
+
+// q[l][(iy M2 + ix) M3 + iz] = q2[l][iy M2 + ix] * Z[iz]
+__global__ void k_q_outer(int nseg, long n2, int M3, const float *q2r, const float *q2i,
+                          const double2 *Z, float *q_r, float *q_i) {
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long nq = n2 * M3;
+    if (g >= nq) return;
+    const int iz = (int)(g % M3);
+    const long p2 = g / M3;
+    const double2 z = Z[iz];
+    for (int l = 0; l < nseg; ++l) {
+        const double ar = q2r[(size_t)l * n2 + p2], ai = q2i[(size_t)l * n2 + p2];
+        q_r[(size_t)l * nq + g] = (float)(ar * z.x - ai * z.y);
+        q_i[(size_t)l * nq + g] = (float)(ar * z.y + ai * z.x);
+    }
+}
+
+}  // namespace
+
 int toeplitz_q(imp_ctx *ctx, int num_k, const float *kx, const float *ky,
                const float *kz, const float *t, const float *phi_r,
                const float *phi_i, int Nx, int Ny, int Nz, int num_q,
@@ -1752,6 +1962,33 @@ int toeplitz_q(imp_ctx *ctx, int num_k, const float *kx, const float *ky,
     const bool d3 = kz && (std_grid ? Nz > 1 : gz != nullptr);
     const int nseg = ntime_segs + 1;
     if (nseg > 32) return fail(IMP_EUNSUPPORTED, "toeplitz_q: at most 31 time segments");
+    // a stack on the standard grid: the 2D kernel of one partition times Z(z)
+    if (std_grid && d3 && !(flags & IMP_Q_DIRECT)) {
+        int K = 0;
+        IMP_TRY(find_stack(ctx, num_k, kx, ky, kz, t, phi_r, phi_i, &K));
+        if (K > 0) {
+            const long n2 = 4L * Nx * Ny;
+            const size_t planes = (size_t)nseg * n2;
+            void *buf = nullptr;
+            IMP_TRY(workspace(ctx, WS_QSTACK,
+                              2 * planes * sizeof(float) + 2 * Nz * sizeof(double2), &buf));
+            float *q2r = (float *)buf, *q2i = q2r + planes;
+            double2 *Z = (double2 *)(q2i + planes);
+            IMP_TRY(toeplitz_q(ctx, K, kx, ky, nullptr, t, phi_r, phi_i, Nx, Ny, 1, (int)n2,
+                               nullptr, nullptr, nullptr, ntime_segs, flags, t_host_first_last,
+                               q2r, q2i));
+            hipLaunchKernelGGL(k_qz_factor, dim3(ceil_div(2 * Nz, 64)), dim3(64), 0, ctx->stream,
+                               num_k / K, K, kz, Nz, Z);
+            IMP_CHECK_LAUNCH();
+            const int rec = prof_begin(ctx, "k_q_outer");
+            hipLaunchKernelGGL(k_q_outer, dim3(ceil_div(n2 * 2 * Nz, 256)), dim3(256), 0,
+                               ctx->stream, nseg, n2, 2 * Nz, (const float *)q2r,
+                               (const float *)q2i, (const double2 *)Z, q_r, q_i);
+            prof_end(ctx, rec);
+            IMP_CHECK_LAUNCH();
+            return IMP_OK;
+        }
+    }
     // tau and t0 as the reference computes them, in FP32 on the host
     // (computeQ.cmem.cu:870-871)
     const float t0 = t_host_first_last[0];

@@ -82,11 +82,14 @@ def main(argv=None):
     ops.nudft_adjoint(ctx, p.kx, p.ky, p.kz, p.t, p.x, p.y, p.z, p.fm, d, p.sens)
     timer_on()
     b = ops.nudft_adjoint(ctx, p.kx, p.ky, p.kz, p.t, p.x, p.y, p.z, p.fm, d, p.sens)
-    fhd_ms, nl = kernel_ms("k_nudft_mfma")
-    if nl == 0:
-        fhd_ms, nl = kernel_ms("k_nudft_valu")
+    # direct kernels, or the stacked path's group launches and its reduction
+    parts = {k: kernel_ms(k) for k in ("k_nudft_mfma", "k_nudft_valu", "k_nudft_stack",
+                                       "k_reduce_stack")}
+    fhd_ms = sum(v[0] for v in parts.values())
+    path = "stacked (z-separable)" if parts["k_nudft_stack"][1] else "direct"
     pairs = float(nc) * nk * nv
-    log(f"F^H d: {fhd_ms:.1f} ms in {nl} launch(es), {pairs / fhd_ms / 1e6:.1f} GSample*vox/s")
+    log(f"F^H d ({path}): {fhd_ms:.1f} ms, {pairs / fhd_ms / 1e6:.1f} GSample*vox/s; "
+        + ", ".join(f"{k} {v[1]} x {v[0] / max(v[1], 1):.2f} ms" for k, v in parts.items() if v[1]))
 
     # Q_l and the plan from device arrays (no host round trip of the planes)
     up = lambda a: ctx.to_device(np.ascontiguousarray(a, dtype=np.float32).reshape(-1))
@@ -132,7 +135,8 @@ def main(argv=None):
             f"true residual {st['true_rel_residual']:.2e}")
     op.close()
     out = {"config": p.name, "nsamples": nk, "nvox": nv, "ncoils": nc, "segments": L,
-           "fhd_ms": round(fhd_ms, 3), "fhd_gsample_vox_s": round(pairs / fhd_ms / 1e6, 1),
+           "fhd_path": path, "fhd_ms": round(fhd_ms, 3),
+           "fhd_gsample_vox_s": round(pairs / fhd_ms / 1e6, 1),
            "q_s": round(t_q, 4), "toeplitz_plan_s": round(t_plan, 4),
            "toeplitz_passes": passes, "tol": args.tol, "solves": solves}
     print(json.dumps(out), flush=True)

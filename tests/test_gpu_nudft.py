@@ -34,6 +34,38 @@ def test_adjoint_matches_oracle(ctx, N, Nz, ncoils):
     assert err < OP_TOL
 
 
+@pytest.mark.parametrize("N,Nz,ncoils,perturb", [(8, 8, 3, False), (8, 8, 3, True),
+                                                 (8, 16, 3, False), (12, 16, 4, False)])
+def test_adjoint_stacked_trajectory_path(ctx, N, Nz, ncoils, perturb):
+    """A stack of 2D trajectories (every kz partition repeats partition 0's
+    kx, ky and t) takes the z-separable path (k_nudft_stack: 24, 48 and 64
+    right-hand sides, one and two groups); one perturbed sample sends the
+    same data to the direct 3D kernel.  Both match the FP64 oracle."""
+    import ctypes as C
+    from impatient import ops
+    from impatient._lib import call
+    from oracle import oracle as orc
+    p = _prob(N=N, Nz=Nz, ncoils=ncoils)
+    kx = p.kx.copy()
+    if perturb:
+        kx[len(kx) // 2 + 3] += np.float32(1e-3)
+    rng = np.random.default_rng(11)
+    d = (rng.standard_normal((ncoils, p.nk)) + 1j * rng.standard_normal((ncoils, p.nk))).astype(np.complex64)
+    phi = (1 + 0.3 * rng.standard_normal((ncoils, p.nk)) + 0.2j * rng.standard_normal((ncoils, p.nk)))
+    call("imp_profile_enable", ctx.h, C.c_int(1))
+    call("imp_profile_reset", ctx.h)
+    got = ops.nudft_adjoint(ctx, kx, p.ky, p.kz, p.t, p.x, p.y, p.z, p.fm, d, p.sens, phi)
+    tot, n = C.c_double(), C.c_int()
+    call("imp_profile_read", ctx.h, b"k_nudft_stack", C.byref(tot), C.byref(n))
+    call("imp_profile_enable", ctx.h, C.c_int(0))
+    ref = orc.nudft_adjoint(kx, p.ky, p.kz, p.t, p.x, p.y, p.z, p.fm, d, p.sens, phi)
+    err = rel_l2(got, ref)
+    print(f"adjoint stack N={N} Nz={Nz} C={ncoils} perturbed={perturb}: "
+          f"{n.value} stacked launches, rel_l2={err:.2e}")
+    assert (n.value > 0) == (not perturb)
+    assert err < OP_TOL
+
+
 @pytest.mark.parametrize("N,Nz,ncoils", [(16, 1, 1), (16, 1, 4), (8, 4, 3)])
 def test_forward_matches_oracle(ctx, N, Nz, ncoils):
     from impatient import ops
@@ -89,6 +121,39 @@ def test_q_matches_oracle(ctx, N, Nz, L):
     err = rel_l2(got, ref)
     print(f"Q N={N} Nz={Nz} L={L}: rel_l2={err:.2e}")
     assert got.shape == ref.shape
+    assert err < OP_TOL
+
+
+@pytest.mark.parametrize("N,Nz,L,phi_mode", [(8, 4, 3, None), (12, 8, 4, "stack"),
+                                              (8, 4, 3, "free")])
+def test_q_stacked_trajectory_path(ctx, N, Nz, L, phi_mode):
+    """Q_l of a stack is the 2D kernel of one partition times
+    Z(z) = sum_p e^{+i 2pi kz_p z} (k_q_outer).  Density weights that repeat
+    per partition keep the stacked path; weights that differ between
+    partitions send it to the 3D kernels.  All match the FP64 oracle."""
+    import ctypes as C
+    from impatient import ops
+    from impatient._lib import call
+    from oracle import oracle as orc
+    p = _prob(N=N, Nz=Nz, ncoils=1, ntime_segs=L)
+    rng = np.random.default_rng(12)
+    K = p.nk // Nz
+    phi = None
+    if phi_mode == "stack":
+        phi = np.tile(1 + 0.3 * rng.standard_normal(K) + 0.2j * rng.standard_normal(K), Nz)
+    elif phi_mode == "free":
+        phi = 1 + 0.3 * rng.standard_normal(p.nk) + 0.2j * rng.standard_normal(p.nk)
+    call("imp_profile_enable", ctx.h, C.c_int(1))
+    call("imp_profile_reset", ctx.h)
+    got = ops.toeplitz_q(ctx, p.kx, p.ky, p.kz, p.t, N, N, Nz, L, phi=phi)
+    tot, n = C.c_double(), C.c_int()
+    call("imp_profile_read", ctx.h, b"k_q_outer", C.byref(tot), C.byref(n))
+    call("imp_profile_enable", ctx.h, C.c_int(0))
+    ref = orc.toeplitz_q(p.kx, p.ky, p.kz, p.t, N, N, Nz, L, phi=phi)
+    err = rel_l2(got, ref)
+    print(f"Q stack N={N} Nz={Nz} L={L} phi={phi_mode}: {n.value} outer-product launches, "
+          f"rel_l2={err:.2e}")
+    assert (n.value == 1) == (phi_mode != "free")
     assert err < OP_TOL
 
 
