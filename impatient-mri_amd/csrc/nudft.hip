@@ -467,6 +467,28 @@ __global__ void k_w_forward(int num_x, int ncoils, const float *rho_r, const flo
     W[i] = make_float2(sr * rr - si * ri, sr * ri + si * rr);
 }
 
+// forward of a stack: W[(c P + p)][r] = S_c(r) rho(r) e^{-i 2pi kz_p z_r}
+// (the kz factor of partition p, FP64 phase)
+__global__ void k_w_forward_stack(int num_x, int ncoils, int P, int K, const float *kz,
+                                  const float *z, const float *rho_r, const float *rho_i,
+                                  const float *s_r, const float *s_i, float2 *W) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)num_x * ncoils * P) return;
+    const int v = (int)(i % num_x);
+    const int q = (int)(i / num_x);
+    const int c = q / P, p = q % P;
+    const double rr = rho_r[v], ri = rho_i[v];
+    double sr = 1.0, si = 0.0;
+    if (s_r) {
+        sr = s_r[(size_t)c * num_x + v];
+        si = s_i[(size_t)c * num_x + v];
+    }
+    const double ar = sr * rr - si * ri, ai = sr * ri + si * rr;
+    double sn, cs;
+    sincospi(2.0 * ((double)kz[(size_t)p * K] * (double)z[v]), &sn, &cs);
+    W[i] = make_float2((float)(ar * cs + ai * sn), (float)(ai * cs - ar * sn));
+}
+
 // Q: a_l(t_k) |phi_k|^2 with the Hanning interpolator of hanning_d
 // (computeQ.cmem.cu:125-138), evaluated in FP64.
 __global__ void k_w_q(int num_k, int nseg, const float *t, const float *phi_r,
@@ -724,6 +746,7 @@ struct Job {
     Out out;
     const float *s_r, *s_i;  // Combine only
     float *out_r, *out_i;
+    const char *prof = "k_nudft_mfma";   // kernel-timer name of the MFMA launch
 };
 
 int run_job(imp_ctx *ctx, const Job &J) {
@@ -755,7 +778,7 @@ int run_job(imp_ctx *ctx, const Job &J) {
         const int chunks = pick_chunks(ctx, wgs, n_steps, 4 * MF_FOLD);
         const int spc = ceil_div(n_steps, chunks);
         IMP_TRY(workspace(ctx, WS_PARTIAL, (size_t)chunks * tiles * 32 * n * sizeof(float), &part));
-        const int rec = prof_begin(ctx, "k_nudft_mfma");
+        const int rec = prof_begin(ctx, J.prof);
         hipLaunchKernelGGL(pick_mfma(tiles, J.d3), dim3((unsigned)wgs, chunks), dim3(NU_THREADS),
                            0, st, J.lanes, n, J.spos, (const BPair *)bp, n_steps, spc,
                            (float *)part, 1.0f);
@@ -829,10 +852,13 @@ int build_spos(imp_ctx *ctx, int n, const float *x, const float *y, const float 
 // [C][P][K], i.e. already [(c P + p)][k].  The kz factor is applied in FP64
 // in the reduction.
 
-// smallest i with kz[i] != kz[0] (first initialised to n)
+// smallest i with kz[i] != kz[0] (first initialised to n); one atomic per wave
 __global__ void k_first_change(int n, const float *kz, int *first) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n && kz[i] != kz[0]) atomicMin(first, i);
+    int m = (i < n && kz[i] != kz[0]) ? i : n;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = min(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0 && m < n) atomicMin(first, m);
 }
 
 // bad = 1 unless every partition repeats partition 0's kx, ky, t (and phi,
@@ -845,7 +871,9 @@ __global__ void k_stack_check(int n, int K, const float *kx, const float *ky, co
     const int j = i % K;
     bool same = kx[i] == kx[j] && ky[i] == ky[j] && t[i] == t[j] && kz[i] == kz[i - j];
     if (phi_r) same = same && phi_r[i] == phi_r[j] && phi_i[i] == phi_i[j];
-    if (!same) atomicOr(bad, 1);
+    // one atomic per wave that holds a mismatch
+    const unsigned long long miss = __ballot(!same);
+    if (miss && (threadIdx.x & 63) == (unsigned)(__ffsll(miss) - 1)) atomicOr(bad, 1);
 }
 
 // Partials of the group launches [group][chunk][64][n] (32 rhs per group,
@@ -1147,6 +1175,41 @@ int nudft_forward(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *k
         !x || !y || !rho_r || !rho_i || !d_r || !d_i || (s_r && !s_i))
         return fail(IMP_EINVAL, "nudft_forward: bad arguments");
     const bool d3 = kz && z;
+    // a stack: a 2D forward NUDFT of one partition with the C P coil-partition
+    // pairs as right-hand sides, the kz factor folded into their weights
+    int K = 0;
+    if (d3) {
+        IMP_TRY(find_stack(ctx, num_k, kx, ky, kz, t, nullptr, nullptr, &K));
+        if (K > 0) {
+            const int P = num_k / K;
+            const long groups = ceil_div((long)ncoils * P, 2 * MF_COLS);
+            if (2 * groups > P || (size_t)ncoils * P * num_x > ((size_t)1 << 30)) K = 0;
+        }
+    }
+    if (K > 0) {
+        const int P = num_k / K, nrhs = ncoils * P;
+        const double *spos = nullptr;
+        IMP_TRY(build_spos(ctx, num_x, x, y, nullptr, fm, 1.0 / (2.0 * M_PI), &spos));
+        void *lanes = nullptr, *W = nullptr;
+        IMP_TRY(workspace(ctx, WS_LANES, (size_t)K * sizeof(float4), &lanes));
+        hipLaunchKernelGGL(k_lanes_from_arrays, dim3(ceil_div(K, 256)), dim3(256), 0,
+                           ctx->stream, K, kx, ky, nullptr, t, 1.0, (float4 *)lanes);
+        IMP_CHECK_LAUNCH();
+        const size_t nw = (size_t)nrhs * num_x;
+        IMP_TRY(workspace(ctx, WS_W, nw * sizeof(float2), &W));
+        hipLaunchKernelGGL(k_w_forward_stack, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0,
+                           ctx->stream, num_x, ncoils, P, K, kz, z, rho_r, rho_i, s_r, s_i,
+                           (float2 *)W);
+        IMP_CHECK_LAUNCH();
+        for (int q0 = 0; q0 < nrhs; q0 += 2 * MF_COLS) {
+            const int nq = std::min(2 * MF_COLS, nrhs - q0);
+            Job J{(const float4 *)lanes, K, spos, num_x, (const float2 *)W + (size_t)q0 * num_x,
+                  nq, false, -1.f, Out::PerRhs, nullptr, nullptr, d_r + (size_t)q0 * K,
+                  d_i + (size_t)q0 * K, "k_nudft_stack"};
+            IMP_TRY(run_job(ctx, J));
+        }
+        return IMP_OK;
+    }
     const double *spos = nullptr;
     IMP_TRY(build_spos(ctx, num_x, x, y, d3 ? z : nullptr, fm, 1.0 / (2.0 * M_PI), &spos));
     void *lanes = nullptr, *W = nullptr;

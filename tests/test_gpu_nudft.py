@@ -66,6 +66,37 @@ def test_adjoint_stacked_trajectory_path(ctx, N, Nz, ncoils, perturb):
     assert err < OP_TOL
 
 
+@pytest.mark.parametrize("N,Nz,ncoils,perturb", [(8, 8, 3, False), (8, 8, 3, True),
+                                                 (12, 16, 4, False)])
+def test_forward_stacked_trajectory_path(ctx, N, Nz, ncoils, perturb):
+    """The forward NUDFT of a stack runs as a 2D NUDFT of one partition with
+    the coil-partition pairs as right-hand sides (their weights carry
+    e^{-i 2pi kz_p z}); a perturbed sample sends it to the 3D kernel."""
+    import ctypes as C
+    from impatient import ops
+    from impatient._lib import call
+    from oracle import oracle as orc
+    p = _prob(N=N, Nz=Nz, ncoils=ncoils)
+    t = p.t.copy()
+    if perturb:
+        t[len(t) // 2 + 5] += np.float32(1e-6)
+    rng = np.random.default_rng(13)
+    rho = (rng.standard_normal(p.nvox) + 1j * rng.standard_normal(p.nvox)).astype(np.complex64)
+    call("imp_profile_enable", ctx.h, C.c_int(1))
+    call("imp_profile_reset", ctx.h)
+    got = ops.nudft_forward(ctx, p.kx, p.ky, p.kz, t, p.x, p.y, p.z, p.fm, rho, p.sens, ncoils)
+    tot, n = C.c_double(), C.c_int()
+    call("imp_profile_read", ctx.h, b"k_nudft_stack", C.byref(tot), C.byref(n))
+    call("imp_profile_enable", ctx.h, C.c_int(0))
+    ref = orc.nudft_forward(p.kx, p.ky, p.kz, t, p.x, p.y, p.z, p.fm, rho, p.sens)
+    err = rel_l2(got, ref)
+    print(f"forward stack N={N} Nz={Nz} C={ncoils} perturbed={perturb}: "
+          f"{n.value} stacked launches, rel_l2={err:.2e}")
+    assert (n.value > 0) == (not perturb)
+    assert got.shape == (ncoils, p.nk)
+    assert err < OP_TOL
+
+
 @pytest.mark.parametrize("N,Nz,ncoils", [(16, 1, 1), (16, 1, 4), (8, 4, 3)])
 def test_forward_matches_oracle(ctx, N, Nz, ncoils):
     from impatient import ops
