@@ -226,6 +226,16 @@ __device__ __forceinline__ void mfma_body(const float4 *__restrict__ lanes, int 
     BPair nxt[TILES];
 #pragma unroll
     for (int t = 0; t < TILES; ++t) nxt[t] = bpack[((size_t)min(s0, n_steps) * TILES + t) * 64 + l];
+    // positions of the current step (wave-uniform, scalar loads); the next
+    // step's are loaded as soon as these are consumed, so their latency
+    // overlaps the MFMAs instead of stalling the next step's phases
+    constexpr int NPOS = pos_stride(D3);
+    double pos[NPOS];
+    {
+        const double *B = spos + (size_t)min(s0, max(n_steps - 1, 0)) * NPOS;
+#pragma unroll
+        for (int j = 0; j < NPOS; ++j) pos[j] = B[j];
+    }
 
     for (int f0 = s0; f0 < s1; f0 += MF_FOLD) {
         const int f1 = min(s1, f0 + MF_FOLD);
@@ -236,17 +246,25 @@ __device__ __forceinline__ void mfma_body(const float4 *__restrict__ lanes, int 
                 cur[t] = nxt[t];
                 nxt[t] = bpack[((size_t)(st + 1) * TILES + t) * 64 + l];
             }
-            const double *B = spos + (size_t)st * pos_stride(D3);
             float cv[MF_K], sv[MF_K];
 #pragma unroll
             for (int j = 0; j < MF_K; ++j) {
-                double u = B[2 * MF_K + j] * pw;
-                if constexpr (D3) u = __builtin_fma(B[3 * MF_K + j], pz, u);
-                u = __builtin_fma(B[MF_K + j], py, u);
-                u = __builtin_fma(B[j], px, u);
+                double u = pos[2 * MF_K + j] * pw;
+                if constexpr (D3) u = __builtin_fma(pos[3 * MF_K + j], pz, u);
+                u = __builtin_fma(pos[MF_K + j], py, u);
+                u = __builtin_fma(pos[j], px, u);
                 const float r = rev_fract(u);
                 cv[j] = __builtin_amdgcn_cosf(r);
                 sv[j] = __builtin_amdgcn_sinf(r);
+            }
+            {
+                // next step (the last step reloads itself: stays inside spos)
+                const double *B = spos + (size_t)min(st + 1, s1 - 1) * NPOS;
+#pragma unroll
+                for (int j = 0; j < NPOS; ++j) pos[j] = B[j];
+                // keep the loads here: the scheduler would otherwise sink
+                // them below the MFMAs
+                __builtin_amdgcn_sched_barrier(0);
             }
             u32x4 chu, clu, shu, slu;
             split8(cv, one, chu, clu);
