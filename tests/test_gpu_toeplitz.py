@@ -54,6 +54,35 @@ def test_apply_is_normal_operator_without_fieldmap(ctx):
     assert err < 1e-5
 
 
+@pytest.mark.parametrize("cfg,Ls", [(0, (2, 4, 8, 12)), (1, (2, 4, 6, 12))])
+def test_time_segmentation_model_error(ctx, cfg, Ls):
+    """Model error of the time-segmented Toeplitz operator (the reference's
+    approximation of F^H F with a field map, L + 1 Hanning interpolators)
+    against the exact F^H F (forward then adjoint NUDFT with the field map),
+    on the configuration's phantom, trajectory, coils and field map: stated
+    per L; it falls with L (configuration 0 runs at the reference default
+    L = 12, configuration 1 at L = 6)."""
+    from impatient import ops, synth
+    from oracle import oracle as orc
+    p = synth.config(cfg)
+    v = p.rho.astype(np.complex64)
+    nc = p.ncoils
+    Fv = ops.nudft_forward(ctx, p.kx, p.ky, p.kz, p.t, p.x, p.y, p.z, p.fm, v, p.sens, nc)
+    exact = ops.nudft_adjoint(ctx, p.kx, p.ky, p.kz, p.t, p.x, p.y, p.z, p.fm, Fv, p.sens)
+    errs = {}
+    for L in Ls:
+        tau, t0 = orc.segment_timing(p.t, L)
+        q = ops.toeplitz_q(ctx, p.kx, p.ky, p.kz, p.t, p.Nx, p.Ny, p.Nz, L)
+        op = ops.Toeplitz(ctx, p.shape, q, tau, t0, p.fm, p.sens, 0.0)
+        errs[L] = rel_l2(op.apply(v), exact)
+        op.close()
+    print(f"{p.name} time-segmentation model error: "
+          + ", ".join(f"L={L} {e:.2e}" for L, e in errs.items()))
+    vals = [errs[L] for L in Ls]
+    assert all(b < a for a, b in zip(vals, vals[1:]))
+    assert errs[Ls[-1]] < 1e-2
+
+
 @pytest.mark.parametrize("shape,weighted", [((16, 16), False), ((16, 16), True),
                                             ((8, 12, 4), False), ((8, 8, 6), True)])
 def test_fd_reg_matches_oracle(ctx, shape, weighted):
