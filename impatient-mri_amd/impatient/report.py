@@ -86,8 +86,17 @@ def main(argv=None):
     parts = {k: kernel_ms(k) for k in ("k_nudft_mfma", "k_nudft_valu", "k_nudft_stack",
                                        "k_reduce_stack")}
     fhd_ms = sum(v[0] for v in parts.values())
-    path = "stacked (z-separable)" if parts["k_nudft_stack"][1] else "direct"
+    stacked = parts["k_nudft_stack"][1] > 0
+    path = "stacked (z-separable)" if stacked else "direct"
     pairs = float(nc) * nk * nv
+    # (sample, voxel) pairs whose phase and sin/cos were evaluated: once per
+    # pair for all coils on the direct path; on the stacked path once per
+    # pair of one partition and group launch of 32 coil-partition columns
+    if stacked:
+        K = nk // p.Nz
+        evaluated = float(parts["k_nudft_stack"][1]) * K * nv
+    else:
+        evaluated = float(nk) * nv
     log(f"F^H d ({path}): {fhd_ms:.1f} ms, {pairs / fhd_ms / 1e6:.1f} GSample*vox/s; "
         + ", ".join(f"{k} {v[1]} x {v[0] / max(v[1], 1):.2f} ms" for k, v in parts.items() if v[1]))
 
@@ -136,6 +145,7 @@ def main(argv=None):
     op.close()
     out = {"config": p.name, "nsamples": nk, "nvox": nv, "ncoils": nc, "segments": L,
            "fhd_path": path, "fhd_ms": round(fhd_ms, 3),
+           "fhd_direct_equivalent_pairs": pairs, "fhd_evaluated_phase_pairs": evaluated,
            "fhd_gsample_vox_s": round(pairs / fhd_ms / 1e6, 1),
            "q_s": round(t_q, 4), "toeplitz_plan_s": round(t_plan, 4),
            "toeplitz_passes": passes, "tol": args.tol, "solves": solves}
