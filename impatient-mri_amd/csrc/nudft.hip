@@ -870,13 +870,17 @@ int build_spos(imp_ctx *ctx, int n, const float *x, const float *y, const float 
 // [C][P][K], i.e. already [(c P + p)][k].  The kz factor is applied in FP64
 // in the reduction.
 
-// smallest i with kz[i] != kz[0] (first initialised to n); one atomic per wave
+// smallest i with kz[i] != kz[0] (first initialised to n).  One atomic per
+// wave at most, and only when it would lower the current value: atomics on
+// one address serialise, and every wave past the first partition finds a change.
 __global__ void k_first_change(int n, const float *kz, int *first) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     int m = (i < n && kz[i] != kz[0]) ? i : n;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) m = min(m, __shfl_xor(m, off, 64));
-    if ((threadIdx.x & 63) == 0 && m < n) atomicMin(first, m);
+    if ((threadIdx.x & 63) == 0 && m < n &&
+        m < __hip_atomic_load(first, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMin(first, m);
 }
 
 // bad = 1 unless every partition repeats partition 0's kx, ky, t (and phi,
