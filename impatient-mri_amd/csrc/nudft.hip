@@ -52,6 +52,7 @@ constexpr int NU_THREADS = 256;
 constexpr int NU_BLK = 32;        // VALU: samples per FP32 block before the compensated fold
 constexpr int MF_K = 8;           // MFMA: samples per 32x32x16 step (8 Re + 8 Im rows of K)
 constexpr int MF_FOLD = 32;       // MFMA: steps per compensated fold (256 samples)
+constexpr int MF_BPAD = 2;        // MFMA: zero steps packed after the last (loads run 2 ahead)
 constexpr int MF_COLS = 16;       // MFMA: right-hand sides per 32-column tile (Re + Im)
 constexpr int W_EXP = 12;         // W is scaled so max |W| < 2^12
 constexpr int MF_WAVE_PTS = 64;   // MFMA: points per wave (two 32-point M tiles)
@@ -182,8 +183,8 @@ __global__ __launch_bounds__(NU_THREADS) void k_nudft_valu(
 // hi/lo vectors; v_permlane32_swap(cos, sin) then yields
 // [cos of A | sin of A] and [cos of B | sin of B], i.e. the A operands with
 // K rows (Re|Im, sample) that k_pack_mfma's B operands expect.  B operands
-// of the next step are fetched while the current one is used (bpack holds
-// one extra zero step).  tot stays in registers and the two-sum
+// are fetched two steps ahead into two alternating register sets (bpack
+// holds MF_BPAD zero steps after the last).  tot stays in registers and the two-sum
 // compensation in LDS.  Output planes: partial[chunk][TILES*32][n_lane]
 // (plane = tile*32 + part*16 + rhs%16).  `one` is 1.0f, passed at run time
 // so the lo split compiles to v_fma_mix.
@@ -194,7 +195,7 @@ struct MfmaLds {
 
 // Workgroup (bx, chunk) of one NUDFT job; shared by the single-job and the
 // batched launch.
-template <int TILES, bool D3>
+template <int TILES, bool D3, bool PAIRS>
 __device__ __forceinline__ void mfma_body(const float4 *__restrict__ lanes, int n_lane,
                                           const double *__restrict__ spos,
                                           const BPair *__restrict__ bpack, int n_steps,
@@ -223,9 +224,18 @@ __device__ __forceinline__ void mfma_body(const float4 *__restrict__ lanes, int 
             cmp_lds[wave][q * 16 + i][l] = 0.f;
         }
     }
-    BPair nxt[TILES];
+    // B operands in two register sets that alternate by step: step st uses
+    // b0 (even offset in a pair) or b1, and the set is refilled with step
+    // st + 2 as soon as st's MFMAs have read it, so no register holding a
+    // load in flight is ever copied (bpack holds MF_BPAD zero steps after the
+    // last, so st + 2 and st + 3 stay inside it)
+    BPair b0[TILES], b1[TILES];
+    const int sb = min(s0, n_steps);
 #pragma unroll
-    for (int t = 0; t < TILES; ++t) nxt[t] = bpack[((size_t)min(s0, n_steps) * TILES + t) * 64 + l];
+    for (int t = 0; t < TILES; ++t) {
+        b0[t] = bpack[((size_t)sb * TILES + t) * 64 + l];
+        b1[t] = bpack[((size_t)(sb + 1) * TILES + t) * 64 + l];
+    }
     // positions of the current step (wave-uniform, scalar loads); the next
     // step's are loaded as soon as these are consumed, so their latency
     // overlaps the MFMAs instead of stalling the next step's phases
@@ -237,67 +247,95 @@ __device__ __forceinline__ void mfma_body(const float4 *__restrict__ lanes, int 
         for (int j = 0; j < NPOS; ++j) pos[j] = B[j];
     }
 
+    // one step of 8 samples with B operands b
+    auto step = [&](int st, const BPair(&b)[TILES]) {
+        float cv[MF_K], sv[MF_K];
+#pragma unroll
+        for (int j = 0; j < MF_K; ++j) {
+            double u = pos[2 * MF_K + j] * pw;
+            if constexpr (D3) u = __builtin_fma(pos[3 * MF_K + j], pz, u);
+            u = __builtin_fma(pos[MF_K + j], py, u);
+            u = __builtin_fma(pos[j], px, u);
+            const float r = rev_fract(u);
+            cv[j] = __builtin_amdgcn_cosf(r);
+            sv[j] = __builtin_amdgcn_sinf(r);
+        }
+        {
+            // next step (the last step reloads itself: stays inside spos)
+            const double *B = spos + (size_t)min(st + 1, s1 - 1) * NPOS;
+#pragma unroll
+            for (int j = 0; j < NPOS; ++j) pos[j] = B[j];
+            // keep the loads here: the scheduler would otherwise sink them
+            // below the MFMAs
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        u32x4 chu, clu, shu, slu;
+        split8(cv, one, chu, clu);
+        split8(sv, one, shu, slu);
+        u32x4 hA, hB, lA, lB;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const auto rh = __builtin_amdgcn_permlane32_swap(chu[i], shu[i], false, false);
+            hA[i] = rh[0];
+            hB[i] = rh[1];
+            const auto rl = __builtin_amdgcn_permlane32_swap(clu[i], slu[i], false, false);
+            lA[i] = rl[0];
+            lB[i] = rl[1];
+        }
+        const half8 AhA = __builtin_bit_cast(half8, hA), AhB = __builtin_bit_cast(half8, hB);
+        const half8 AlA = __builtin_bit_cast(half8, lA), AlB = __builtin_bit_cast(half8, lB);
+#pragma unroll
+        for (int t = 0; t < TILES; ++t) {
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(AhA, b[t].hi, acc[t], 0, 0, 0);
+            acc[TILES + t] =
+                __builtin_amdgcn_mfma_f32_32x32x16_f16(AhB, b[t].hi, acc[TILES + t], 0, 0, 0);
+        }
+#pragma unroll
+        for (int t = 0; t < TILES; ++t) {
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(AhA, b[t].lo, acc[t], 0, 0, 0);
+            acc[TILES + t] =
+                __builtin_amdgcn_mfma_f32_32x32x16_f16(AhB, b[t].lo, acc[TILES + t], 0, 0, 0);
+        }
+#pragma unroll
+        for (int t = 0; t < TILES; ++t) {
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(AlA, b[t].hi, acc[t], 0, 0, 0);
+            acc[TILES + t] =
+                __builtin_amdgcn_mfma_f32_32x32x16_f16(AlB, b[t].hi, acc[TILES + t], 0, 0, 0);
+        }
+    };
+    auto refill = [&](BPair(&b)[TILES], int st) {
+#pragma unroll
+        for (int t = 0; t < TILES; ++t) b[t] = bpack[((size_t)st * TILES + t) * 64 + l];
+    };
+
     for (int f0 = s0; f0 < s1; f0 += MF_FOLD) {
         const int f1 = min(s1, f0 + MF_FOLD);
-        for (int st = f0; st < f1; ++st) {
-            BPair cur[TILES];
-#pragma unroll
-            for (int t = 0; t < TILES; ++t) {
-                cur[t] = nxt[t];
-                nxt[t] = bpack[((size_t)(st + 1) * TILES + t) * 64 + l];
+        if constexpr (PAIRS) {
+            int st = f0;
+            // invariant at the top: b0 holds step st, b1 step st + 1
+            for (; st + 1 < f1; st += 2) {
+                step(st, b0);
+                refill(b0, st + 2);
+                step(st + 1, b1);
+                refill(b1, st + 3);
             }
-            float cv[MF_K], sv[MF_K];
+            if (st < f1) {   // odd tail of a chunk: one step, then shift the sets
+                step(st, b0);
 #pragma unroll
-            for (int j = 0; j < MF_K; ++j) {
-                double u = pos[2 * MF_K + j] * pw;
-                if constexpr (D3) u = __builtin_fma(pos[3 * MF_K + j], pz, u);
-                u = __builtin_fma(pos[MF_K + j], py, u);
-                u = __builtin_fma(pos[j], px, u);
-                const float r = rev_fract(u);
-                cv[j] = __builtin_amdgcn_cosf(r);
-                sv[j] = __builtin_amdgcn_sinf(r);
+                for (int t = 0; t < TILES; ++t) b0[t] = b1[t];
+                refill(b1, st + 2);
             }
-            {
-                // next step (the last step reloads itself: stays inside spos)
-                const double *B = spos + (size_t)min(st + 1, s1 - 1) * NPOS;
+        } else {
+            // one set ahead, copied per step: fewer registers (the batched
+            // launch, whose job lookup leaves less room)
+            for (int st = f0; st < f1; ++st) {
+                BPair cur[TILES];
 #pragma unroll
-                for (int j = 0; j < NPOS; ++j) pos[j] = B[j];
-                // keep the loads here: the scheduler would otherwise sink
-                // them below the MFMAs
-                __builtin_amdgcn_sched_barrier(0);
-            }
-            u32x4 chu, clu, shu, slu;
-            split8(cv, one, chu, clu);
-            split8(sv, one, shu, slu);
-            u32x4 hA, hB, lA, lB;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const auto rh = __builtin_amdgcn_permlane32_swap(chu[i], shu[i], false, false);
-                hA[i] = rh[0];
-                hB[i] = rh[1];
-                const auto rl = __builtin_amdgcn_permlane32_swap(clu[i], slu[i], false, false);
-                lA[i] = rl[0];
-                lB[i] = rl[1];
-            }
-            const half8 AhA = __builtin_bit_cast(half8, hA), AhB = __builtin_bit_cast(half8, hB);
-            const half8 AlA = __builtin_bit_cast(half8, lA), AlB = __builtin_bit_cast(half8, lB);
-#pragma unroll
-            for (int t = 0; t < TILES; ++t) {
-                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(AhA, cur[t].hi, acc[t], 0, 0, 0);
-                acc[TILES + t] =
-                    __builtin_amdgcn_mfma_f32_32x32x16_f16(AhB, cur[t].hi, acc[TILES + t], 0, 0, 0);
-            }
-#pragma unroll
-            for (int t = 0; t < TILES; ++t) {
-                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(AhA, cur[t].lo, acc[t], 0, 0, 0);
-                acc[TILES + t] =
-                    __builtin_amdgcn_mfma_f32_32x32x16_f16(AhB, cur[t].lo, acc[TILES + t], 0, 0, 0);
-            }
-#pragma unroll
-            for (int t = 0; t < TILES; ++t) {
-                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(AlA, cur[t].hi, acc[t], 0, 0, 0);
-                acc[TILES + t] =
-                    __builtin_amdgcn_mfma_f32_32x32x16_f16(AlB, cur[t].hi, acc[TILES + t], 0, 0, 0);
+                for (int t = 0; t < TILES; ++t) {
+                    cur[t] = b0[t];
+                    b0[t] = bpack[((size_t)(st + 1) * TILES + t) * 64 + l];
+                }
+                step(st, cur);
             }
         }
 #pragma unroll
@@ -337,8 +375,8 @@ void k_nudft_mfma(const float4 *__restrict__ lanes, int n_lane, const double *__
                   const BPair *__restrict__ bpack, int n_steps, int steps_per_chunk,
                   float *__restrict__ partial, float one) {
     __shared__ MfmaLds<TILES> lds;
-    mfma_body<TILES, D3>(lanes, n_lane, spos, bpack, n_steps, steps_per_chunk, partial, one,
-                         blockIdx.x, blockIdx.y, lds);
+    mfma_body<TILES, D3, true>(lanes, n_lane, spos, bpack, n_steps, steps_per_chunk, partial,
+                               one, blockIdx.x, blockIdx.y, lds);
 }
 
 // One job of a batched launch (images of different sizes in one grid).
@@ -364,8 +402,9 @@ void k_nudft_mfma_batch(const MfmaJob *__restrict__ jobs, int njobs, float one) 
     while (j + 1 < njobs && jobs[j + 1].wg_begin <= g) ++j;
     const MfmaJob J = jobs[j];
     const int rem = g - J.wg_begin;
-    mfma_body<TILES, D3>(J.lanes, J.n_lane, J.spos, J.bpack, J.n_steps, J.steps_per_chunk,
-                         J.partial, one, rem % J.wgs_x, rem / J.wgs_x, lds);
+    mfma_body<TILES, D3, false>(J.lanes, J.n_lane, J.spos, J.bpack, J.n_steps,
+                                J.steps_per_chunk, J.partial, one, rem % J.wgs_x, rem / J.wgs_x,
+                                lds);
 }
 
 // ---- packing ---------------------------------------------------------------
@@ -785,11 +824,11 @@ int run_job(imp_ctx *ctx, const Job &J) {
         hipLaunchKernelGGL(k_absmax, dim3(512), dim3(256), 0, st, J.W,
                            (size_t)J.nrhs * J.n_stream, maxbits);
         IMP_CHECK_LAUNCH();
-        // one extra zero step: the kernel prefetches step st + 1
-        const size_t nb = (size_t)(n_steps + 1) * tiles * 64;
+        // MF_BPAD zero steps after the last: the kernel fetches up to step st + 2
+        const size_t nb = (size_t)(n_steps + MF_BPAD) * tiles * 64;
         IMP_TRY(workspace(ctx, WS_BPACK, nb * sizeof(BPair), &bp));
         hipLaunchKernelGGL(k_pack_mfma, dim3(ceil_div((long)nb, 256)), dim3(256), 0, st,
-                           J.n_stream, J.nrhs, tiles, n_steps + 1, J.sign, J.W, maxbits,
+                           J.n_stream, J.nrhs, tiles, n_steps + MF_BPAD, J.sign, J.W, maxbits,
                            (BPair *)bp);
         IMP_CHECK_LAUNCH();
         const long wgs = ceil_div(n, MF_WAVE_PTS * (NU_THREADS / 64));
@@ -1005,7 +1044,7 @@ int run_stack_adjoint(imp_ctx *ctx, int num_k, int K, int ncoils, const float4 *
     hipLaunchKernelGGL(k_absmax, dim3(512), dim3(256), 0, st, W, (size_t)nrhs * K, maxbits);
     IMP_CHECK_LAUNCH();
     const int n_steps = ceil_div(K, MF_K);
-    const size_t nb = (size_t)(n_steps + 1) * tiles * 64;
+    const size_t nb = (size_t)(n_steps + MF_BPAD) * tiles * 64;
     IMP_TRY(workspace(ctx, WS_BPACK, nb * sizeof(BPair), &bp));
     const long wgs = ceil_div(n, MF_WAVE_PTS * (NU_THREADS / 64));
     const int chunks = pick_chunks(ctx, wgs, n_steps, 4 * MF_FOLD);
@@ -1015,7 +1054,7 @@ int run_stack_adjoint(imp_ctx *ctx, int num_k, int K, int ncoils, const float4 *
     for (int g = 0; g < groups; ++g) {
         const int nrhs_g = std::min(2 * MF_COLS, nrhs - g * 2 * MF_COLS);
         hipLaunchKernelGGL(k_pack_mfma, dim3(ceil_div((long)nb, 256)), dim3(256), 0, st, K,
-                           nrhs_g, tiles, n_steps + 1, 1.f, W + (size_t)g * 2 * MF_COLS * K,
+                           nrhs_g, tiles, n_steps + MF_BPAD, 1.f, W + (size_t)g * 2 * MF_COLS * K,
                            maxbits, (BPair *)bp);
         IMP_CHECK_LAUNCH();
         const int rec = prof_begin(ctx, "k_nudft_stack");
@@ -1120,7 +1159,7 @@ int nudft_adjoint_batch(imp_ctx *ctx, int njobs, const imp_nudft_job *jobs, int 
         L.lanes = take((size_t)b.num_x * sizeof(float4));
         L.w = take((size_t)b.num_k * b.ncoils * sizeof(float2));
         L.mb = take(sizeof(unsigned int));
-        L.bp = take((size_t)(L.n_steps + 1) * tiles * 64 * sizeof(BPair));
+        L.bp = take((size_t)(L.n_steps + MF_BPAD) * tiles * 64 * sizeof(BPair));
         L.part = take((size_t)L.chunks * tiles * 32 * b.num_x * sizeof(float));
     }
     const size_t table = take((size_t)njobs * sizeof(MfmaJob));
@@ -1148,9 +1187,9 @@ int nudft_adjoint_batch(imp_ctx *ctx, int njobs, const imp_nudft_job *jobs, int 
         IMP_HIP(hipMemsetAsync(mb, 0, sizeof(unsigned int), st));
         hipLaunchKernelGGL(k_absmax, dim3(512), dim3(256), 0, st, (const float2 *)W,
                            (size_t)b.ncoils * b.num_k, mb);
-        const size_t nb = (size_t)(L.n_steps + 1) * tiles * 64;
+        const size_t nb = (size_t)(L.n_steps + MF_BPAD) * tiles * 64;
         hipLaunchKernelGGL(k_pack_mfma, dim3(ceil_div((long)nb, 256)), dim3(256), 0, st, b.num_k,
-                           b.ncoils, tiles, L.n_steps + 1, +1.f, (const float2 *)W,
+                           b.ncoils, tiles, L.n_steps + MF_BPAD, +1.f, (const float2 *)W,
                            (const unsigned int *)mb, bp);
         IMP_CHECK_LAUNCH();
         MfmaJob &J = tab[j];
