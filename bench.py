@@ -113,6 +113,47 @@ def cpu_baseline(prob, d_host, seconds_target=15.0):
                       f"samples x {nv} voxels = {pairs:.3e} pairs in {dt:.2f} s"}
 
 
+def cpu_bruteforce_cfg0(ctx):
+    """The reference's CPU brute force on its own small case (64x64, 1 coil,
+    8 x 1024 spiral, cg_num = 10, lambda = 1e-3): the oracle's FP32 C
+    restatement of bruteForceCpu (bruteforce_cpu) timed in full on the host,
+    beside imp_bruteforce_cg on the GPU (wall time from host buffers)."""
+    from oracle import oracle as _orc  # noqa: F401  (oracle package: checker/baseline only)
+    from impatient import ops, synth
+    so = REPO / "oracle" / "liboracle_cpu.so"
+    if not so.exists():
+        return None
+    lib = C.CDLL(str(so))
+    p = synth.config(0)
+    d = ops.nudft_forward(ctx, p.kx, p.ky, None, p.t, p.x, p.y, None, p.fm, p.rho, None, 1)
+    d = synth.add_noise(np.asarray(d).reshape(1, -1), 40.0, p.meta["seed"])
+    x0 = np.zeros(p.nvox, np.complex64)
+    run_gpu = lambda: ops.bruteforce_cg(ctx, p.kx, p.ky, p.t, p.x, p.y, p.fm, d, None,
+                                        p.cg_iters, p.lam, x0)
+    run_gpu()                                   # untimed: kernel code loading
+    t0 = time.perf_counter()
+    xg = run_gpu()
+    t_gpu = time.perf_counter() - t0
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+    P = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    keep = [f32(v) for v in (p.kx, p.ky, p.x, p.y, p.fm, p.t, d.real.reshape(-1),
+                             d.imag.reshape(-1))]
+    xr = np.zeros(p.nvox, np.float32)
+    xi = np.zeros(p.nvox, np.float32)
+    t0 = time.perf_counter()
+    lib.bruteforce_cpu(P(xr), P(xi), P(keep[6]), P(keep[7]), P(keep[0]), P(keep[1]), P(keep[2]),
+                       P(keep[3]), P(keep[4]), P(keep[5]), None, None, C.c_int(1), C.c_int(p.nk),
+                       C.c_int(p.Nx), C.c_int(p.Ny), C.c_int(p.cg_iters), C.c_float(p.lam))
+    t_cpu = time.perf_counter() - t0
+    xc = xr + 1j * xi
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"workload": "64x64x1 1-coil spiral 8x1024, brute-force CG, 9 iterations "
+                        "(cg_num 10), lambda 1e-3",
+            "cpu_s": round(t_cpu, 4), "gpu_s": round(t_gpu, 4),
+            "gpu_speedup": round(t_cpu / t_gpu, 1), "cores": cores, "kind": "port",
+            "gpu_vs_cpu_rel_l2": float(np.linalg.norm(xg - xc) / np.linalg.norm(xc))}
+
+
 MODEL = REPO / "impatient-mri_amd" / "models" / "cfg3_warmstart.npz"
 
 
@@ -414,6 +455,10 @@ def main():
             result["cpu_baseline"] = {k: (round(v, 4) if isinstance(v, float) else v)
                                       for k, v in cb.items()}
             log(f"cpu baseline: {cb['value']:.3f} GSample*vox/s ({cb['sample']})")
+            bf = cpu_bruteforce_cfg0(ctx)
+            if bf is not None:
+                result["cpu_baseline"]["bruteforce_small_case"] = bf
+                log(f"brute force, small case: CPU {bf['cpu_s']:.3f} s, GPU {bf['gpu_s']:.4f} s")
     print(json.dumps(result), flush=True)
     ctx.close()
 

@@ -144,3 +144,90 @@ void q_cpu(float *q_r, float *q_i, const float *kx, const float *ky, const float
         q_i[j] = si;
     }
 }
+
+/* lam * sum_d D_d^T D_d p on a row-major [ny][nx] image with periodic first
+ * differences (the D of createDWithPeriodicBoundary.m, W = 1): the operator
+ * of DHWD2dCpu (kernels/DHWD2dCpu.cpp:96-325) in this image layout. */
+static void fd_cpu(float *s_r, float *s_i, const float *p_r, const float *p_i, int nx, int ny,
+                   float lam) {
+    for (int y = 0; y < ny; y++) {
+        for (int x = 0; x < nx; x++) {
+            const int i = y * nx + x;
+            const int xp = y * nx + (x + 1) % nx, xm = y * nx + (x + nx - 1) % nx;
+            const int yp = ((y + 1) % ny) * nx + x, ym = ((y + ny - 1) % ny) * nx + x;
+            s_r[i] = lam * (4.0f * p_r[i] - p_r[xp] - p_r[xm] - p_r[yp] - p_r[ym]);
+            s_i[i] = lam * (4.0f * p_i[i] - p_i[xp] - p_i[xm] - p_i[yp] - p_i[ym]);
+        }
+    }
+}
+
+static float dot_cpu(const float *a_r, const float *a_i, const float *b_r, const float *b_i,
+                     int n) {
+    /* dotProductCpu (kernels/dotProductCpu.cpp:84-113): real part, FP32 */
+    float s = 0.0f;
+    for (int i = 0; i < n; i++) s += a_r[i] * b_r[i] + a_i[i] * b_i[i];
+    return s;
+}
+
+/* bruteForceCpu's CG (bruteForce/bruteForceCpu.cpp:121-534, the loop at
+ * :280-420) with the finite-difference regulariser: cg_num - 1 iterations
+ * from x (in/out), grad = F^H (y - A f) - lam D^T D x, d = grad + gamma d,
+ * alpha = Re(d^H grad) / (|F d|^2 + Re((lam D^T D d)^H d)), A f += alpha F d,
+ * x += alpha d.  FP32 vectors and scalars (FLOAT_T = float). */
+void bruteforce_cpu(float *x_r, float *x_i, const float *y_r, const float *y_i,
+                    const float *kx, const float *ky, const float *ix, const float *iy,
+                    const float *fm, const float *t, const float *s_r, const float *s_i,
+                    int ncoils, int num_k, int nx, int ny, int cg_num, float lam) {
+    const int ni = nx * ny, nk = num_k * ncoils;
+    float *af_r = calloc(nk, sizeof(float)), *af_i = calloc(nk, sizeof(float));
+    float *q_r = calloc(nk, sizeof(float)), *q_i = calloc(nk, sizeof(float));
+    float *r_r = calloc(nk, sizeof(float)), *r_i = calloc(nk, sizeof(float));
+    float *g_r = calloc(ni, sizeof(float)), *g_i = calloc(ni, sizeof(float));
+    float *d_r = calloc(ni, sizeof(float)), *d_i = calloc(ni, sizeof(float));
+    float *c_r = calloc(ni, sizeof(float)), *c_i = calloc(ni, sizeof(float));
+    parallel_ft_cpu(af_r, af_i, x_r, x_i, kx, ky, ix, iy, fm, t, num_k, ni, s_r, s_i, ncoils);
+    float oldinprod = 0.0f;
+    for (int it = 1; it < cg_num; it++) {
+        for (int i = 0; i < nk; i++) {
+            r_r[i] = y_r[i] - af_r[i];
+            r_i[i] = y_i[i] - af_i[i];
+        }
+        parallel_ift_cpu(g_r, g_i, r_r, r_i, kx, ky, ix, iy, fm, t, num_k, ni, s_r, s_i, ncoils);
+        if (lam != 0.0f) {
+            fd_cpu(c_r, c_i, x_r, x_i, nx, ny, lam);
+            for (int i = 0; i < ni; i++) {
+                g_r[i] -= c_r[i];
+                g_i[i] -= c_i[i];
+            }
+        }
+        const float newinprod = dot_cpu(g_r, g_i, g_r, g_i, ni);
+        if (it == 1) {
+            memcpy(d_r, g_r, ni * sizeof(float));
+            memcpy(d_i, g_i, ni * sizeof(float));
+        } else {
+            const float gamma = oldinprod == 0.0f ? 0.0f : newinprod / oldinprod;
+            for (int i = 0; i < ni; i++) {
+                d_r[i] = g_r[i] + gamma * d_r[i];
+                d_i[i] = g_i[i] + gamma * d_i[i];
+            }
+        }
+        oldinprod = newinprod;
+        parallel_ft_cpu(q_r, q_i, d_r, d_i, kx, ky, ix, iy, fm, t, num_k, ni, s_r, s_i, ncoils);
+        float a_ctcd = 0.0f;
+        if (lam != 0.0f) {
+            fd_cpu(c_r, c_i, d_r, d_i, nx, ny, lam);
+            a_ctcd = dot_cpu(c_r, c_i, d_r, d_i, ni);
+        }
+        const float alpha = dot_cpu(d_r, d_i, g_r, g_i, ni) / (dot_cpu(q_r, q_i, q_r, q_i, nk) + a_ctcd);
+        for (int i = 0; i < nk; i++) {
+            af_r[i] += alpha * q_r[i];
+            af_i[i] += alpha * q_i[i];
+        }
+        for (int i = 0; i < ni; i++) {
+            x_r[i] += alpha * d_r[i];
+            x_i[i] += alpha * d_i[i];
+        }
+    }
+    free(af_r); free(af_i); free(q_r); free(q_i); free(r_r); free(r_i);
+    free(g_r); free(g_i); free(d_r); free(d_i); free(c_r); free(c_i);
+}

@@ -126,6 +126,34 @@ def test_c_restatement_agrees_with_fp64():
     assert rel_l2(ir + 1j * ii, ref2) < 1e-4
 
 
+def test_c_bruteforce_restatement_agrees_with_fp64():
+    """bruteForceCpu's CG restated in FP32 C (bruteforce_cpu: the timed CPU
+    brute-force baseline) follows the FP64 oracle CG (with the same finite
+    differences) on a 2-coil 12x12 problem: 5 iterations from zero."""
+    orc = _orc()
+    lib = _cpu_lib()
+    p = _tiny(N=12, ncoils=2)
+    N, nc, lam, cg = 12, 2, 5.0, 6
+    rng = np.random.default_rng(9)
+    y = (rng.standard_normal((nc, p.nk)) + 1j * rng.standard_normal((nc, p.nk))).astype(np.complex64)
+    F = lambda v: orc.nudft_forward(p.kx, p.ky, None, p.t, p.x, p.y, None, p.fm, v, p.sens)
+    FH = lambda d: orc.nudft_adjoint(p.kx, p.ky, None, p.t, p.x, p.y, None, p.fm,
+                                     np.asarray(d).reshape(nc, -1), p.sens)
+    R = lambda v: orc.fd_reg(np.asarray(v).reshape(N, N), lam).reshape(-1)
+    x0 = np.zeros(p.nvox, np.complex64)
+    ref = orc.bruteforce_cg(F, FH, R, y, x0, cg)
+    keep = [_fp(v) for v in (p.kx, p.ky, p.x, p.y, p.fm, p.t, y.real.reshape(-1),
+                            y.imag.reshape(-1), p.sens.real.reshape(-1), p.sens.imag.reshape(-1))]
+    xr = np.zeros(p.nvox, np.float32)
+    xi = np.zeros(p.nvox, np.float32)
+    P = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    lib.bruteforce_cpu(P(xr), P(xi), keep[6][1], keep[7][1], keep[0][1], keep[1][1], keep[2][1],
+                       keep[3][1], keep[4][1], keep[5][1], keep[8][1], keep[9][1], C.c_int(nc),
+                       C.c_int(p.nk), C.c_int(N), C.c_int(N), C.c_int(cg), C.c_float(lam))
+    err = rel_l2(xr + 1j * xi, ref)
+    assert err < 1e-3, err
+
+
 # ---------------------------------------------------------------------- Q
 def test_hanning_partition_of_unity():
     orc = _orc()
