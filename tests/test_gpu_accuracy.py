@@ -1,7 +1,6 @@
 """Error of each workload configuration against the FP64 oracle.
 
-What each configuration states (printed by every test; DESIGN.md collects
-the measured values):
+What each configuration states (every test prints its measured values):
 
 - Configuration 0 (64x64, 1 coil, L = 12): the whole pipeline in FP64
   (F^H d, Q_l, ten Toeplitz CG iterations, and the brute-force CG) against
