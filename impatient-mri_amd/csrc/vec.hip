@@ -203,8 +203,10 @@ __global__ __launch_bounds__(VT) void k_cg_alpha(const double *__restrict__ part
     if (threadIdx.x == 0) {
         sc[SC_PQ] = s;
         // As the reference (recon.cu:2071-2077), alpha = rho / p^H A p of either
-        // sign: the time-segmented operator is Hermitian but only approximately
-        // positive, and the best-iterate tracking keeps the result.  Only a zero
+        // sign: the operator is positive semidefinite in exact arithmetic, but
+        // its smallest eigenvalues (unsampled k-space, lambda-sized) sit below
+        // the FP32 rounding of Q_l, so p^H A p < 0 can occur at tight
+        // tolerances; the best-iterate tracking keeps the result.  Only a zero
         // or non-finite p^H A p stops the iteration.
         if (s == 0.0 || !isfinite(s)) {
             sc[SC_BREAK] = 1.0;
