@@ -46,7 +46,7 @@ MFMA_FLOPS_PER_PAIR = 24.0
 # rocprofv3 --pmc FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0), the factors a
 # 256 MiB device copy calibrates (profiles/traffic_512x512x1_32coil_pmc.md);
 # re-measure when the kernel changes
-NUDFT_TRAFFIC_BYTES = 2.0 * 3436286.6 * 1024 + 1.0 * 262053.1 * 1024
+NUDFT_TRAFFIC_BYTES = 2.0 * 3436252.2 * 1024 + 1.0 * 242554.8 * 1024
 
 
 def hip_events():
