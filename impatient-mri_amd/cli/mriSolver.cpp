@@ -52,7 +52,7 @@ void usage(FILE *f) {
                  "  -fd                  finite-difference regularizer (needs -fdp)\n"
                  "  -fdp <lambda>        regularization weight\n"
                  "  -tv / -tv_num <n>    total-variation outer iterations\n"
-                 "  -reg                 sparse-matrix regularizer (not supported)\n"
+                 "  -reg                 sparse-matrix regularizer C^T W C, C from <idir>/c.rmtx\n"
                  "  -cpu / -nogpu / -mgpu  CPU and multi-GPU paths (not part of this engine)\n"
                  "  -version, -help\n");
 }
@@ -327,12 +327,33 @@ int run_bruteforce(const Options &o) {
     const int ncoils = a[6].info.ncoils > 0 ? a[6].info.ncoils : 1;
     std::vector<float> xr(a[8].v.begin(), a[8].v.begin() + num_i),
         xi(a[9].v.begin(), a[9].v.begin() + num_i);
-    const float lambda = (o.fd || o.tv) ? o.fdp : 0.0f;
-    const int st = imp_bruteForceGpu(xr.data(), xi.data(), a[6].v.data(), a[7].v.data(),
-                                     a[0].v.data(), a[1].v.data(), a[2].v.data(), a[3].v.data(),
-                                     a[5].v.data(), a[4].v.data(), o.cg_num, num_k, num_i,
-                                     lambda, ncoils > 1 ? a[10].v.data() : nullptr,
-                                     ncoils > 1 ? a[11].v.data() : nullptr, ncoils);
+    const float lambda = (o.fd || o.tv || o.reg) ? o.fdp : 0.0f;
+    int st;
+    if (o.reg) {
+        // C from <idir>/c.rmtx (loadInputData, structures.cpp:598, 722-737)
+        const std::string fn = o.idir + "/c.rmtx";
+        imp_coo_matrix c{};
+        st = imp_read_mtx_size(fn.c_str(), &c.num_rows, &c.num_cols, &c.num_nonzeros);
+        if (report(st, "read c.rmtx")) return st;
+        std::vector<int> ri((size_t)c.num_nonzeros + 1), rj((size_t)c.num_nonzeros + 1);
+        std::vector<float> rv((size_t)c.num_nonzeros + 1);
+        st = imp_read_mtx(fn.c_str(), c.num_nonzeros, ri.data(), rj.data(), rv.data());
+        if (report(st, "read c.rmtx")) return st;
+        c.I = ri.data();
+        c.J = rj.data();
+        c.V = rv.data();
+        st = imp_bruteForceGpu_reg(xr.data(), xi.data(), a[6].v.data(), a[7].v.data(),
+                                   a[0].v.data(), a[1].v.data(), a[2].v.data(), a[3].v.data(),
+                                   a[5].v.data(), a[4].v.data(), o.cg_num, num_k, num_i, lambda,
+                                   ncoils > 1 ? a[10].v.data() : nullptr,
+                                   ncoils > 1 ? a[11].v.data() : nullptr, ncoils, &c);
+    } else {
+        st = imp_bruteForceGpu(xr.data(), xi.data(), a[6].v.data(), a[7].v.data(),
+                               a[0].v.data(), a[1].v.data(), a[2].v.data(), a[3].v.data(),
+                               a[5].v.data(), a[4].v.data(), o.cg_num, num_k, num_i, lambda,
+                               ncoils > 1 ? a[10].v.data() : nullptr,
+                               ncoils > 1 ? a[11].v.data() : nullptr, ncoils);
+    }
     if (report(st, "bruteForceGpu")) return st;
     mkdir(o.odir.c_str(), 0755);
     // output_gpu_{r,i}.dat, text "%f\n" (exportDataCpu, structures.cpp:1166-1175)
@@ -363,10 +384,6 @@ int main(int argc, char **argv) {
         std::fprintf(stderr, "mriSolver: the CPU path is not part of this engine; running on "
                              "the GPU only.\n");
     if (o.mgpu) std::fprintf(stderr, "mriSolver: -mgpu ignored (one GPU).\n");
-    if (o.reg) {
-        std::fprintf(stderr, "Error: -reg (sparse-matrix regularizer) is not supported\n");
-        return 1;
-    }
     imp_set_default_device(o.gpu_id);
     const int st = (o.toeplitz_direct || o.toeplitz_gridding) ? run_toeplitz(o) : run_bruteforce(o);
     if (st != IMP_OK) {

@@ -90,14 +90,20 @@ struct BF {
 int bruteforce_cg(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *kx,
                   const float *ky, const float *t, const float *x, const float *y,
                   const float *fm, const float *d_r, const float *d_i, const float *s_r,
-                  const float *s_i, int cg_num, float lambda, float *x_r, float *x_i) {
+                  const float *s_i, int cg_num, float lambda, const SparseReg *sreg,
+                  float *x_r, float *x_i) {
     if (!ctx || num_k <= 0 || num_x <= 0 || ncoils <= 0 || !kx || !ky || !t || !x || !y ||
         !d_r || !d_i || !x_r || !x_i || (s_r == nullptr) != (s_i == nullptr))
         return fail(IMP_EINVAL, "imp_bruteforce_cg: bad arguments");
     const int nimg = (int)std::lround(std::sqrt((double)num_x));
     const bool reg = lambda > 0.f;
-    if (reg && (long)nimg * nimg != num_x)
+    if (reg && !sreg && (long)nimg * nimg != num_x)
         return fail(IMP_EINVAL, "imp_bruteforce_cg: regularizer needs a square image (%d)", num_x);
+    // the sparse C of a 2D slice has two blocks of num_x rows (horizontal,
+    // vertical differences; bruteForceGpu.cu:379-381, 513-541)
+    if (sreg && (sreg->cols != num_x || sreg->rows != 2 * num_x))
+        return fail(IMP_EINVAL, "imp_bruteforce_cg: C is %d x %d, expected %d x %d", sreg->rows,
+                    sreg->cols, 2 * num_x, num_x);
     const long nk = (long)ncoils * num_k;
     const long nx = num_x;
     // buffers: y, Af, yAf, q  [2nk];  X, GtAf, grad, d, CtCf, CtCd [2nx];  cbuf [2nx float2]
@@ -127,7 +133,10 @@ int bruteforce_cg(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *k
         hipLaunchKernelGGL(k_split_pair_to_c, dim3(ceil_div(nx, 256)), dim3(256), 0, st, img,
                            nx, cb);
         IMP_CHECK_LAUNCH();
-        IMP_TRY(fd_reg(ctx, nimg, nimg, 1, lambda, cb, nullptr, cb2, false));
+        if (sreg)
+            IMP_TRY(sparse_reg(ctx, *sreg, lambda, cb, nullptr, cb2, false));
+        else
+            IMP_TRY(fd_reg(ctx, nimg, nimg, 1, lambda, cb, nullptr, cb2, false));
         hipLaunchKernelGGL(k_c_to_split_pair, dim3(ceil_div(nx, 256)), dim3(256), 0, st, cb2,
                            nx, out);
         IMP_CHECK_LAUNCH();
@@ -188,5 +197,16 @@ extern "C" int imp_bruteforce_cg(imp_ctx *ctx, int num_k, int num_x, int ncoils,
                                  const float *s_i, int cg_num, float lambda, float *x_r,
                                  float *x_i) {
     return imp::bruteforce_cg(ctx, num_k, num_x, ncoils, kx, ky, t, x, y, fm, d_r, d_i, s_r,
-                              s_i, cg_num, lambda, x_r, x_i);
+                              s_i, cg_num, lambda, nullptr, x_r, x_i);
+}
+
+extern "C" int imp_bruteforce_cg_reg(imp_ctx *ctx, int num_k, int num_x, int ncoils,
+                                     const float *kx, const float *ky, const float *t,
+                                     const float *x, const float *y, const float *fm,
+                                     const float *d_r, const float *d_i, const float *s_r,
+                                     const float *s_i, int cg_num, float lambda,
+                                     const imp_sparse_reg *reg, float *x_r, float *x_i) {
+    if (!reg) return imp::fail(IMP_EINVAL, "imp_bruteforce_cg_reg: NULL regularizer");
+    return imp::bruteforce_cg(ctx, num_k, num_x, ncoils, kx, ky, t, x, y, fm, d_r, d_i, s_r,
+                              s_i, cg_num, lambda, imp::sparse_reg_of(reg), x_r, x_i);
 }

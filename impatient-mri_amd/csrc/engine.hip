@@ -581,6 +581,19 @@ extern "C" int imp_toep_set_weight(imp_toep *op, const float *w) {
     return IMP_OK;
 }
 
+extern "C" int imp_toep_set_sparse_reg(imp_toep *op, const imp_sparse_reg *reg) {
+    if (!op) return fail(IMP_EINVAL, "imp_toep_set_sparse_reg: NULL operator");
+    const SparseReg *R = sparse_reg_of(reg);
+    if (R) {
+        const long ndim = op->P.N3 > 1 ? 3 : 2;
+        if ((long)R->cols != op->P.nvox || (long)R->rows != ndim * op->P.nvox)
+            return fail(IMP_EINVAL, "imp_toep_set_sparse_reg: C is %d x %d, expected %ld x %ld",
+                        R->rows, R->cols, ndim * op->P.nvox, op->P.nvox);
+    }
+    op->P.sreg = R;
+    return IMP_OK;
+}
+
 extern "C" int imp_toep_apply(imp_toep *op, const float *p_r, const float *p_i, float *out_r,
                               float *out_i) {
     if (!op || !p_r || !p_i || !out_r || !out_i)

@@ -154,7 +154,71 @@ int read_v10(FILE *f, const char *path, imp_file_info *info, float *data, int ca
     return IMP_OK;
 }
 
+// Matrix Market coordinate file (readMtxFile, kernels/mmio.cpp:668-830, with
+// mm_read_banner / mm_read_mtx_crd_size): a "%%MatrixMarket matrix
+// coordinate real general" banner (integer values are accepted too), '%'
+// comment lines, "rows cols nnz", then nnz lines "i j value", 1-based.
+// With cap < 0 only the sizes are read; otherwise up to cap entries are
+// stored 0-based.  Indices outside the stated size are an error here (the
+// reference checks them only in debug builds).
+int read_mtx(const char *path, int *rows, int *cols, int *nnz, int cap, int *I, int *J,
+             float *V) {
+    File f(path, "r");
+    if (!f.f) return fail(IMP_EIO, "%s not found", path);
+    std::string line;
+    if (!read_line(f.f, line)) return fail(IMP_EFORMAT, "%s: empty file", path);
+    std::string banner;
+    for (char ch : line) banner.push_back((char)std::tolower((unsigned char)ch));
+    char w[5][64] = {};
+    if (std::sscanf(banner.c_str(), "%63s %63s %63s %63s %63s", w[0], w[1], w[2], w[3], w[4]) != 5 ||
+        std::strcmp(w[0], "%%matrixmarket") != 0 || std::strcmp(w[1], "matrix") != 0)
+        return fail(IMP_EFORMAT, "%s: no Matrix Market banner", path);
+    if (std::strcmp(w[2], "coordinate") != 0 ||
+        (std::strcmp(w[3], "real") != 0 && std::strcmp(w[3], "integer") != 0) ||
+        std::strcmp(w[4], "general") != 0)
+        return fail(IMP_EFORMAT, "%s: Matrix Market type [%s %s %s] is not supported "
+                                 "(coordinate real general)", path, w[2], w[3], w[4]);
+    do {
+        if (!read_line(f.f, line)) return fail(IMP_EFORMAT, "%s: missing size line", path);
+        line = trim(line);
+    } while (line.empty() || line[0] == '%');
+    int r = 0, c = 0, n = 0;
+    if (std::sscanf(line.c_str(), "%d %d %d", &r, &c, &n) != 3 || r <= 0 || c <= 0 || n < 0)
+        return fail(IMP_EFORMAT, "%s: bad size line '%s'", path, line.c_str());
+    if (rows) *rows = r;
+    if (cols) *cols = c;
+    if (nnz) *nnz = n;
+    if (cap < 0) return IMP_OK;
+    if (cap < n) return fail(IMP_EINVAL, "%s: %d entries do not fit in %d", path, n, cap);
+    for (int e = 0; e < n; ++e) {
+        int ci = 0, cj = 0;
+        double v = 0.0;
+        if (std::fscanf(f.f, "%d %d %lg", &ci, &cj, &v) != 3)
+            return fail(IMP_EFORMAT, "%s: entry %d unreadable", path, e + 1);
+        if (ci < 1 || ci > r || cj < 1 || cj > c)
+            return fail(IMP_EFORMAT, "%s: entry %d (%d, %d) outside the %d x %d matrix", path,
+                        e + 1, ci, cj, r, c);
+        I[e] = ci - 1;
+        J[e] = cj - 1;
+        V[e] = (float)v;
+    }
+    return IMP_OK;
+}
+
 }  // namespace
+
+extern "C" int imp_read_mtx_size(const char *path, int *num_rows, int *num_cols,
+                                 int *num_nonzeros) {
+    if (!path || !num_rows || !num_cols || !num_nonzeros)
+        return fail(IMP_EINVAL, "imp_read_mtx_size: NULL argument");
+    return read_mtx(path, num_rows, num_cols, num_nonzeros, -1, nullptr, nullptr, nullptr);
+}
+
+extern "C" int imp_read_mtx(const char *path, int cap, int *I, int *J, float *V) {
+    if (!path || cap < 0 || (cap > 0 && (!I || !J || !V)))
+        return fail(IMP_EINVAL, "imp_read_mtx: bad arguments");
+    return read_mtx(path, nullptr, nullptr, nullptr, cap, I, J, V);
+}
 
 extern "C" int imp_read_data_file(const char *path, imp_file_info *info, float *data, int cap) {
     if (!path || !info) return fail(IMP_EINVAL, "imp_read_data_file: NULL argument");

@@ -7,6 +7,18 @@
 
 namespace imp {
 
+// A real sparse matrix C (rows x cols) as two resident CSR copies, C and
+// C^T, plus the work vector W C p [rows].
+struct SparseReg {
+    int rows = 0, cols = 0;
+    long nnz = 0;
+    int *c_ptr = nullptr, *c_idx = nullptr;
+    float *c_val = nullptr;
+    int *t_ptr = nullptr, *t_idx = nullptr;
+    float *t_val = nullptr;
+    float2 *cf = nullptr;
+};
+
 // Resident state of one Toeplitz normal operator A = sum_c S_c^H T S_c + lambda R.
 struct ToepPlan {
     int N1 = 0, N2 = 0, N3 = 1;   // image [N1][N2][N3] = [Ny][Nx][Nz]
@@ -25,6 +37,7 @@ struct ToepPlan {
     float2 *sens = nullptr;       // [ncoils][nvox] or null (S = 1)
     float *fcyc = nullptr;        // [nvox] field map in Hz (w / 2pi)
     float *w = nullptr;           // [nvox] TV weight or null (W = I), not owned
+    const SparseReg *sreg = nullptr;  // R = C^T W C instead of the stencil, not owned
     // CG work vectors (interleaved)
     float2 *vb = nullptr, *vx = nullptr, *vr = nullptr, *vp = nullptr,
            *vq = nullptr, *vbest = nullptr, *vz = nullptr;
@@ -47,6 +60,10 @@ int c_to_split(imp_ctx *ctx, const float2 *in, float *re, float *im, long n);
 int fd_reg(imp_ctx *ctx, int N1, int N2, int N3, float lambda, const float2 *p,
            const float *w, float2 *out, bool accumulate);
 int tv_weight(imp_ctx *ctx, int N1, int N2, int N3, const float2 *x, float *w);
+// out (+)= lambda C^T W C p, W[r mod cols] on row r of C (sparse.hip)
+int sparse_reg(imp_ctx *ctx, const SparseReg &R, float lambda, const float2 *p, const float *w,
+               float2 *out, bool accumulate);
+const SparseReg *sparse_reg_of(const imp_sparse_reg *reg);
 // Re(a^H b) in FP64, result to host (synchronous)
 int dot_re(imp_ctx *ctx, const float2 *a, const float2 *b, long n, double *out);
 int axpy_c(imp_ctx *ctx, float alpha, const float2 *x, float2 *y, long n);  // y += alpha x
@@ -81,7 +98,8 @@ int toeplitz_q(imp_ctx *ctx, int num_k, const float *kx, const float *ky,
 int bruteforce_cg(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *kx,
                   const float *ky, const float *t, const float *x, const float *y,
                   const float *fm, const float *d_r, const float *d_i, const float *s_r,
-                  const float *s_i, int cg_num, float lambda, float *x_r, float *x_i);
+                  const float *s_i, int cg_num, float lambda, const SparseReg *sreg,
+                  float *x_r, float *x_i);
 
 // internal CG flag: track the best iterate in ToepPlan::vbest but leave the
 // last iterate in x

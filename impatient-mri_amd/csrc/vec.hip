@@ -425,7 +425,12 @@ int axpy_c(imp_ctx *ctx, float alpha, const float2 *x, float2 *y, long n) {
 // out = A p = normal(p) + lambda R p
 int toep_apply_full(imp_ctx *ctx, ToepPlan &P, const float2 *p, float2 *out) {
     IMP_TRY(toep_apply_normal(ctx, P, p, out));
-    if (P.lambda != 0.f) IMP_TRY(fd_reg(ctx, P.N1, P.N2, P.N3, P.lambda, p, P.w, out, true));
+    if (P.lambda != 0.f) {
+        if (P.sreg)
+            IMP_TRY(sparse_reg(ctx, *P.sreg, P.lambda, p, P.w, out, true));
+        else
+            IMP_TRY(fd_reg(ctx, P.N1, P.N2, P.N3, P.lambda, p, P.w, out, true));
+    }
     return IMP_OK;
 }
 

@@ -262,13 +262,11 @@ extern "C" int imp_toeplitz_computeFH_GPU(const char *dir, float ntime_segs, int
 }
 
 extern "C" int imp_toeplitz_recon(int argc, char **argv, int enable_tv, int enable_reg,
-                                  int enable_fd, int enable_reuseQ, const void *coo,
+                                  int enable_fd, int enable_reuseQ, const imp_coo_matrix *coo,
                                   float *Fhd_R, float *Fhd_I, float **Q_R, float **Q_I) {
     (void)enable_fd;
     if (argc != 18 || !argv || !Fhd_R || !Fhd_I)
         return fail(IMP_EINVAL, "imp_toeplitz_recon: expected 18 arguments (recon.cu:1775)");
-    if (enable_reg || coo)
-        return fail(IMP_EUNSUPPORTED, "imp_toeplitz_recon: sparse-matrix regularizer (-reg)");
     const int N1 = std::atoi(argv[1]), N2 = std::atoi(argv[2]), N3 = std::atoi(argv[3]);
     const int numRestarts = std::atoi(argv[4]);
     const int numIterMax = std::atoi(argv[5]);
@@ -345,10 +343,49 @@ extern "C" int imp_toeplitz_recon(int argc, char **argv, int enable_tv, int enab
     IMP_TRY(dx.alloc(2 * nvox));
     IMP_HIP(hipMemset(dx.p, 0, 2 * nvox * sizeof(float)));
     IMP_TRY(dw.alloc(nvox));
+    // sparse-matrix regularizer: C from the caller, or from <dir>/c.rmtx as
+    // loadInputData reads it (structures.cpp:598, 722-737); one block of
+    // nvox rows per image axis (recon.cu:1590-1631)
+    imp_sparse_reg *reg = nullptr;
+    struct RegGuard {
+        imp_sparse_reg *&r;
+        ~RegGuard() { imp_sparse_reg_destroy(r); }
+    } reg_guard{reg};
+    if (enable_reg) {
+        imp_coo_matrix fc{};
+        std::vector<int> ri, rj;
+        std::vector<float> rv;
+        const imp_coo_matrix *c = coo;
+        if (!c) {
+            const std::string fn = dir + "/c.rmtx";
+            IMP_TRY(imp_read_mtx_size(fn.c_str(), &fc.num_rows, &fc.num_cols, &fc.num_nonzeros));
+            ri.resize((size_t)fc.num_nonzeros + 1);
+            rj.resize((size_t)fc.num_nonzeros + 1);
+            rv.resize((size_t)fc.num_nonzeros + 1);
+            IMP_TRY(imp_read_mtx(fn.c_str(), fc.num_nonzeros, ri.data(), rj.data(), rv.data()));
+            fc.I = ri.data();
+            fc.J = rj.data();
+            fc.V = rv.data();
+            c = &fc;
+        }
+        const long ndim = N3 > 1 ? 3 : 2;
+        if ((long)c->num_cols != nvox || (long)c->num_rows != ndim * nvox)
+            return fail(IMP_EINVAL, "imp_toeplitz_recon: C is %d x %d, expected %ld x %ld",
+                        c->num_rows, c->num_cols, ndim * nvox, nvox);
+        IMP_TRY(imp_sparse_reg_create(ctx, c->num_rows, c->num_cols, c->num_nonzeros, c->I, c->J,
+                                      c->V, &reg));
+    }
     imp_toep *op = nullptr;
     IMP_TRY(imp_toep_create(ctx, N1, N2, N3, nc, L, tau, t0, dq.p, dq.p + (size_t)(L + 1) * G,
                             dfm.p, sense ? dsr.p : nullptr, sense ? dsi.p : nullptr, lambda2,
                             &op));
+    if (reg) {
+        const int rs = imp_toep_set_sparse_reg(op, reg);
+        if (rs != IMP_OK) {
+            imp_toep_destroy(op);
+            return rs;
+        }
+    }
     // x stays the last iterate across TV passes; reconBest is the output
     // (recon.cu:2013-2126, 2150-2162)
     std::vector<float> best(2 * nvox, 0.f);
@@ -390,16 +427,25 @@ extern "C" int imp_toeplitz_recon(int argc, char **argv, int enable_tv, int enab
     return write_raw(outfile, {best.data(), best.data() + nvox}, nvox);
 }
 
-extern "C" int imp_bruteForceGpu(float *x_r, float *x_i, const float *y_r, const float *y_i,
-                                 const float *kx, const float *ky, const float *ix,
-                                 const float *iy, const float *fm, const float *t, int cg_num,
-                                 int num_k, int num_i, float fd_penalizer, const float *s_r,
-                                 const float *s_i, int ncoils) {
+namespace {
+// imp_bruteForceGpu with an optional sparse C (host COO)
+int bforce_host(float *x_r, float *x_i, const float *y_r, const float *y_i, const float *kx,
+                const float *ky, const float *ix, const float *iy, const float *fm,
+                const float *t, int cg_num, int num_k, int num_i, float fd_penalizer,
+                const float *s_r, const float *s_i, int ncoils, const imp_coo_matrix *c) {
     if (!x_r || !x_i || !y_r || !y_i || !kx || !ky || !ix || !iy || !t || num_k <= 0 ||
         num_i <= 0 || ncoils <= 0 || (s_r == nullptr) != (s_i == nullptr))
         return fail(IMP_EINVAL, "imp_bruteForceGpu: bad arguments");
     imp_ctx *ctx = nullptr;
     IMP_TRY(default_ctx(&ctx));
+    imp_sparse_reg *reg = nullptr;
+    if (c)
+        IMP_TRY(imp_sparse_reg_create(ctx, c->num_rows, c->num_cols, c->num_nonzeros, c->I, c->J,
+                                      c->V, &reg));
+    struct Guard {
+        imp_sparse_reg *r;
+        ~Guard() { imp_sparse_reg_destroy(r); }
+    } guard{reg};
     const size_t nk = (size_t)num_k * ncoils;
     Dev a[12];
     IMP_TRY(a[0].up(kx, num_k));
@@ -418,8 +464,29 @@ extern "C" int imp_bruteForceGpu(float *x_r, float *x_i, const float *y_r, const
     IMP_TRY(a[11].up(x_i, num_i));
     IMP_TRY(bruteforce_cg(ctx, num_k, num_i, ncoils, a[0].p, a[1].p, a[2].p, a[3].p, a[4].p,
                           a[5].p, a[6].p, a[7].p, a[8].p, a[9].p, cg_num, fd_penalizer,
-                          a[10].p, a[11].p));
+                          sparse_reg_of(reg), a[10].p, a[11].p));
     IMP_HIP(hipMemcpy(x_r, a[10].p, num_i * sizeof(float), hipMemcpyDeviceToHost));
     IMP_HIP(hipMemcpy(x_i, a[11].p, num_i * sizeof(float), hipMemcpyDeviceToHost));
     return IMP_OK;
+}
+}  // namespace
+
+extern "C" int imp_bruteForceGpu(float *x_r, float *x_i, const float *y_r, const float *y_i,
+                                 const float *kx, const float *ky, const float *ix,
+                                 const float *iy, const float *fm, const float *t, int cg_num,
+                                 int num_k, int num_i, float fd_penalizer, const float *s_r,
+                                 const float *s_i, int ncoils) {
+    return bforce_host(x_r, x_i, y_r, y_i, kx, ky, ix, iy, fm, t, cg_num, num_k, num_i,
+                       fd_penalizer, s_r, s_i, ncoils, nullptr);
+}
+
+extern "C" int imp_bruteForceGpu_reg(float *x_r, float *x_i, const float *y_r,
+                                     const float *y_i, const float *kx, const float *ky,
+                                     const float *ix, const float *iy, const float *fm,
+                                     const float *t, int cg_num, int num_k, int num_i,
+                                     float fd_penalizer, const float *s_r, const float *s_i,
+                                     int ncoils, const imp_coo_matrix *c) {
+    if (!c) return fail(IMP_EINVAL, "imp_bruteForceGpu_reg: NULL matrix");
+    return bforce_host(x_r, x_i, y_r, y_i, kx, ky, ix, iy, fm, t, cg_num, num_k, num_i,
+                       fd_penalizer, s_r, s_i, ncoils, c);
 }

@@ -218,6 +218,12 @@ class Toeplitz:
         return _cplx(*self._out), st.as_dict()
 
 
+    def set_sparse_reg(self, reg=None):
+        """Use lambda C^T W C of a SparseReg as R (None: the FD stencil).
+        The operator keeps a reference, so reg stays alive while in use."""
+        call("imp_toep_set_sparse_reg", self.h, reg.h if reg is not None else None)
+        self._reg = reg
+
     def precond_weights(self):
         """Current circulant preconditioner weights (imp_toep_get_precond),
         shaped like the image, natural DFT order."""
@@ -242,8 +248,60 @@ class Toeplitz:
         return _cplx(*self._out)
 
 
-def bruteforce_cg(ctx: Context, kx, ky, t, x, y, fm, d, sens, cg_num, lam, x0):
-    """bruteForceGpu's CG from x0 (cg_num - 1 iterations); d is [C, M]."""
+def read_mtx(path):
+    """Matrix Market coordinate file (imp_read_mtx_size / imp_read_mtx, host
+    only): (num_rows, num_cols, I, J, V) with 0-based int32 indices."""
+    r, c, n = C.c_int(), C.c_int(), C.c_int()
+    p = str(path).encode()
+    call("imp_read_mtx_size", p, C.byref(r), C.byref(c), C.byref(n))
+    I = np.zeros(max(n.value, 1), np.int32)
+    J = np.zeros(max(n.value, 1), np.int32)
+    V = np.zeros(max(n.value, 1), np.float32)
+    call("imp_read_mtx", p, C.c_int(n.value), I.ctypes.data_as(C.c_void_p),
+         J.ctypes.data_as(C.c_void_p), V.ctypes.data_as(C.c_void_p))
+    return r.value, c.value, I[:n.value], J[:n.value], V[:n.value]
+
+
+class SparseReg:
+    """Sparse-matrix regularizer lambda C^T W C (imp_sparse_reg_*), C given
+    as 0-based COO (rows, cols, I, J, V)."""
+
+    def __init__(self, ctx: Context, num_rows, num_cols, I, J, V):
+        self.ctx = ctx
+        self.rows, self.cols = int(num_rows), int(num_cols)
+        I = np.ascontiguousarray(I, np.int32)
+        J = np.ascontiguousarray(J, np.int32)
+        V = np.ascontiguousarray(V, np.float32)
+        h = C.c_void_p()
+        call("imp_sparse_reg_create", ctx.h, C.c_int(self.rows), C.c_int(self.cols),
+             C.c_int(len(V)), I.ctypes.data_as(C.c_void_p), J.ctypes.data_as(C.c_void_p),
+             V.ctypes.data_as(C.c_void_p), C.byref(h))
+        self.h = h
+
+    def apply(self, p, lam, w=None):
+        p = np.asarray(p)
+        pr, pi = _split(self.ctx, p.reshape(-1))
+        aw = _opt(self.ctx, None if w is None else np.asarray(w).reshape(-1))
+        sr, si = _pair(self.ctx, self.cols)
+        call("imp_sparse_reg_apply", self.h, C.c_float(lam), ptr(aw), ptr(pr), ptr(pi),
+             ptr(sr), ptr(si))
+        return _cplx(sr, si, p.shape)
+
+    def close(self):
+        if getattr(self, "h", None):
+            call("imp_sparse_reg_destroy", self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def bruteforce_cg(ctx: Context, kx, ky, t, x, y, fm, d, sens, cg_num, lam, x0, reg=None):
+    """bruteForceGpu's CG from x0 (cg_num - 1 iterations); d is [C, M].
+    reg: a SparseReg for R = C^T C (the -reg branch) instead of the stencil."""
     d = np.atleast_2d(np.asarray(d))
     ncoils, nk = d.shape
     nx = len(x)
@@ -254,9 +312,13 @@ def bruteforce_cg(ctx: Context, kx, ky, t, x, y, fm, d, sens, cg_num, lam, x0):
     if sens is not None:
         sr, si = _split(ctx, np.asarray(sens).reshape(-1))
     xr, xi = _split(ctx, np.asarray(x0).reshape(-1))
-    call("imp_bruteforce_cg", ctx.h, C.c_int(nk), C.c_int(nx), C.c_int(ncoils), ptr(a[0]),
-         ptr(a[1]), ptr(a[2]), ptr(a[3]), ptr(a[4]), ptr(afm), ptr(dr), ptr(di), ptr(sr),
-         ptr(si), C.c_int(cg_num), C.c_float(lam), ptr(xr), ptr(xi))
+    head = (ctx.h, C.c_int(nk), C.c_int(nx), C.c_int(ncoils), ptr(a[0]), ptr(a[1]),
+            ptr(a[2]), ptr(a[3]), ptr(a[4]), ptr(afm), ptr(dr), ptr(di), ptr(sr), ptr(si),
+            C.c_int(cg_num), C.c_float(lam))
+    if reg is None:
+        call("imp_bruteforce_cg", *head, ptr(xr), ptr(xi))
+    else:
+        call("imp_bruteforce_cg_reg", *head, reg.h, ptr(xr), ptr(xi))
     return _cplx(xr, xi)
 
 

@@ -192,6 +192,13 @@ int imp_toep_create(imp_ctx *ctx, int N1, int N2, int N3, int ncoils,
 int imp_toep_destroy(imp_toep *op);
 /* Replace the TV weight W of lambda R (device [N], NULL = identity). */
 int imp_toep_set_weight(imp_toep *op, const float *w);
+/* Use lambda C^T W C (a sparse-matrix regularizer, below) as R instead of
+ * the finite-difference stencil: the -reg branch of Computeq
+ * (toeplitz/recon.cu:1543-1640).  C must be (ndim N) x N, ndim = 2 for
+ * N3 = 1 and 3 otherwise; reg is not copied and must outlive its use by
+ * the operator.  NULL restores the stencil. */
+struct imp_sparse_reg;
+int imp_toep_set_sparse_reg(imp_toep *op, const struct imp_sparse_reg *reg);
 /* Pass-E schedule: coils summed per workgroup (more coils per group = fewer
  * partial images for pass F, fewer workgroups).  imp_toep_create picks one
  * that fills the GPU; set accepts [that value, ncoils] (the autotuner's
@@ -281,6 +288,28 @@ int imp_cnn_gradient(imp_cnn *net, const float *z_r, const float *z_i, const flo
 int imp_cnn_train_step(imp_cnn *net, const float *z_r, const float *z_i, const float *t_r,
                        const float *t_i, float lr, double *loss);
 
+/* ------------------------------------------ sparse-matrix regularizer
+ * R = C^T W C for a real sparse C (the -reg option and its Matrix Market
+ * file c.rmtx).  The reference converts C and C^T to CSR and applies
+ * Cf = C p, Cf[block b] *= W, lambda C^T Cf each iteration
+ * (toeplitz/recon.cu:1543-1640, bruteForce/bruteForceGpu.cu:309-360,
+ * 513-541, 579-609); here both CSR copies stay resident on the device.
+ * W (device [num_cols], NULL = identity) multiplies row r of C by
+ * W[r mod num_cols], i.e. every block of num_cols rows. */
+typedef struct imp_sparse_reg imp_sparse_reg;
+/* Matrix Market "coordinate real general" reader (readMtxFile,
+ * kernels/mmio.cpp:668-830): sizes first, then up to cap entries, 0-based.
+ * Host only; no GPU is touched. */
+int imp_read_mtx_size(const char *path, int *num_rows, int *num_cols, int *num_nonzeros);
+int imp_read_mtx(const char *path, int cap, int *I, int *J, float *V);
+/* C from host COO arrays (0-based; duplicates add; every index is checked). */
+int imp_sparse_reg_create(imp_ctx *ctx, int num_rows, int num_cols, int num_nonzeros,
+                          const int *I, const int *J, const float *V, imp_sparse_reg **out);
+int imp_sparse_reg_destroy(imp_sparse_reg *reg);
+/* out = lambda C^T W C p (device pointers, [num_cols] each). */
+int imp_sparse_reg_apply(imp_sparse_reg *reg, float lambda, const float *w,
+                         const float *p_r, const float *p_i, float *out_r, float *out_i);
+
 /* ---------------------------------------------- brute-force CG (a11)
  * bruteForceGpu semantics (bruteForce/bruteForceGpu.cu:142-813): Fessler CG
  * on (F^H F + lambda R) x = F^H y starting from x, cg_num - 1 iterations,
@@ -294,6 +323,16 @@ int imp_bruteforce_cg(imp_ctx *ctx, int num_k, int num_x, int ncoils,
                       const float *s_r, const float *s_i,
                       int cg_num, float lambda,
                       float *x_r, float *x_i);
+/* The same with R = C^T C from a sparse-matrix regularizer (the brute
+ * force's -reg branch, bruteForceGpu.cu:513-541, 579-629); C is
+ * (2 num_x) x num_x. */
+int imp_bruteforce_cg_reg(imp_ctx *ctx, int num_k, int num_x, int ncoils,
+                          const float *kx, const float *ky, const float *t,
+                          const float *x, const float *y, const float *fm,
+                          const float *d_r, const float *d_i,
+                          const float *s_r, const float *s_i,
+                          int cg_num, float lambda, const imp_sparse_reg *reg,
+                          float *x_r, float *x_i);
 
 /* ------------------------------------------------------------ data files
  * Reference file formats (kernels/structures.cpp:718-1110,
@@ -344,25 +383,45 @@ int imp_toeplitz_computeFH_GPU(const char *dir, float ntime_segs, int Nx,
                                float *outR, float *outI, int enable_direct,
                                int enable_gridding, float gridOS);
 
+/* Sparse matrix in coordinate form, 0-based, HOST arrays: the reference's
+ * CooMatrix (kernels/structures.h:927-960) that loadInputData fills from
+ * c.rmtx (structures.cpp:598, 722-737). */
+typedef struct {
+    int num_rows, num_cols, num_nonzeros;
+    const int *I;      /* row indices    [num_nonzeros] */
+    const int *J;      /* column indices [num_nonzeros] */
+    const float *V;    /* values         [num_nonzeros] */
+} imp_coo_matrix;
+
 /* toeplitz_recon (toeplitz/recon.cuh:55): argv[1..17] = N1 N2 N3
  * numRestarts numIterMax symTraj lambda2 D Dp R W Qfile FhDfile dir tv_num
- * ntime_segs outfile (main_mri.cu:578-598).  reg (sparse-matrix
- * regularizer) is IMP_EUNSUPPORTED; coo must be NULL. */
+ * ntime_segs outfile (main_mri.cu:578-598).  enable_reg: R = C^T W C with
+ * C from coo, or from <dir>/c.rmtx when coo is NULL (C is (ndim N) x N);
+ * otherwise the finite-difference stencil. */
 int imp_toeplitz_recon(int argc, char **argv, int enable_tv, int enable_reg,
-                       int enable_fd, int enable_reuseQ, const void *coo,
+                       int enable_fd, int enable_reuseQ, const imp_coo_matrix *coo,
                        float *Fhd_R, float *Fhd_I, float **Q_R, float **Q_I);
 
 /* bruteForceGpu for one slice (bruteForce/gpuPrototypes.cuh:90-105, def
  * bruteForceGpu.cu:143), host pointers: x_r/x_i [num_i] in/out (initial
  * estimate in, image out); y_r/y_i [ncoils*num_k]; kx,ky,t [num_k];
  * ix,iy,fm [num_i]; s_r/s_i [ncoils*num_i] or NULL (S = 1).  fd_penalizer
- * > 0 enables the finite-difference term (the -fd/-fdp options); the
- * sparse-matrix and TV variants are not offered.  No power-of-two padding. */
+ * > 0 enables the finite-difference term (the -fd/-fdp options); the TV
+ * variant is not offered.  No power-of-two padding. */
 int imp_bruteForceGpu(float *x_r, float *x_i, const float *y_r, const float *y_i,
                       const float *kx, const float *ky, const float *ix,
                       const float *iy, const float *fm, const float *t,
                       int cg_num, int num_k, int num_i, float fd_penalizer,
                       const float *s_r, const float *s_i, int ncoils);
+/* The same with the sparse-matrix regularizer fd_penalizer C^T C in place of
+ * the stencil (bruteForceGpu's enable_regularization and CooMatrix *c,
+ * bruteForceGpu.cu:150, 309-360); c is (2 num_i) x num_i, host arrays. */
+int imp_bruteForceGpu_reg(float *x_r, float *x_i, const float *y_r, const float *y_i,
+                          const float *kx, const float *ky, const float *ix,
+                          const float *iy, const float *fm, const float *t,
+                          int cg_num, int num_k, int num_i, float fd_penalizer,
+                          const float *s_r, const float *s_i, int ncoils,
+                          const imp_coo_matrix *c);
 
 #ifdef __cplusplus
 }

@@ -227,6 +227,25 @@ def fd_reg(p, lam, w=None):
     return lam * s
 
 
+def sparse_reg(p, lam, num_rows, num_cols, I, J, V, w=None):
+    """s = lam * C^T W C p for a real sparse C (num_rows x num_cols, 0-based
+    COO; duplicate entries add, as coo2Csr keeps them, structures.cpp:380-392).
+    The -reg branch of Computeq (toeplitz/recon.cu:1543-1640) and of the
+    brute-force CG (bruteForce/bruteForceGpu.cu:513-541, 579-609):
+    Cf = C p (smvmGpu), each block of num_cols rows of Cf times W
+    (pointMultGpu with the imaginary part zero), C^T Cf (smvmGpu on the
+    transposed CSR), times lam (addGpu with alpha = lambda2)."""
+    import scipy.sparse as sp
+    p = np.asarray(p, np.complex128)
+    Cm = sp.coo_matrix((np.asarray(V, np.float64), (np.asarray(I), np.asarray(J))),
+                       shape=(num_rows, num_cols)).tocsr()
+    cf = Cm @ p.reshape(-1)
+    if w is not None:
+        wv = np.asarray(w, np.float64).reshape(-1)
+        cf = cf * wv[np.arange(num_rows) % num_cols]
+    return (lam * (Cm.T @ cf)).reshape(p.shape)
+
+
 def tv_weight(x):
     """w = 0.5 / sqrt(sum_d |D_d x|^2 + 1e-6): multiply2dGpu / multiply3dGpu
     (toeplitz/multiplyGpu_toeplitz.cu:134-172) on Dhori / Dverti / Dzaxis."""

@@ -28,3 +28,22 @@ def rel_l2(a, b):
     a = np.asarray(a, np.complex128).reshape(-1)
     b = np.asarray(b, np.complex128).reshape(-1)
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def difference_coo(shape):
+    """[D_0; D_1; ...]: one block of periodic first differences
+    (Dp)(i) = p(i) - p(i + e_b) per image axis b, 0-based COO."""
+    import numpy as np
+    import scipy.sparse as sp
+    n = int(np.prod(shape))
+    blocks = []
+    for b, na in enumerate(shape):
+        D = sp.identity(na, format="csr") - sp.csr_matrix(
+            (np.ones(na), (np.arange(na), (np.arange(na) + 1) % na)), shape=(na, na))
+        K = None
+        for a, s in enumerate(shape):
+            m = D if a == b else sp.identity(s, format="csr")
+            K = m if K is None else sp.kron(K, m, format="csr")
+        blocks.append(K)
+    C = sp.vstack(blocks).tocoo()
+    return C.shape[0], n, C.row.astype(np.int32), C.col.astype(np.int32), C.data

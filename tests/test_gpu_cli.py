@@ -8,7 +8,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from conftest import GOLDEN, REPO, rel_l2
+from conftest import GOLDEN, REPO, difference_coo, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -79,3 +79,42 @@ def test_cli_bruteforce(data_dir, ctx):
     # text output keeps 6 decimals ("%f", structures.cpp:1170-1173)
     assert np.max(np.abs(xr - ref.real)) <= 5e-7 + 1e-6 * np.abs(ref.real).max()
     assert np.max(np.abs(xi - ref.imag)) <= 5e-7 + 1e-6 * np.abs(ref.imag).max()
+
+
+def _write_difference_rmtx(d, shape):
+    rows, cols, I, J, V = difference_coo(shape)
+    with open(d / "c.rmtx", "w") as f:
+        f.write("%%MatrixMarket matrix coordinate real general\n")
+        f.write(f"{rows} {cols} {len(V)}\n")
+        for i, j, v in zip(I, J, V):
+            f.write(f"{i + 1} {j + 1} {v:g}\n")
+
+
+def test_cli_sparse_reg_with_difference_matrix(tmp_path, data_dir):
+    """-reg with c.rmtx = the periodic difference matrix gives the -fd image,
+    on both drivers (the two regularizers are the same operator)."""
+    other = tmp_path / "fd"
+    shutil.copytree(data_dir, other)
+    _write_difference_rmtx(data_dir, (64, 64))
+    common = ["-toeplitzDirect", "-ntime_segs", "3", "-cg_num", "10", "-fdp", "0.001"]
+    r1 = subprocess.run([str(CLI), "-idir", str(data_dir), *common, "-reg"],
+                        capture_output=True, text=True, timeout=120)
+    r2 = subprocess.run([str(CLI), "-idir", str(other), *common, "-fd"],
+                        capture_output=True, text=True, timeout=120)
+    print(r1.stdout[-1500:], r1.stderr[-1500:], r2.stderr[-500:])
+    assert r1.returncode == 0 and r2.returncode == 0
+    a = np.fromfile(data_dir / "out.file", dtype=np.float32)
+    b = np.fromfile(other / "out.file", dtype=np.float32)
+    err_t = rel_l2(a, b)
+    common = ["-cg_num", "6", "-fdp", "0.01"]
+    r1 = subprocess.run([str(CLI), "-idir", str(data_dir), "-odir", str(data_dir / "o"),
+                         *common, "-reg"], capture_output=True, text=True, timeout=120)
+    r2 = subprocess.run([str(CLI), "-idir", str(other), "-odir", str(other / "o"), *common,
+                         "-fd"], capture_output=True, text=True, timeout=120)
+    print(r1.stderr[-1500:], r2.stderr[-500:])
+    assert r1.returncode == 0 and r2.returncode == 0
+    ga = np.loadtxt(data_dir / "o" / "output_gpu_r.dat")
+    gb = np.loadtxt(other / "o" / "output_gpu_r.dat")
+    err_b = rel_l2(ga, gb)
+    print(f"CLI -reg (C = D) vs -fd: toeplitz {err_t:.2e}, brute force {err_b:.2e}")
+    assert err_t < 1e-4 and err_b < 1e-4

@@ -148,7 +148,8 @@ def test_file_level_wrappers_reject_bad_arguments():
     argv18 = (C.c_char_p * 18)(*([b"1"] * 18))
     x = np.zeros(1, np.float32)
     P = x.ctypes.data_as(C.c_void_p)
-    assert lib.imp_toeplitz_recon(18, argv18, 0, 1, 0, 0, None, P, P, None, None) == -8
+    # -reg reads its inputs (here a missing directory) before any device work
+    assert lib.imp_toeplitz_recon(18, argv18, 0, 1, 0, 0, None, P, P, None, None) == -3
     assert lib.imp_toeplitz_computeQ_GPU(b"/nonexistent", C.c_float(2), 8, 8, 1, None,
                                          None, None, 1, 0, C.c_float(1.0), 0) == -1
     assert lib.imp_toeplitz_computeFH_GPU(b"/nonexistent", C.c_float(2), 8, 8, 1, None,
@@ -179,3 +180,53 @@ def test_synthetic_workload_layout():
     assert np.allclose(ss, 1.0, atol=1e-5)
     assert p1.z[1] - p1.z[0] == pytest.approx(1 / 4)        # z fastest in 3D
     assert sorted(set(p1.kz.tolist())) == [-2, -1, 0, 1]
+
+
+def _write_mtx(path, rows, cols, I, J, V, banner="%%MatrixMarket matrix coordinate real general"):
+    with open(path, "w") as f:
+        f.write(banner + "\n% a comment line\n%\n")
+        f.write(f"{rows} {cols} {len(V)}\n")
+        for i, j, v in zip(I, J, V):
+            f.write(f"{i + 1} {j + 1} {v:.17g}\n")
+
+
+def test_read_mtx(tmp_path):
+    """Matrix Market coordinate files through the C ABI (host only): sizes,
+    1-based to 0-based indices, values rounded to float32 as readMtxFile
+    stores them (kernels/mmio.cpp:760-770); scipy reads the same file."""
+    import scipy.io
+    from impatient import ops
+    rng = np.random.default_rng(3)
+    rows, cols, nnz = 12, 7, 30
+    I = rng.integers(0, rows, nnz)
+    J = rng.integers(0, cols, nnz)
+    V = rng.standard_normal(nnz)
+    f = tmp_path / "c.rmtx"
+    _write_mtx(f, rows, cols, I, J, V)
+    r, c, gi, gj, gv = ops.read_mtx(f)
+    assert (r, c) == (rows, cols)
+    assert np.array_equal(gi, I) and np.array_equal(gj, J)
+    assert np.array_equal(gv, V.astype(np.float32))
+    m = scipy.io.mmread(str(f)).tocoo()
+    assert m.shape == (rows, cols) and np.allclose(m.toarray().sum(), V.sum())
+
+
+def test_read_mtx_rejects_unsupported_and_bad_files(tmp_path):
+    from impatient import _lib, ops
+    cases = {
+        "complex.rmtx": ("%%MatrixMarket matrix coordinate complex general", "3 3 1\n1 1 1 0\n"),
+        "pattern.rmtx": ("%%MatrixMarket matrix coordinate pattern general", "3 3 1\n1 1\n"),
+        "array.rmtx": ("%%MatrixMarket matrix array real general", "3 3\n"),
+        "range.rmtx": ("%%MatrixMarket matrix coordinate real general", "3 3 1\n4 1 1.0\n"),
+        "short.rmtx": ("%%MatrixMarket matrix coordinate real general", "3 3 2\n1 1 1.0\n"),
+        "nobanner.rmtx": ("3 3 1", "1 1 1.0\n"),
+    }
+    for name, (banner, body) in cases.items():
+        f = tmp_path / name
+        f.write_text(banner + "\n" + body)
+        with pytest.raises(_lib.ImpError) as e:
+            ops.read_mtx(f)
+        assert e.value.status in (-5, -1), name
+    with pytest.raises(_lib.ImpError) as e:
+        ops.read_mtx(tmp_path / "missing.rmtx")
+    assert e.value.status == -3

@@ -290,6 +290,56 @@ def test_fd_reg_equals_explicit_matrix(shape):
                   0.7 * R @ p) < 1e-13
 
 
+def _difference_coo(shape):
+    """C = [D_0; D_1; ...] (one block of periodic first differences per image
+    axis, the matrix createDWithPeriodicBoundary.m writes as c.rmtx) as
+    0-based COO."""
+    n = int(np.prod(shape))
+    I, J, V = [], [], []
+    for b, na in enumerate(shape):
+        mats = [np.eye(s) for s in shape]
+        mats[b] = _periodic_D(na)
+        K = mats[0]
+        for m_ in mats[1:]:
+            K = np.kron(K, m_)
+        r, c = np.nonzero(K)
+        I.append(r + b * n)
+        J.append(c)
+        V.append(K[r, c])
+    return len(shape) * n, n, np.concatenate(I), np.concatenate(J), np.concatenate(V)
+
+
+@pytest.mark.parametrize("shape", [(6, 5), (4, 3, 5)])
+def test_sparse_reg_with_difference_matrix_equals_fd_reg(shape):
+    """-reg with C = the periodic difference matrix is the -fd stencil."""
+    orc = _orc()
+    rng = np.random.default_rng(8)
+    rows, cols, I, J, V = _difference_coo(shape)
+    w = rng.uniform(0.2, 2.0, shape)
+    p = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    got = orc.sparse_reg(p, 0.7, rows, cols, I, J, V, w)
+    assert rel_l2(got, orc.fd_reg(p, 0.7, w)) < 1e-13
+
+
+def test_sparse_reg_dense_brute_force():
+    """Random C with duplicate entries: lam C^T diag([w; w]) C p densely."""
+    orc = _orc()
+    rng = np.random.default_rng(9)
+    n, rows, nnz = 20, 40, 90
+    I = rng.integers(0, rows, nnz)
+    J = rng.integers(0, n, nnz)
+    I[:5], J[:5] = I[5:10], J[5:10]          # duplicates add
+    V = rng.standard_normal(nnz)
+    Cd = np.zeros((rows, n))
+    np.add.at(Cd, (I, J), V)
+    w = rng.uniform(0.5, 1.5, n)
+    p = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    ref = 0.3 * Cd.T @ (np.concatenate([w, w]) * (Cd @ p))
+    assert rel_l2(orc.sparse_reg(p, 0.3, rows, n, I, J, V, w), ref) < 1e-13
+    assert rel_l2(orc.sparse_reg(p, 0.3, rows, n, I, J, V),
+                  0.3 * Cd.T @ (Cd @ p)) < 1e-13
+
+
 def test_fd_reg_plane_wave_eigenvalue():
     orc = _orc()
     N1, N2 = 8, 12
