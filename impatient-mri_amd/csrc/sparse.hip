@@ -93,12 +93,16 @@ void build_csr(int rows, long nnz, const int *row, const int *col, const float *
 
 int sparse_reg(imp_ctx *ctx, const SparseReg &R, float lambda, const float2 *p, const float *w,
                float2 *out, bool accumulate) {
+    int rec = prof_begin(ctx, "k_csr_wc");
     hipLaunchKernelGGL(k_csr_wc, dim3(ceil_div(R.rows, 256)), dim3(256), 0, ctx->stream, R.rows,
                        (long)R.cols, R.c_ptr, R.c_idx, R.c_val, p, w, R.cf);
     IMP_CHECK_LAUNCH();
+    prof_end(ctx, rec);
+    rec = prof_begin(ctx, "k_csr_t");
     hipLaunchKernelGGL(k_csr_t, dim3(ceil_div(R.cols, 256)), dim3(256), 0, ctx->stream, R.cols,
                        R.t_ptr, R.t_idx, R.t_val, R.cf, lambda, out, accumulate ? 1 : 0);
     IMP_CHECK_LAUNCH();
+    prof_end(ctx, rec);
     return IMP_OK;
 }
 

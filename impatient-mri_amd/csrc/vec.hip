@@ -386,9 +386,11 @@ int c_to_split(imp_ctx *ctx, const float2 *in, float *re, float *im, long n) {
 int fd_reg(imp_ctx *ctx, int N1, int N2, int N3, float lambda, const float2 *p, const float *w,
            float2 *out, bool accumulate) {
     const long n = (long)N1 * N2 * N3;
+    const int rec = prof_begin(ctx, "k_fd_reg");
     hipLaunchKernelGGL(k_fd_reg, dim3(ceil_div(n, 256)), dim3(256), 0, ctx->stream, N1, N2, N3,
                        lambda, p, w, out, accumulate ? 1 : 0);
     IMP_CHECK_LAUNCH();
+    prof_end(ctx, rec);
     return IMP_OK;
 }
 

@@ -16,6 +16,13 @@ once (complex64 = 8 bytes) on the pruned 2x grid:
             line of the 2x grid, plus Qf_l (8 x 2^d per voxel)
     pass D  (3D) reads 2N2 x values, keeps N2                      (32C, 16C)
     pass E  reads the G1 lines and S, keeps N along the last axis
+
+With --sparse-reg the report also times lambda C^T W C with C the periodic
+difference matrix of the image (the -reg option's c.rmtx, 2 or 3 blocks of
+N rows, 2 nonzeros per row) against the finite-difference stencil that
+computes the same operator; algorithmic bytes per voxel and block: the two
+CSR copies (4 + 4 x 2 + 4 x 2 bytes per row of C and of C^T), W, p and the
+work vector Cf (8 bytes, written then read), and out (read and written).
 """
 from __future__ import annotations
 
@@ -48,11 +55,62 @@ def pass_bytes(nv, ncoils, d3):
             "toep_pass_e": 16 * c * nv + 8 * c * nv}
 
 
+def difference_matrix(shape):
+    """[D_0; D_1; ...] with (D_b p)(i) = p(i) - p(i + e_b) periodic, COO."""
+    import scipy.sparse as sp
+    blocks = []
+    for b, na in enumerate(shape):
+        D = sp.identity(na, format="csr") - sp.csr_matrix(
+            (np.ones(na), (np.arange(na), (np.arange(na) + 1) % na)), shape=(na, na))
+        K = None
+        for a, s in enumerate(shape):
+            m = D if a == b else sp.identity(s, format="csr")
+            K = m if K is None else sp.kron(K, m, format="csr")
+        blocks.append(K)
+    Cm = sp.vstack(blocks).tocoo()
+    return Cm.shape[0], Cm.shape[1], Cm.row, Cm.col, Cm.data
+
+
+def sparse_reg_timing(ctx, p, timer_on, kernel_ms, log):
+    """lambda C^T W C p (C = difference matrix) vs the stencil, on the device."""
+    shape = (p.Ny, p.Nx, p.Nz) if p.Nz > 1 else (p.Ny, p.Nx)
+    rows, cols, I, J, V = difference_matrix(shape)
+    R = ops.SparseReg(ctx, rows, cols, I, J, V)
+    rng = np.random.default_rng(1)
+    v = (rng.standard_normal(cols) + 1j * rng.standard_normal(cols)).astype(np.complex64)
+    w = rng.uniform(0.5, 1.5, cols).astype(np.float32)
+    a = R.apply(v, 1e-3, w)
+    ref = ops.fd_reg(ctx, v.reshape(shape), 1e-3, w.reshape(shape)).reshape(-1)
+    diff = float(np.linalg.norm(a - ref) / np.linalg.norm(ref))
+    timer_on()
+    for _ in range(5):
+        R.apply(v, 1e-3, w)
+        ops.fd_reg(ctx, v.reshape(shape), 1e-3, w.reshape(shape))
+    res = {"rows": rows, "cols": cols, "nnz": int(len(V)), "rel_diff_vs_stencil": diff}
+    nnz, nd = len(V), rows // cols
+    # CSR of C (rows+1 pointers, nnz idx + val), Cf written, p gathered, W
+    b_wc = 4 * (rows + 1) + 8 * nnz + 8 * rows + 8 * cols + 4 * cols
+    # CSR of C^T, Cf gathered, out read and written
+    b_t = 4 * (cols + 1) + 8 * nnz + 8 * rows + 16 * cols
+    b_st = 8 * cols + 4 * cols + 16 * cols   # p, W, out read and written
+    for name, nbytes in (("k_csr_wc", b_wc), ("k_csr_t", b_t), ("k_fd_reg", b_st)):
+        tot, n = kernel_ms(name)
+        avg = tot / max(n, 1)
+        res[name] = {"avg_ms": round(avg, 4), "algorithmic_bytes": nbytes,
+                     "gbs": round(nbytes / (avg * 1e-3) / 1e9, 1) if avg > 0 else None}
+        log(f"{name}: {n} launches, {avg:.4f} ms, {res[name]['gbs']} GB/s")
+    log(f"sparse reg ({nd} blocks, {nnz} nonzeros) vs stencil: rel diff {diff:.2e}")
+    R.close()
+    return res
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--cg-max-iter", type=int, default=1000)
     ap.add_argument("--tol", type=float, default=1e-5)
+    ap.add_argument("--sparse-reg", action="store_true",
+                    help="also time the sparse-matrix regularizer against the stencil")
     args = ap.parse_args(argv)
 
     def log(m):
@@ -132,6 +190,9 @@ def main(argv=None):
                         "gbs": round(nbytes / (avg * 1e-3) / 1e9, 1) if avg > 0 else None}
         log(f"{name}: {n} launches, {avg:.4f} ms, {passes[name]['gbs']} GB/s")
     call("imp_profile_enable", ctx.h, C.c_int(0))
+    sreg = None
+    if args.sparse_reg:
+        sreg = sparse_reg_timing(ctx, p, timer_on, kernel_ms, log)
     base = IMP_CG_BEST_ITERATE | IMP_CG_TRUE_RESIDUAL
     solves = {}
     for name, flags in (("plain", base), ("chan", base | IMP_CG_PRECOND)):
@@ -149,6 +210,8 @@ def main(argv=None):
            "fhd_gsample_vox_s": round(pairs / fhd_ms / 1e6, 1),
            "q_s": round(t_q, 4), "toeplitz_plan_s": round(t_plan, 4),
            "toeplitz_passes": passes, "tol": args.tol, "solves": solves}
+    if sreg is not None:
+        out["sparse_reg"] = sreg
     print(json.dumps(out), flush=True)
     ctx.close()
     return 0
