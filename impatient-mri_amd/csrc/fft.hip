@@ -94,23 +94,31 @@ __global__ __launch_bounds__(ROW_THREADS) void k_fft_rows(float2 *data, long nli
 // workgroup, thread = (line-local index, line) with the line fastest, so each
 // global access covers P consecutive complex values.  LDS holds [i][line]
 // with a row stride of P + 1.
-template <int L>
+//   Single transforms (MODE 0 / 1): P = 16, a whole 128-byte row segment per
+//   access, while the workgroup stays within 1024 threads and 150 KB of LDS
+//   (measured at 512-point lines: 25% faster than P = 8).
+//   FFT x Qf x IFFT (MODE 2): P = 8 (or fewer for long lines), so two
+//   workgroups share a CU (at 1024-point lines P = 16 leaves one
+//   1024-thread workgroup per CU and measured 27% slower).
+template <int L, int MODE>
 constexpr int col_lines() {
-    return fftc::threads_per_line(L) * 8 <= 512 ? (fftc::threads_per_line(L) * 16 <= 256 ? 16 : 8)
-                                                : 512 / fftc::threads_per_line(L);
+    constexpr int T = fftc::threads_per_line(L);
+    return (MODE != 2 && T * 16 <= 1024 && (long)L * 17 * 8 <= 150 * 1024)
+               ? 16
+               : (T * 8 <= 512 ? (T * 16 <= 256 ? 16 : 8) : 512 / T);
 }
-template <int L>
-constexpr int col_threads() { return col_lines<L>() * fftc::threads_per_line(L); }
+template <int L, int MODE>
+constexpr int col_threads() { return col_lines<L, MODE>() * fftc::threads_per_line(L); }
 
 // MODE 0: forward, 1: inverse, 2: forward, times qf (same addressing with
 // batch stride 0), inverse.  HALF_IN: inputs i >= L/2 are zero (the
 // zero-padded axis of the 2x grid); HALF_OUT: only outputs i < L/2 are
 // stored (the cropped axis).  blockIdx.y selects the batch (bstride).
 template <int L, int MODE, bool HALF_IN, bool HALF_OUT>
-__global__ __launch_bounds__(col_threads<L>()) void k_fft_cols(
+__global__ __launch_bounds__((col_threads<L, MODE>())) void k_fft_cols(
     float2 *data, long bstride, long es, int nlines, const float2 *__restrict__ qf,
     const float2 *__restrict__ tw) {
-    constexpr int P = col_lines<L>(), PS = P + 1;
+    constexpr int P = col_lines<L, MODE>(), PS = P + 1;
     constexpr int E = fftc::vals_per_thread(L);
     __shared__ float2 buf[fftc::num_stages(L) > 1 ? L * PS : 1];
     const int c = threadIdx.x % P, tl = threadIdx.x / P;
@@ -121,17 +129,25 @@ __global__ __launch_bounds__(col_threads<L>()) void k_fft_cols(
     const int panel = (gridDim.x & 15) == 0 ? (bx & ~15) | ((bx & 7) << 1) | ((bx >> 3) & 1) : bx;
     const int j = panel * P + c;
     const bool ok = j < nlines;
-    float2 *base = data + (long)blockIdx.y * bstride + (ok ? j : 0);
+    // element i = tl + T m of the line sits at col[m * step]: the register
+    // index m is a compile-time constant and step is uniform, so each access
+    // is one scalar multiply and one vector add (no per-element 64-bit
+    // multiply of i * es)
+    constexpr int T = fftc::threads_per_line(L);
+    const long step = (long)T * es;
+    float2 *col = data + (long)blockIdx.y * bstride + (ok ? j : 0) + (long)tl * es;
     auto lld = [&](int, int, int i) { return buf[i * PS + c]; };
     auto lst = [&](int, int, int i, float2 v) { buf[i * PS + c] = v; };
-    // element i = tl + T m: i < L/2 exactly when m < E/2 (known per register)
-    auto gld = [&](int b, int r, int i) {
-        const bool in = !HALF_IN || fftc::first_reg<L>(b, r) < E / 2;
-        return (ok && in) ? base[(long)i * es] : make_float2(0.f, 0.f);
+    // i < L/2 exactly when m < E/2 (known per register)
+    auto gld = [&](int b, int r, int) {
+        const int m = fftc::first_reg<L>(b, r);
+        const bool in = !HALF_IN || m < E / 2;
+        return (ok && in) ? col[m * step] : make_float2(0.f, 0.f);
     };
-    auto gst = [&](int b, int r, int i, float2 v) {
-        const bool in = !HALF_OUT || fftc::last_reg<L>(b, r) < E / 2;
-        if (ok && in) base[(long)i * es] = v;
+    auto gst = [&](int b, int r, int, float2 v) {
+        const int m = fftc::last_reg<L>(b, r);
+        const bool in = !HALF_OUT || m < E / 2;
+        if (ok && in) col[m * step] = v;
     };
     if constexpr (MODE == 0) {
         run_fft<L, false>(tl, tw, gld, [] {}, gst, lld, lst);
@@ -139,11 +155,12 @@ __global__ __launch_bounds__(col_threads<L>()) void k_fft_cols(
         run_fft<L, true>(tl, tw, gld, [] {}, gst, lld, lst);
     } else {
         float2 regs[E];
-        const float2 *q = qf + (ok ? j : 0);
+        const float2 *q = qf + (ok ? j : 0) + (long)tl * es;
         run_fft<L, false>(
             tl, tw, gld, [] {},
-            [&](int b, int r, int i, float2 v) {
-                regs[fftc::last_reg<L>(b, r)] = ok ? cmul(v, q[(long)i * es]) : v;
+            [&](int b, int r, int, float2 v) {
+                const int m = fftc::last_reg<L>(b, r);
+                regs[m] = ok ? cmul(v, q[m * step]) : v;
             },
             lld, lst);
         run_fft<L, true>(
@@ -367,14 +384,14 @@ int fft_grid(imp_ctx *ctx, float2 *data, int M1, int M2, int M3, bool inverse) {
     if (d3) {  // x axis: batch y, lines z
         IMP_TRY(twiddles(ctx, M2, &tw));
         IMP_L_SWITCH(M2, {
-            dim3 grid(ceil_div(M3, col_lines<L_>()), M1);
+            dim3 grid(ceil_div(M3, col_lines<L_, 0>()), M1);
             if (inverse)
                 hipLaunchKernelGGL((k_fft_cols<L_, 1, false, false>), grid,
-                                   dim3(col_threads<L_>()), 0, ctx->stream, data, (long)M2 * M3,
+                                   dim3(col_threads<L_, 1>()), 0, ctx->stream, data, (long)M2 * M3,
                                    (long)M3, M3, nullptr, tw);
             else
                 hipLaunchKernelGGL((k_fft_cols<L_, 0, false, false>), grid,
-                                   dim3(col_threads<L_>()), 0, ctx->stream, data, (long)M2 * M3,
+                                   dim3(col_threads<L_, 0>()), 0, ctx->stream, data, (long)M2 * M3,
                                    (long)M3, M3, nullptr, tw);
         });
         IMP_CHECK_LAUNCH();
@@ -383,12 +400,12 @@ int fft_grid(imp_ctx *ctx, float2 *data, int M1, int M2, int M3, bool inverse) {
     IMP_TRY(twiddles(ctx, M1, &tw));
     const int nlines = M2 * M3;
     IMP_L_SWITCH(M1, {
-        dim3 grid(ceil_div(nlines, col_lines<L_>()), 1);
+        dim3 grid(ceil_div(nlines, col_lines<L_, 0>()), 1);
         if (inverse)
-            hipLaunchKernelGGL((k_fft_cols<L_, 1, false, false>), grid, dim3(col_threads<L_>()),
+            hipLaunchKernelGGL((k_fft_cols<L_, 1, false, false>), grid, dim3(col_threads<L_, 1>()),
                                0, ctx->stream, data, 0L, (long)nlines, nlines, nullptr, tw);
         else
-            hipLaunchKernelGGL((k_fft_cols<L_, 0, false, false>), grid, dim3(col_threads<L_>()),
+            hipLaunchKernelGGL((k_fft_cols<L_, 0, false, false>), grid, dim3(col_threads<L_, 0>()),
                                0, ctx->stream, data, 0L, (long)nlines, nlines, nullptr, tw);
     });
     IMP_CHECK_LAUNCH();
@@ -443,8 +460,8 @@ int toep_apply_normal(imp_ctx *ctx, ToepPlan &P, const float2 *p, float2 *out) {
         if (d3) {
             rec = prof_begin(ctx, "toep_pass_b");
             IMP_L_SWITCH(P.M2, {
-                dim3 grid(ceil_div(P.M3, col_lines<L_>()), C * P.N1);
-                hipLaunchKernelGGL((k_fft_cols<L_, 0, true, false>), grid, dim3(col_threads<L_>()),
+                dim3 grid(ceil_div(P.M3, col_lines<L_, 0>()), C * P.N1);
+                hipLaunchKernelGGL((k_fft_cols<L_, 0, true, false>), grid, dim3(col_threads<L_, 0>()),
                                    0, ctx->stream, P.g1, (long)P.M2 * P.M3, (long)P.M3, P.M3,
                                    nullptr, tw_x);
             });
@@ -457,8 +474,8 @@ int toep_apply_normal(imp_ctx *ctx, ToepPlan &P, const float2 *p, float2 *out) {
             const float2 *qf = P.qf + (long)l * P.G;
             rec = prof_begin(ctx, "toep_pass_c");
             IMP_L_SWITCH(P.M1, {
-                dim3 grid(ceil_div(nl, col_lines<L_>()), C);
-                hipLaunchKernelGGL((k_fft_cols<L_, 2, true, true>), grid, dim3(col_threads<L_>()),
+                dim3 grid(ceil_div(nl, col_lines<L_, 2>()), C);
+                hipLaunchKernelGGL((k_fft_cols<L_, 2, true, true>), grid, dim3(col_threads<L_, 2>()),
                                    0, ctx->stream, P.g1, (long)P.N1 * nl, (long)nl, nl, qf, tw_y);
             });
             prof_end(ctx, rec);
@@ -468,8 +485,8 @@ int toep_apply_normal(imp_ctx *ctx, ToepPlan &P, const float2 *p, float2 *out) {
         if (d3) {
             rec = prof_begin(ctx, "toep_pass_d");
             IMP_L_SWITCH(P.M2, {
-                dim3 grid(ceil_div(P.M3, col_lines<L_>()), C * P.N1);
-                hipLaunchKernelGGL((k_fft_cols<L_, 1, false, true>), grid, dim3(col_threads<L_>()),
+                dim3 grid(ceil_div(P.M3, col_lines<L_, 1>()), C * P.N1);
+                hipLaunchKernelGGL((k_fft_cols<L_, 1, false, true>), grid, dim3(col_threads<L_, 1>()),
                                    0, ctx->stream, P.g1, (long)P.M2 * P.M3, (long)P.M3, P.M3,
                                    nullptr, tw_x);
             });
