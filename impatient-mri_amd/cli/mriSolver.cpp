@@ -325,34 +325,56 @@ int run_bruteforce(const Options &o) {
         if (load(o.idir + "/" + names[i] + ".dat", a[i])) return IMP_EIO;
     const int num_k = a[0].info.per_coil, num_i = a[2].info.per_coil;
     const int ncoils = a[6].info.ncoils > 0 ? a[6].info.ncoils : 1;
-    std::vector<float> xr(a[8].v.begin(), a[8].v.begin() + num_i),
-        xi(a[9].v.begin(), a[9].v.begin() + num_i);
+    // slices: one brute-force solve each, one after another on this GPU
+    // (main_mri.cu:889-905); slice s starts at s*num_i in idata and fm and at
+    // s*num_k in kdata (the reference's offset, which does not scale with
+    // the coil count)
+    const int nslices = a[6].info.nslices > 1 ? a[6].info.nslices : 1;
+    const size_t need_x = (size_t)nslices * num_i, need_k = (size_t)(nslices - 1) * num_k +
+                                                            (size_t)ncoils * num_k;
+    if (a[8].v.size() < need_x || a[9].v.size() < need_x || a[5].v.size() < need_x ||
+        a[6].v.size() < need_k || a[7].v.size() < need_k) {
+        std::fprintf(stderr, "Error: %d slices need %zu image and %zu k-space values\n", nslices,
+                     need_x, need_k);
+        return IMP_EFORMAT;
+    }
+    std::vector<float> xr(a[8].v.begin(), a[8].v.begin() + need_x),
+        xi(a[9].v.begin(), a[9].v.begin() + need_x);
     const float lambda = (o.fd || o.tv || o.reg) ? o.fdp : 0.0f;
-    int st;
+    int st = IMP_OK;
+    imp_coo_matrix c{};
+    std::vector<int> ri, rj;
+    std::vector<float> rv;
     if (o.reg) {
         // C from <idir>/c.rmtx (loadInputData, structures.cpp:598, 722-737)
         const std::string fn = o.idir + "/c.rmtx";
-        imp_coo_matrix c{};
         st = imp_read_mtx_size(fn.c_str(), &c.num_rows, &c.num_cols, &c.num_nonzeros);
         if (report(st, "read c.rmtx")) return st;
-        std::vector<int> ri((size_t)c.num_nonzeros + 1), rj((size_t)c.num_nonzeros + 1);
-        std::vector<float> rv((size_t)c.num_nonzeros + 1);
+        ri.resize((size_t)c.num_nonzeros + 1);
+        rj.resize((size_t)c.num_nonzeros + 1);
+        rv.resize((size_t)c.num_nonzeros + 1);
         st = imp_read_mtx(fn.c_str(), c.num_nonzeros, ri.data(), rj.data(), rv.data());
         if (report(st, "read c.rmtx")) return st;
         c.I = ri.data();
         c.J = rj.data();
         c.V = rv.data();
-        st = imp_bruteForceGpu_reg(xr.data(), xi.data(), a[6].v.data(), a[7].v.data(),
-                                   a[0].v.data(), a[1].v.data(), a[2].v.data(), a[3].v.data(),
-                                   a[5].v.data(), a[4].v.data(), o.cg_num, num_k, num_i, lambda,
-                                   ncoils > 1 ? a[10].v.data() : nullptr,
-                                   ncoils > 1 ? a[11].v.data() : nullptr, ncoils, &c);
-    } else {
-        st = imp_bruteForceGpu(xr.data(), xi.data(), a[6].v.data(), a[7].v.data(),
-                               a[0].v.data(), a[1].v.data(), a[2].v.data(), a[3].v.data(),
-                               a[5].v.data(), a[4].v.data(), o.cg_num, num_k, num_i, lambda,
-                               ncoils > 1 ? a[10].v.data() : nullptr,
-                               ncoils > 1 ? a[11].v.data() : nullptr, ncoils);
+    }
+    for (int sl = 0; sl < nslices && st == IMP_OK; ++sl) {
+        if (nslices > 1) std::printf("  GPU: Processing slice %d.\n", sl);
+        float *x_r = xr.data() + (size_t)sl * num_i, *x_i = xi.data() + (size_t)sl * num_i;
+        const float *y_r = a[6].v.data() + (size_t)sl * num_k,
+                    *y_i = a[7].v.data() + (size_t)sl * num_k;
+        const float *fm = a[5].v.data() + (size_t)sl * num_i;
+        const float *s_r = ncoils > 1 ? a[10].v.data() : nullptr,
+                    *s_i = ncoils > 1 ? a[11].v.data() : nullptr;
+        if (o.reg)
+            st = imp_bruteForceGpu_reg(x_r, x_i, y_r, y_i, a[0].v.data(), a[1].v.data(),
+                                       a[2].v.data(), a[3].v.data(), fm, a[4].v.data(), o.cg_num,
+                                       num_k, num_i, lambda, s_r, s_i, ncoils, &c);
+        else
+            st = imp_bruteForceGpu(x_r, x_i, y_r, y_i, a[0].v.data(), a[1].v.data(),
+                                   a[2].v.data(), a[3].v.data(), fm, a[4].v.data(), o.cg_num,
+                                   num_k, num_i, lambda, s_r, s_i, ncoils);
     }
     if (report(st, "bruteForceGpu")) return st;
     mkdir(o.odir.c_str(), 0755);
@@ -365,7 +387,8 @@ int run_bruteforce(const Options &o) {
         if (f2) std::fclose(f2);
         return IMP_EIO;
     }
-    for (int i = 0; i < num_i; ++i) {
+    // every slice, num_slices * num_i values (main_mri.cu:917-918)
+    for (size_t i = 0; i < need_x; ++i) {
         std::fprintf(f1, "%f\n", xr[i]);
         std::fprintf(f2, "%f\n", xi[i]);
     }

@@ -118,3 +118,37 @@ def test_cli_sparse_reg_with_difference_matrix(tmp_path, data_dir):
     err_b = rel_l2(ga, gb)
     print(f"CLI -reg (C = D) vs -fd: toeplitz {err_t:.2e}, brute force {err_b:.2e}")
     assert err_t < 1e-4 and err_b < 1e-4
+
+
+def test_cli_bruteforce_multislice(data_dir, ctx):
+    """Two slices in one data set (main_mri.cu:889-918): each slice is solved
+    from its own k-space data, initial image and field map, and the output
+    files hold both slices, slice 0 first."""
+    from impatient import files, ops
+    rd = lambda n: files.read(data_dir / f"{n}.dat")
+    k_r, k_i, x_r, x_i, fm = rd("kdata_r"), rd("kdata_i"), rd("idata_r"), rd("idata_i"), rd("fm")
+    nk, ni = k_r.size, x_r.size
+    rng = np.random.default_rng(4)
+    k2_r = (0.5 * k_r + 0.01 * rng.standard_normal(nk)).astype(np.float32)
+    k2_i = (0.5 * k_i).astype(np.float32)
+    fm2 = (0.7 * fm).astype(np.float32)
+    for name, a, b in (("kdata_r", k_r, k2_r), ("kdata_i", k_i, k2_i)):
+        files.write_v02(data_dir / f"{name}.dat", np.concatenate([a, b]), per_coil=nk, nslices=2)
+    for name, a, b in (("idata_r", x_r, 0 * x_r), ("idata_i", x_i, 0 * x_i), ("fm", fm, fm2)):
+        files.write_v02(data_dir / f"{name}.dat", np.concatenate([a, b]), per_coil=ni, nslices=2)
+    out = data_dir / "out"
+    r = subprocess.run([str(CLI), "-idir", str(data_dir), "-odir", str(out), "-cg_num", "6",
+                        "-fd", "-fdp", "0.01"], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-1500:], r.stderr[-1500:])
+    assert r.returncode == 0
+    xr = np.loadtxt(out / "output_gpu_r.dat")
+    xi = np.loadtxt(out / "output_gpu_i.dat")
+    assert xr.size == 2 * ni
+    for s, (kr, ki, x0, f) in enumerate(((k_r, k_i, x_r + 1j * x_i, fm),
+                                         (k2_r, k2_i, np.zeros(ni, np.complex64), fm2))):
+        ref = ops.bruteforce_cg(ctx, rd("kx"), rd("ky"), rd("t"), rd("ix"), rd("iy"), f,
+                                (kr + 1j * ki)[None, :], None, 6, 0.01, x0)
+        got = xr[s * ni:(s + 1) * ni] + 1j * xi[s * ni:(s + 1) * ni]
+        err = np.max(np.abs(got - ref))
+        print(f"slice {s}: max |CLI - C ABI| = {err:.2e}")
+        assert err <= 5e-7 + 1e-6 * np.abs(ref).max() * 1.5
