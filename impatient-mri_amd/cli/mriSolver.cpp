@@ -367,14 +367,11 @@ int run_bruteforce(const Options &o) {
         const float *fm = a[5].v.data() + (size_t)sl * num_i;
         const float *s_r = ncoils > 1 ? a[10].v.data() : nullptr,
                     *s_i = ncoils > 1 ? a[11].v.data() : nullptr;
-        if (o.reg)
-            st = imp_bruteForceGpu_reg(x_r, x_i, y_r, y_i, a[0].v.data(), a[1].v.data(),
-                                       a[2].v.data(), a[3].v.data(), fm, a[4].v.data(), o.cg_num,
-                                       num_k, num_i, lambda, s_r, s_i, ncoils, &c);
-        else
-            st = imp_bruteForceGpu(x_r, x_i, y_r, y_i, a[0].v.data(), a[1].v.data(),
-                                   a[2].v.data(), a[3].v.data(), fm, a[4].v.data(), o.cg_num,
-                                   num_k, num_i, lambda, s_r, s_i, ncoils);
+        // TV passes as bruteForceGpu runs them (bruteForceGpu.cu:431-757)
+        st = imp_bruteForceGpu_ex(x_r, x_i, y_r, y_i, a[0].v.data(), a[1].v.data(),
+                                  a[2].v.data(), a[3].v.data(), fm, a[4].v.data(), o.cg_num,
+                                  num_k, num_i, lambda, s_r, s_i, ncoils, o.reg ? &c : nullptr,
+                                  o.tv ? 1 : 0, o.tv_num);
     }
     if (report(st, "bruteForceGpu")) return st;
     mkdir(o.odir.c_str(), 0755);

@@ -91,13 +91,14 @@ int bruteforce_cg(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *k
                   const float *ky, const float *t, const float *x, const float *y,
                   const float *fm, const float *d_r, const float *d_i, const float *s_r,
                   const float *s_i, int cg_num, float lambda, const SparseReg *sreg,
-                  float *x_r, float *x_i) {
+                  int tv_num, float *x_r, float *x_i) {
     if (!ctx || num_k <= 0 || num_x <= 0 || ncoils <= 0 || !kx || !ky || !t || !x || !y ||
         !d_r || !d_i || !x_r || !x_i || (s_r == nullptr) != (s_i == nullptr))
         return fail(IMP_EINVAL, "imp_bruteforce_cg: bad arguments");
     const int nimg = (int)std::lround(std::sqrt((double)num_x));
     const bool reg = lambda > 0.f;
-    if (reg && !sreg && (long)nimg * nimg != num_x)
+    const int passes = tv_num > 1 ? tv_num : 1;
+    if (reg && (!sreg || passes > 1) && (long)nimg * nimg != num_x)
         return fail(IMP_EINVAL, "imp_bruteforce_cg: regularizer needs a square image (%d)", num_x);
     // the sparse C of a 2D slice has two blocks of num_x rows (horizontal,
     // vertical differences; bruteForceGpu.cu:379-381, 513-541)
@@ -106,14 +107,18 @@ int bruteforce_cg(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *k
                     sreg->cols, 2 * num_x, num_x);
     const long nk = (long)ncoils * num_k;
     const long nx = num_x;
-    // buffers: y, Af, yAf, q  [2nk];  X, GtAf, grad, d, CtCf, CtCd [2nx];  cbuf [2nx float2]
-    const size_t bytes = (4 * 2 * nk + 6 * 2 * nx) * sizeof(float) + 2 * nx * sizeof(float2);
+    // buffers: y, Af, yAf, q  [2nk];  X, GtAf, grad, d, CtCf, CtCd [2nx];  cbuf [2nx float2];
+    // TV weight [nx]
+    const size_t bytes =
+        (4 * 2 * nk + 6 * 2 * nx) * sizeof(float) + 2 * nx * sizeof(float2) + nx * sizeof(float);
     void *ws = nullptr;
     IMP_TRY(workspace(ctx, WS_BFORCE, bytes, &ws));
     float *yv = (float *)ws, *Af = yv + 2 * nk, *yAf = Af + 2 * nk, *q = yAf + 2 * nk;
     float *X = q + 2 * nk, *GtAf = X + 2 * nx, *grad = GtAf + 2 * nx, *dv = grad + 2 * nx,
           *CtCf = dv + 2 * nx, *CtCd = CtCf + 2 * nx;
     float2 *cb = (float2 *)(CtCd + 2 * nx), *cb2 = cb + nx;
+    float *wtv = (float *)(cb2 + nx);
+    const float *W = nullptr;  // W = 1 until a TV pass sets it
     hipStream_t st = ctx->stream;
     IMP_HIP(hipMemcpyAsync(yv, d_r, nk * sizeof(float), hipMemcpyDeviceToDevice, st));
     IMP_HIP(hipMemcpyAsync(yv + nk, d_i, nk * sizeof(float), hipMemcpyDeviceToDevice, st));
@@ -134,15 +139,26 @@ int bruteforce_cg(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *k
                            nx, cb);
         IMP_CHECK_LAUNCH();
         if (sreg)
-            IMP_TRY(sparse_reg(ctx, *sreg, lambda, cb, nullptr, cb2, false));
+            IMP_TRY(sparse_reg(ctx, *sreg, lambda, cb, W, cb2, false));
         else
-            IMP_TRY(fd_reg(ctx, nimg, nimg, 1, lambda, cb, nullptr, cb2, false));
+            IMP_TRY(fd_reg(ctx, nimg, nimg, 1, lambda, cb, W, cb2, false));
         hipLaunchKernelGGL(k_c_to_split_pair, dim3(ceil_div(nx, 256)), dim3(256), 0, st, cb2,
                            nx, out);
         IMP_CHECK_LAUNCH();
         return IMP_OK;
     };
 
+    // TV passes (bruteForceGpu.cu:431-757): pass 0 with W = 1, each later
+    // pass with W = 0.5 / sqrt(|D_h x|^2 + |D_v x|^2 + 1e-6) of the current
+    // image, and the CG restarted from it; one pass without TV
+    for (int tv_i = 0; tv_i < passes; ++tv_i) {
+    if (tv_i > 0 && reg) {
+        hipLaunchKernelGGL(k_split_pair_to_c, dim3(ceil_div(nx, 256)), dim3(256), 0, st, X, nx,
+                           cb);
+        IMP_CHECK_LAUNCH();
+        IMP_TRY(tv_weight(ctx, nimg, nimg, 1, cb, wtv));
+        W = wtv;
+    }
     IMP_TRY(F(X, Af));
     double oldinprod = 0.0;
     for (int cg_i = 1; cg_i < cg_num; ++cg_i) {
@@ -182,6 +198,7 @@ int bruteforce_cg(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *k
         IMP_TRY(bf.axpby(1.f, Af, alpha, q, Af, 2 * nk));
         IMP_TRY(bf.axpby(1.f, X, alpha, dv, X, 2 * nx));
     }
+    }
     IMP_HIP(hipMemcpyAsync(x_r, X, nx * sizeof(float), hipMemcpyDeviceToDevice, st));
     IMP_HIP(hipMemcpyAsync(x_i, X + nx, nx * sizeof(float), hipMemcpyDeviceToDevice, st));
     IMP_HIP(hipStreamSynchronize(st));
@@ -197,16 +214,16 @@ extern "C" int imp_bruteforce_cg(imp_ctx *ctx, int num_k, int num_x, int ncoils,
                                  const float *s_i, int cg_num, float lambda, float *x_r,
                                  float *x_i) {
     return imp::bruteforce_cg(ctx, num_k, num_x, ncoils, kx, ky, t, x, y, fm, d_r, d_i, s_r,
-                              s_i, cg_num, lambda, nullptr, x_r, x_i);
+                              s_i, cg_num, lambda, nullptr, 1, x_r, x_i);
 }
 
-extern "C" int imp_bruteforce_cg_reg(imp_ctx *ctx, int num_k, int num_x, int ncoils,
-                                     const float *kx, const float *ky, const float *t,
-                                     const float *x, const float *y, const float *fm,
-                                     const float *d_r, const float *d_i, const float *s_r,
-                                     const float *s_i, int cg_num, float lambda,
-                                     const imp_sparse_reg *reg, float *x_r, float *x_i) {
-    if (!reg) return imp::fail(IMP_EINVAL, "imp_bruteforce_cg_reg: NULL regularizer");
+extern "C" int imp_bruteforce_cg_ex(imp_ctx *ctx, int num_k, int num_x, int ncoils,
+                                    const float *kx, const float *ky, const float *t,
+                                    const float *x, const float *y, const float *fm,
+                                    const float *d_r, const float *d_i, const float *s_r,
+                                    const float *s_i, int cg_num, float lambda,
+                                    const imp_sparse_reg *reg, int tv_num, float *x_r,
+                                    float *x_i) {
     return imp::bruteforce_cg(ctx, num_k, num_x, ncoils, kx, ky, t, x, y, fm, d_r, d_i, s_r,
-                              s_i, cg_num, lambda, imp::sparse_reg_of(reg), x_r, x_i);
+                              s_i, cg_num, lambda, imp::sparse_reg_of(reg), tv_num, x_r, x_i);
 }

@@ -99,7 +99,7 @@ int bruteforce_cg(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *k
                   const float *ky, const float *t, const float *x, const float *y,
                   const float *fm, const float *d_r, const float *d_i, const float *s_r,
                   const float *s_i, int cg_num, float lambda, const SparseReg *sreg,
-                  float *x_r, float *x_i);
+                  int tv_num, float *x_r, float *x_i);
 
 // internal CG flag: track the best iterate in ToepPlan::vbest but leave the
 // last iterate in x

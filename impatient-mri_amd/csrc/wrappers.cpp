@@ -432,7 +432,8 @@ namespace {
 int bforce_host(float *x_r, float *x_i, const float *y_r, const float *y_i, const float *kx,
                 const float *ky, const float *ix, const float *iy, const float *fm,
                 const float *t, int cg_num, int num_k, int num_i, float fd_penalizer,
-                const float *s_r, const float *s_i, int ncoils, const imp_coo_matrix *c) {
+                const float *s_r, const float *s_i, int ncoils, const imp_coo_matrix *c,
+                int tv_num) {
     if (!x_r || !x_i || !y_r || !y_i || !kx || !ky || !ix || !iy || !t || num_k <= 0 ||
         num_i <= 0 || ncoils <= 0 || (s_r == nullptr) != (s_i == nullptr))
         return fail(IMP_EINVAL, "imp_bruteForceGpu: bad arguments");
@@ -464,7 +465,7 @@ int bforce_host(float *x_r, float *x_i, const float *y_r, const float *y_i, cons
     IMP_TRY(a[11].up(x_i, num_i));
     IMP_TRY(bruteforce_cg(ctx, num_k, num_i, ncoils, a[0].p, a[1].p, a[2].p, a[3].p, a[4].p,
                           a[5].p, a[6].p, a[7].p, a[8].p, a[9].p, cg_num, fd_penalizer,
-                          sparse_reg_of(reg), a[10].p, a[11].p));
+                          sparse_reg_of(reg), tv_num, a[10].p, a[11].p));
     IMP_HIP(hipMemcpy(x_r, a[10].p, num_i * sizeof(float), hipMemcpyDeviceToHost));
     IMP_HIP(hipMemcpy(x_i, a[11].p, num_i * sizeof(float), hipMemcpyDeviceToHost));
     return IMP_OK;
@@ -477,16 +478,15 @@ extern "C" int imp_bruteForceGpu(float *x_r, float *x_i, const float *y_r, const
                                  int num_k, int num_i, float fd_penalizer, const float *s_r,
                                  const float *s_i, int ncoils) {
     return bforce_host(x_r, x_i, y_r, y_i, kx, ky, ix, iy, fm, t, cg_num, num_k, num_i,
-                       fd_penalizer, s_r, s_i, ncoils, nullptr);
+                       fd_penalizer, s_r, s_i, ncoils, nullptr, 1);
 }
 
-extern "C" int imp_bruteForceGpu_reg(float *x_r, float *x_i, const float *y_r,
-                                     const float *y_i, const float *kx, const float *ky,
-                                     const float *ix, const float *iy, const float *fm,
-                                     const float *t, int cg_num, int num_k, int num_i,
-                                     float fd_penalizer, const float *s_r, const float *s_i,
-                                     int ncoils, const imp_coo_matrix *c) {
-    if (!c) return fail(IMP_EINVAL, "imp_bruteForceGpu_reg: NULL matrix");
+extern "C" int imp_bruteForceGpu_ex(float *x_r, float *x_i, const float *y_r, const float *y_i,
+                                    const float *kx, const float *ky, const float *ix,
+                                    const float *iy, const float *fm, const float *t,
+                                    int cg_num, int num_k, int num_i, float fd_penalizer,
+                                    const float *s_r, const float *s_i, int ncoils,
+                                    const imp_coo_matrix *c, int enable_tv, int tv_num) {
     return bforce_host(x_r, x_i, y_r, y_i, kx, ky, ix, iy, fm, t, cg_num, num_k, num_i,
-                       fd_penalizer, s_r, s_i, ncoils, c);
+                       fd_penalizer, s_r, s_i, ncoils, c, enable_tv ? tv_num : 1);
 }

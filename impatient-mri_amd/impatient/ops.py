@@ -299,9 +299,11 @@ class SparseReg:
             pass
 
 
-def bruteforce_cg(ctx: Context, kx, ky, t, x, y, fm, d, sens, cg_num, lam, x0, reg=None):
+def bruteforce_cg(ctx: Context, kx, ky, t, x, y, fm, d, sens, cg_num, lam, x0, reg=None,
+                  tv_num=1):
     """bruteForceGpu's CG from x0 (cg_num - 1 iterations); d is [C, M].
-    reg: a SparseReg for R = C^T C (the -reg branch) instead of the stencil."""
+    reg: a SparseReg for R = C^T W C (the -reg branch) instead of the stencil;
+    tv_num > 1: total-variation passes (W from the current image)."""
     d = np.atleast_2d(np.asarray(d))
     ncoils, nk = d.shape
     nx = len(x)
@@ -315,10 +317,11 @@ def bruteforce_cg(ctx: Context, kx, ky, t, x, y, fm, d, sens, cg_num, lam, x0, r
     head = (ctx.h, C.c_int(nk), C.c_int(nx), C.c_int(ncoils), ptr(a[0]), ptr(a[1]),
             ptr(a[2]), ptr(a[3]), ptr(a[4]), ptr(afm), ptr(dr), ptr(di), ptr(sr), ptr(si),
             C.c_int(cg_num), C.c_float(lam))
-    if reg is None:
+    if reg is None and tv_num <= 1:
         call("imp_bruteforce_cg", *head, ptr(xr), ptr(xi))
     else:
-        call("imp_bruteforce_cg_reg", *head, reg.h, ptr(xr), ptr(xi))
+        call("imp_bruteforce_cg_ex", *head, reg.h if reg is not None else None,
+             C.c_int(tv_num), ptr(xr), ptr(xi))
     return _cplx(xr, xi)
 
 

@@ -323,16 +323,19 @@ int imp_bruteforce_cg(imp_ctx *ctx, int num_k, int num_x, int ncoils,
                       const float *s_r, const float *s_i,
                       int cg_num, float lambda,
                       float *x_r, float *x_i);
-/* The same with R = C^T C from a sparse-matrix regularizer (the brute
- * force's -reg branch, bruteForceGpu.cu:513-541, 579-629); C is
- * (2 num_x) x num_x. */
-int imp_bruteforce_cg_reg(imp_ctx *ctx, int num_k, int num_x, int ncoils,
-                          const float *kx, const float *ky, const float *t,
-                          const float *x, const float *y, const float *fm,
-                          const float *d_r, const float *d_i,
-                          const float *s_r, const float *s_i,
-                          int cg_num, float lambda, const imp_sparse_reg *reg,
-                          float *x_r, float *x_i);
+/* The same with the brute force's other options (bruteForceGpu.cu:143-156):
+ * reg != NULL: R = C^T W C from a sparse-matrix regularizer (the -reg
+ * branch, bruteForceGpu.cu:513-541, 579-629; C is (2 num_x) x num_x);
+ * tv_num > 1: total-variation passes (bruteForceGpu.cu:431-757), each
+ * restarting the CG from the current image with W = 0.5 / sqrt(|D_h x|^2 +
+ * |D_v x|^2 + 1e-6) of it (W = 1 in the first pass); tv_num <= 1: one pass. */
+int imp_bruteforce_cg_ex(imp_ctx *ctx, int num_k, int num_x, int ncoils,
+                         const float *kx, const float *ky, const float *t,
+                         const float *x, const float *y, const float *fm,
+                         const float *d_r, const float *d_i,
+                         const float *s_r, const float *s_i,
+                         int cg_num, float lambda, const imp_sparse_reg *reg,
+                         int tv_num, float *x_r, float *x_i);
 
 /* ------------------------------------------------------------ data files
  * Reference file formats (kernels/structures.cpp:718-1110,
@@ -406,22 +409,24 @@ int imp_toeplitz_recon(int argc, char **argv, int enable_tv, int enable_reg,
  * bruteForceGpu.cu:143), host pointers: x_r/x_i [num_i] in/out (initial
  * estimate in, image out); y_r/y_i [ncoils*num_k]; kx,ky,t [num_k];
  * ix,iy,fm [num_i]; s_r/s_i [ncoils*num_i] or NULL (S = 1).  fd_penalizer
- * > 0 enables the finite-difference term (the -fd/-fdp options); the TV
- * variant is not offered.  No power-of-two padding. */
+ * > 0 enables the finite-difference term (the -fd/-fdp options).  No
+ * power-of-two padding. */
 int imp_bruteForceGpu(float *x_r, float *x_i, const float *y_r, const float *y_i,
                       const float *kx, const float *ky, const float *ix,
                       const float *iy, const float *fm, const float *t,
                       int cg_num, int num_k, int num_i, float fd_penalizer,
                       const float *s_r, const float *s_i, int ncoils);
-/* The same with the sparse-matrix regularizer fd_penalizer C^T C in place of
- * the stencil (bruteForceGpu's enable_regularization and CooMatrix *c,
- * bruteForceGpu.cu:150, 309-360); c is (2 num_i) x num_i, host arrays. */
-int imp_bruteForceGpu_reg(float *x_r, float *x_i, const float *y_r, const float *y_i,
-                          const float *kx, const float *ky, const float *ix,
-                          const float *iy, const float *fm, const float *t,
-                          int cg_num, int num_k, int num_i, float fd_penalizer,
-                          const float *s_r, const float *s_i, int ncoils,
-                          const imp_coo_matrix *c);
+/* The same with bruteForceGpu's other options (bruteForceGpu.cu:143-156):
+ * c != NULL: the sparse-matrix regularizer fd_penalizer C^T W C in place of
+ * the stencil (enable_regularization with CooMatrix *c; c is
+ * (2 num_i) x num_i, host arrays); enable_tv: tv_num total-variation passes
+ * (enable_total_variation, tv_num; see imp_bruteforce_cg_ex). */
+int imp_bruteForceGpu_ex(float *x_r, float *x_i, const float *y_r, const float *y_i,
+                         const float *kx, const float *ky, const float *ix,
+                         const float *iy, const float *fm, const float *t,
+                         int cg_num, int num_k, int num_i, float fd_penalizer,
+                         const float *s_r, const float *s_i, int ncoils,
+                         const imp_coo_matrix *c, int enable_tv, int tv_num);
 
 #ifdef __cplusplus
 }

@@ -47,3 +47,30 @@ def test_bruteforce_matches_oracle(ctx, J, S, ncoils, lam, tol32, tol64):
     assert e32 < tol32
     assert e64 < tol64
     assert abs(cg - c32) / c32 < 2e-2
+
+
+def test_bruteforce_tv_passes_match_oracle(ctx):
+    """Total-variation passes (bruteForceGpu.cu:431-757): pass 0 with W = 1,
+    then W = tv_weight(x) of the current image and the CG restarted from it."""
+    from impatient import ops, synth
+    from oracle import oracle as orc
+    lam, cg_num, tv_num = 0.1, 6, 3
+    p = synth.tiny(N=16, ncoils=4, ntime_segs=3, J=16, S=512)
+    y = orc.nudft_forward(p.kx, p.ky, None, p.t, p.x, p.y, None, p.fm, p.rho, p.sens)
+    y = synth.add_noise(y, 40.0, 3)
+    x0 = np.zeros(p.nvox, np.complex64)
+    got = ops.bruteforce_cg(ctx, p.kx, p.ky, p.t, p.x, p.y, p.fm, y, p.sens, cg_num, lam, x0,
+                            tv_num=tv_num)
+    one = ops.bruteforce_cg(ctx, p.kx, p.ky, p.t, p.x, p.y, p.fm, y, p.sens, cg_num, lam, x0)
+    F = lambda v: orc.nudft_forward(p.kx, p.ky, None, p.t, p.x, p.y, None, p.fm, v, p.sens)
+    FH = lambda d: orc.nudft_adjoint(p.kx, p.ky, None, p.t, p.x, p.y, None, p.fm, d, p.sens)
+    x = x0.astype(np.complex128)
+    for tv_i in range(tv_num):
+        w = None if tv_i == 0 else orc.tv_weight(x.reshape(16, 16))
+        R = lambda v, w=w: orc.fd_reg(v.reshape(16, 16), lam, w).reshape(-1)
+        x = orc.bruteforce_cg(F, FH, R, y, x, cg_num, vec_dtype=np.complex64)
+    err, moved = rel_l2(got, x), rel_l2(got, one)
+    print(f"brute force TV x{tv_num}: vs FP32-storage oracle {err:.2e}; "
+          f"vs one pass {moved:.2e}")
+    assert err < 5e-3
+    assert moved > 10 * err

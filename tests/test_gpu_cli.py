@@ -152,3 +152,23 @@ def test_cli_bruteforce_multislice(data_dir, ctx):
         err = np.max(np.abs(got - ref))
         print(f"slice {s}: max |CLI - C ABI| = {err:.2e}")
         assert err <= 5e-7 + 1e-6 * np.abs(ref).max() * 1.5
+
+
+def test_cli_bruteforce_tv(data_dir, ctx):
+    """-tv -tv_num n on the brute-force driver: n passes with TV weights."""
+    from impatient import files, ops
+    out = data_dir / "out"
+    r = subprocess.run([str(CLI), "-idir", str(data_dir), "-odir", str(out), "-cg_num", "5",
+                        "-tv", "-tv_num", "3", "-fdp", "0.01"], capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout[-1500:], r.stderr[-1500:])
+    assert r.returncode == 0
+    xr = np.loadtxt(out / "output_gpu_r.dat")
+    xi = np.loadtxt(out / "output_gpu_i.dat")
+    rd = lambda n: files.read(data_dir / f"{n}.dat")
+    d = (rd("kdata_r") + 1j * rd("kdata_i"))[None, :]
+    x0 = rd("idata_r") + 1j * rd("idata_i")
+    ref = ops.bruteforce_cg(ctx, rd("kx"), rd("ky"), rd("t"), rd("ix"), rd("iy"), rd("fm"), d,
+                            None, 5, 0.01, x0, tv_num=3)
+    assert np.max(np.abs(xr - ref.real)) <= 5e-7 + 1e-6 * np.abs(ref.real).max()
+    assert np.max(np.abs(xi - ref.imag)) <= 5e-7 + 1e-6 * np.abs(ref.imag).max()
