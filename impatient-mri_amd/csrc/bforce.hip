@@ -224,6 +224,9 @@ extern "C" int imp_bruteforce_cg_ex(imp_ctx *ctx, int num_k, int num_x, int ncoi
                                     const float *s_i, int cg_num, float lambda,
                                     const imp_sparse_reg *reg, int tv_num, float *x_r,
                                     float *x_i) {
+    const imp::SparseReg *R = imp::sparse_reg_of(reg);
+    if (reg && !R)
+        return imp::fail(IMP_EINVAL, "imp_bruteforce_cg_ex: regularizer has no context");
     return imp::bruteforce_cg(ctx, num_k, num_x, ncoils, kx, ky, t, x, y, fm, d_r, d_i, s_r,
-                              s_i, cg_num, lambda, imp::sparse_reg_of(reg), tv_num, x_r, x_i);
+                              s_i, cg_num, lambda, R, tv_num, x_r, x_i);
 }

@@ -95,6 +95,8 @@ struct imp_ctx {
     std::set<struct imp_toep *> toeps;
     // live networks created on this context
     std::set<struct imp_cnn *> cnns;
+    // live sparse-matrix regularizers created on this context
+    std::set<struct imp_sparse_reg *> sregs;
     // kernel timer (imp_profile_*): off by default
     bool prof_on = false;
     std::vector<imp::ProfRec> prof_recs;   // recorded this session
@@ -114,6 +116,9 @@ int ensure_dscratch(imp_ctx *ctx, size_t n);
 // Frees a network's device memory and detaches it from its context (for
 // imp_ctx_destroy; a later imp_cnn_destroy then frees only the host object).
 void detach_cnn(struct imp_cnn *net);
+// The same for a sparse-matrix regularizer (imp_sparse_reg_destroy then
+// frees only the host object).
+void detach_sparse_reg(struct imp_sparse_reg *reg);
 
 // Kernel timer: prof_begin records an event before a launch and returns a
 // record index (-1 when the timer is off); prof_end records the closing event.

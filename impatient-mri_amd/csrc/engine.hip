@@ -179,6 +179,8 @@ extern "C" int imp_ctx_destroy(imp_ctx *ctx) {
     ctx->toeps.clear();
     for (imp_cnn *net : ctx->cnns) detach_cnn(net);
     ctx->cnns.clear();
+    for (imp_sparse_reg *reg : ctx->sregs) detach_sparse_reg(reg);
+    ctx->sregs.clear();
     prof_release(ctx);
     for (hipEvent_t e : ctx->prof_free) (void)hipEventDestroy(e);
     for (auto &w : ctx->ws)
@@ -584,6 +586,7 @@ extern "C" int imp_toep_set_weight(imp_toep *op, const float *w) {
 extern "C" int imp_toep_set_sparse_reg(imp_toep *op, const imp_sparse_reg *reg) {
     if (!op) return fail(IMP_EINVAL, "imp_toep_set_sparse_reg: NULL operator");
     const SparseReg *R = sparse_reg_of(reg);
+    if (reg && !R) return fail(IMP_EINVAL, "imp_toep_set_sparse_reg: regularizer has no context");
     if (R) {
         const long ndim = op->P.N3 > 1 ? 3 : 2;
         if ((long)R->cols != op->P.nvox || (long)R->rows != ndim * op->P.nvox)

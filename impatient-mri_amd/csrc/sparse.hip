@@ -116,10 +116,24 @@ struct imp_sparse_reg {
     std::vector<void *> owned;
 };
 
+namespace imp {
+void detach_sparse_reg(imp_sparse_reg *reg) {
+    for (void *p : reg->owned) (void)hipFree(p);
+    reg->owned.clear();
+    reg->R = SparseReg{};
+    reg->ctx = nullptr;
+}
+}  // namespace imp
+
+// After imp_ctx_destroy the object is detached (device memory already freed,
+// no context): only the host object is freed here.
 extern "C" int imp_sparse_reg_destroy(imp_sparse_reg *reg) {
     if (!reg) return IMP_OK;
-    if (reg->ctx) (void)hipStreamSynchronize(reg->ctx->stream);
-    for (void *p : reg->owned) (void)hipFree(p);
+    if (reg->ctx) {
+        (void)hipStreamSynchronize(reg->ctx->stream);
+        reg->ctx->sregs.erase(reg);
+        detach_sparse_reg(reg);
+    }
     delete reg;
     return IMP_OK;
 }
@@ -144,6 +158,7 @@ extern "C" int imp_sparse_reg_create(imp_ctx *ctx, int num_rows, int num_cols, i
     build_csr(num_cols, num_nonzeros, J, I, V, tp, ti, tv);
     imp_sparse_reg *reg = new imp_sparse_reg;
     reg->ctx = ctx;
+    ctx->sregs.insert(reg);
     SparseReg &R = reg->R;
     R.rows = num_rows;
     R.cols = num_cols;
@@ -180,6 +195,7 @@ extern "C" int imp_sparse_reg_apply(imp_sparse_reg *reg, float lambda, const flo
                                     float *out_i) {
     if (!reg || !p_r || !p_i || !out_r || !out_i)
         return fail(IMP_EINVAL, "imp_sparse_reg_apply: bad arguments");
+    if (!reg->ctx) return fail(IMP_EINVAL, "imp_sparse_reg_apply: regularizer has no context");
     imp_ctx *ctx = reg->ctx;
     const long n = reg->R.cols;
     void *buf = nullptr;
@@ -192,5 +208,8 @@ extern "C" int imp_sparse_reg_apply(imp_sparse_reg *reg, float lambda, const flo
 }
 
 namespace imp {
-const SparseReg *sparse_reg_of(const imp_sparse_reg *reg) { return reg ? &reg->R : nullptr; }
+// nullptr for NULL and for a regularizer whose context was destroyed
+const SparseReg *sparse_reg_of(const imp_sparse_reg *reg) {
+    return reg && reg->ctx ? &reg->R : nullptr;
+}
 }  // namespace imp

@@ -94,6 +94,15 @@ def test_toeplitz_operator_with_sparse_reg(ctx):
     e_cg = rel_l2(x_d, x_fd)
     print(f"CG(6) C = D vs stencil: {e_cg:.2e}")
     assert e_cg < 1e-4
+    # the graph-captured iteration replays the same kernels (random C, twice)
+    from impatient._lib import IMP_CG_BEST_ITERATE, IMP_CG_GRAPH
+    op.set_sparse_reg(R)
+    x_e, _ = op.cg(b, 6)
+    for _ in range(2):
+        x_g, _ = op.cg(b, 6, 0.0, IMP_CG_BEST_ITERATE | IMP_CG_GRAPH)
+        e_g = rel_l2(x_g, x_e)
+        print(f"CG(6) with C, graph vs eager: {e_g:.2e}")
+        assert e_g < 1e-6
     op.close()
 
 
@@ -122,3 +131,24 @@ def test_bruteforce_with_sparse_reg(ctx):
     e_d = rel_l2(x_d, x_fd)
     print(f"brute force + sparse reg vs FP32-storage oracle {err:.2e}; C = D vs stencil {e_d:.2e}")
     assert err < 5e-3 and e_d < 1e-4
+
+
+def test_regularizer_outlives_its_context(ctx):
+    """Destroying the context first detaches the regularizer: its device
+    memory is freed with the context, later calls report an error, and the
+    final destroy frees only the host object."""
+    from impatient import _lib, ops
+    from impatient._lib import Context, ptr
+    c2 = Context(0)
+    coo = random_coo(32, 16, 2, seed=18)
+    R = ops.SparseReg(c2, *coo)
+    p = np.ones(16, np.complex64)
+    ref = R.apply(p, 1.0)
+    assert np.isfinite(ref).all()
+    c2.close()
+    a = [ctx.alloc((16,)) for _ in range(4)]   # valid arrays on a live context
+    with pytest.raises(_lib.ImpError) as e:
+        _lib.call("imp_sparse_reg_apply", R.h, _lib.C.c_float(1.0), None, ptr(a[0]),
+                  ptr(a[1]), ptr(a[2]), ptr(a[3]))
+    assert e.value.status == -1
+    R.close()
