@@ -152,3 +152,26 @@ def test_regularizer_outlives_its_context(ctx):
                   ptr(a[1]), ptr(a[2]), ptr(a[3]))
     assert e.value.status == -1
     R.close()
+
+
+def test_toeplitz_operator_with_sparse_reg_3d(ctx):
+    """3D image: C has three blocks of N rows (recon.cu:1613-1631)."""
+    from impatient import ops, synth
+    from oracle import oracle as orc
+    L, lam = 2, 1e-2
+    p = synth.tiny(N=8, Nz=4, ncoils=2, ntime_segs=L, seed=7, max_hz=60.0)
+    q = orc.toeplitz_q(p.kx, p.ky, p.kz, p.t, p.Nx, p.Ny, p.Nz, L)
+    tau, t0 = orc.segment_timing(p.t, L)
+    op = ops.Toeplitz(ctx, p.shape, q.astype(np.complex64), tau, t0, p.fm, p.sens, lam)
+    rng = np.random.default_rng(19)
+    v = (rng.standard_normal(p.nvox) + 1j * rng.standard_normal(p.nvox)).astype(np.complex64)
+    coo = random_coo(3 * p.nvox, p.nvox, 2, seed=20)
+    R = ops.SparseReg(ctx, *coo)
+    op.set_sparse_reg(R)
+    got = op.apply(v)
+    ref = (orc.toeplitz_apply(v.reshape(p.shape), q, tau, t0, p.fm, p.sens, 0.0).reshape(-1)
+           + orc.sparse_reg(v, lam, *coo))
+    err = rel_l2(got, ref)
+    print(f"3D toeplitz + sparse reg: {err:.2e}")
+    assert err < 2e-6
+    op.close()
