@@ -24,13 +24,16 @@ namespace imp {
 
 namespace {
 
-// cf[r] = W[r mod n] * sum_e val[e] p[idx[e]],  e in [ptr[r], ptr[r+1])
-__global__ void k_csr_wc(int rows, long n, const int *__restrict__ ptr,
+// cf[r] = W[r mod n] * sum_e val[e] p[idx[e]],  e in [ptr[r], ptr[r+1]).
+// blockIdx.y is the block of n rows, so W's index needs no division.
+__global__ void k_csr_wc(int rows, int n, const int *__restrict__ ptr,
                          const int *__restrict__ idx, const float *__restrict__ val,
                          const float2 *__restrict__ p, const float *__restrict__ w,
                          float2 *__restrict__ cf) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= rows) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const long rl = (long)blockIdx.y * n + i;
+    if (i >= n || rl >= rows) return;
+    const int r = (int)rl;
     float sx = 0.f, sy = 0.f;
     const int e1 = ptr[r + 1];
     for (int e = ptr[r]; e < e1; ++e) {
@@ -40,7 +43,7 @@ __global__ void k_csr_wc(int rows, long n, const int *__restrict__ ptr,
         sy = fmaf(v, q.y, sy);
     }
     if (w) {
-        const float ww = w[r % n];
+        const float ww = w[i];
         sx *= ww;
         sy *= ww;
     }
@@ -94,8 +97,8 @@ void build_csr(int rows, long nnz, const int *row, const int *col, const float *
 int sparse_reg(imp_ctx *ctx, const SparseReg &R, float lambda, const float2 *p, const float *w,
                float2 *out, bool accumulate) {
     int rec = prof_begin(ctx, "k_csr_wc");
-    hipLaunchKernelGGL(k_csr_wc, dim3(ceil_div(R.rows, 256)), dim3(256), 0, ctx->stream, R.rows,
-                       (long)R.cols, R.c_ptr, R.c_idx, R.c_val, p, w, R.cf);
+    hipLaunchKernelGGL(k_csr_wc, dim3(ceil_div(R.cols, 256), ceil_div(R.rows, R.cols)), dim3(256),
+                       0, ctx->stream, R.rows, R.cols, R.c_ptr, R.c_idx, R.c_val, p, w, R.cf);
     IMP_CHECK_LAUNCH();
     prof_end(ctx, rec);
     rec = prof_begin(ctx, "k_csr_t");
@@ -145,6 +148,10 @@ extern "C" int imp_sparse_reg_create(imp_ctx *ctx, int num_rows, int num_cols, i
     if (!ctx || !out || num_rows <= 0 || num_cols <= 0 || num_nonzeros < 0 ||
         (num_nonzeros > 0 && (!I || !J || !V)))
         return fail(IMP_EINVAL, "imp_sparse_reg_create: bad arguments");
+    // one launch row of the first product per block of num_cols rows
+    if (ceil_div(num_rows, num_cols) > 65535)
+        return fail(IMP_EINVAL, "imp_sparse_reg_create: %d x %d has more than 65535 blocks of %d "
+                                "rows", num_rows, num_cols, num_cols);
     // every index is checked here, so the kernels' gathers stay in bounds
     for (long e = 0; e < num_nonzeros; ++e) {
         if (I[e] < 0 || I[e] >= num_rows || J[e] < 0 || J[e] >= num_cols)
