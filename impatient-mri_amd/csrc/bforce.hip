@@ -227,6 +227,8 @@ extern "C" int imp_bruteforce_cg_ex(imp_ctx *ctx, int num_k, int num_x, int ncoi
     const imp::SparseReg *R = imp::sparse_reg_of(reg);
     if (reg && !R)
         return imp::fail(IMP_EINVAL, "imp_bruteforce_cg_ex: regularizer has no context");
+    if (R && (!ctx || imp::sparse_reg_ctx(reg)->device != ctx->device))
+        return imp::fail(IMP_EINVAL, "imp_bruteforce_cg_ex: regularizer is on another device");
     return imp::bruteforce_cg(ctx, num_k, num_x, ncoils, kx, ky, t, x, y, fm, d_r, d_i, s_r,
                               s_i, cg_num, lambda, R, tv_num, x_r, x_i);
 }

@@ -587,6 +587,9 @@ extern "C" int imp_toep_set_sparse_reg(imp_toep *op, const imp_sparse_reg *reg) 
     if (!op) return fail(IMP_EINVAL, "imp_toep_set_sparse_reg: NULL operator");
     const SparseReg *R = sparse_reg_of(reg);
     if (reg && !R) return fail(IMP_EINVAL, "imp_toep_set_sparse_reg: regularizer has no context");
+    if (R && (!op->ctx || sparse_reg_ctx(reg)->device != op->ctx->device))
+        return fail(IMP_EINVAL, "imp_toep_set_sparse_reg: regularizer and operator are on "
+                                "different devices");
     if (R) {
         const long ndim = op->P.N3 > 1 ? 3 : 2;
         if ((long)R->cols != op->P.nvox || (long)R->rows != ndim * op->P.nvox)

@@ -219,4 +219,5 @@ namespace imp {
 const SparseReg *sparse_reg_of(const imp_sparse_reg *reg) {
     return reg && reg->ctx ? &reg->R : nullptr;
 }
+imp_ctx *sparse_reg_ctx(const imp_sparse_reg *reg) { return reg ? reg->ctx : nullptr; }
 }  // namespace imp

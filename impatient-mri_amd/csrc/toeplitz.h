@@ -64,6 +64,7 @@ int tv_weight(imp_ctx *ctx, int N1, int N2, int N3, const float2 *x, float *w);
 int sparse_reg(imp_ctx *ctx, const SparseReg &R, float lambda, const float2 *p, const float *w,
                float2 *out, bool accumulate);
 const SparseReg *sparse_reg_of(const imp_sparse_reg *reg);
+imp_ctx *sparse_reg_ctx(const imp_sparse_reg *reg);
 // Re(a^H b) in FP64, result to host (synchronous)
 int dot_re(imp_ctx *ctx, const float2 *a, const float2 *b, long n, double *out);
 int axpy_c(imp_ctx *ctx, float alpha, const float2 *x, float2 *y, long n);  // y += alpha x
