@@ -597,6 +597,7 @@ extern "C" int imp_toep_set_sparse_reg(imp_toep *op, const imp_sparse_reg *reg) 
                         R->rows, R->cols, ndim * op->P.nvox, op->P.nvox);
     }
     op->P.sreg = R;
+    op->P.sreg_lost = false;
     return IMP_OK;
 }
 

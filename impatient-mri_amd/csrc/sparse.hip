@@ -129,11 +129,17 @@ void detach_sparse_reg(imp_sparse_reg *reg) {
 }  // namespace imp
 
 // After imp_ctx_destroy the object is detached (device memory already freed,
-// no context): only the host object is freed here.
+// no context): only the host object is freed here.  Operators that still use
+// it lose it, and their applies fail until imp_toep_set_sparse_reg.
 extern "C" int imp_sparse_reg_destroy(imp_sparse_reg *reg) {
     if (!reg) return IMP_OK;
     if (reg->ctx) {
         (void)hipStreamSynchronize(reg->ctx->stream);
+        for (imp_toep *op : reg->ctx->toeps)
+            if (op->P.sreg == &reg->R) {
+                op->P.sreg = nullptr;
+                op->P.sreg_lost = true;
+            }
         reg->ctx->sregs.erase(reg);
         detach_sparse_reg(reg);
     }

@@ -38,6 +38,7 @@ struct ToepPlan {
     float *fcyc = nullptr;        // [nvox] field map in Hz (w / 2pi)
     float *w = nullptr;           // [nvox] TV weight or null (W = I), not owned
     const SparseReg *sreg = nullptr;  // R = C^T W C instead of the stencil, not owned
+    bool sreg_lost = false;           // sreg was destroyed while set: applies fail
     // CG work vectors (interleaved)
     float2 *vb = nullptr, *vx = nullptr, *vr = nullptr, *vp = nullptr,
            *vq = nullptr, *vbest = nullptr, *vz = nullptr;

@@ -426,6 +426,8 @@ int axpy_c(imp_ctx *ctx, float alpha, const float2 *x, float2 *y, long n) {
 
 // out = A p = normal(p) + lambda R p
 int toep_apply_full(imp_ctx *ctx, ToepPlan &P, const float2 *p, float2 *out) {
+    if (P.sreg_lost)
+        return fail(IMP_EINVAL, "Toeplitz operator: its sparse regularizer was destroyed");
     IMP_TRY(toep_apply_normal(ctx, P, p, out));
     if (P.lambda != 0.f) {
         if (P.sreg)

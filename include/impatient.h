@@ -195,8 +195,9 @@ int imp_toep_set_weight(imp_toep *op, const float *w);
 /* Use lambda C^T W C (a sparse-matrix regularizer, below) as R instead of
  * the finite-difference stencil: the -reg branch of Computeq
  * (toeplitz/recon.cu:1543-1640).  C must be (ndim N) x N, ndim = 2 for
- * N3 = 1 and 3 otherwise; reg is not copied and must outlive its use by
- * the operator.  NULL restores the stencil. */
+ * N3 = 1 and 3 otherwise; reg is not copied.  Destroying reg while it is
+ * set makes the operator's applies fail (IMP_EINVAL) until it is set again;
+ * NULL restores the stencil. */
 struct imp_sparse_reg;
 int imp_toep_set_sparse_reg(imp_toep *op, const struct imp_sparse_reg *reg);
 /* Pass-E schedule: coils summed per workgroup (more coils per group = fewer

@@ -103,6 +103,15 @@ def test_toeplitz_operator_with_sparse_reg(ctx):
         e_g = rel_l2(x_g, x_e)
         print(f"CG(6) with C, graph vs eager: {e_g:.2e}")
         assert e_g < 1e-6
+    # destroying the regularizer while the operator uses it: applies fail
+    # (no dangling use) until the regularizer is replaced
+    R.close()
+    with pytest.raises(_lib.ImpError) as e2:
+        op.apply(v)
+    assert e2.value.status == -1
+    op.set_sparse_reg(None)
+    assert rel_l2(op.apply(v), orc.toeplitz_apply(v.reshape(shape), q, tau, t0, p.fm, p.sens,
+                                                  lam).reshape(-1)) < 2e-6
     op.close()
 
 
