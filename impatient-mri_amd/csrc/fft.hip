@@ -93,7 +93,10 @@ __global__ __launch_bounds__(ROW_THREADS) void k_fft_rows(float2 *data, long nli
 // Strided lines (element i of line j at i*es + j): P adjacent lines per
 // workgroup, thread = (line-local index, line) with the line fastest, so each
 // global access covers P consecutive complex values.  LDS holds [i][line]
-// with a row stride of P + 1.
+// with one pad row after every 16 rows (row lpad(i)): a stage's accesses,
+// P lines x 64/P consecutive or 16-strided elements per wave, then spread
+// over all LDS banks (a row stride of P + 1 gave 2-way conflicts at
+// 8-line panels).
 //   Single transforms (MODE 0 / 1): P = 16, a whole 128-byte row segment per
 //   access, while the workgroup stays within 1024 threads and 150 KB of LDS
 //   (measured at 512-point lines: 25% faster than P = 8).
@@ -118,9 +121,9 @@ template <int L, int MODE, bool HALF_IN, bool HALF_OUT>
 __global__ __launch_bounds__((col_threads<L, MODE>())) void k_fft_cols(
     float2 *data, long bstride, long es, int nlines, const float2 *__restrict__ qf,
     const float2 *__restrict__ tw) {
-    constexpr int P = col_lines<L, MODE>(), PS = P + 1;
+    constexpr int P = col_lines<L, MODE>();
     constexpr int E = fftc::vals_per_thread(L);
-    __shared__ float2 buf[fftc::num_stages(L) > 1 ? L * PS : 1];
+    __shared__ float2 buf[fftc::num_stages(L) > 1 ? fftc::line_lds(L) * P : 1];
     const int c = threadIdx.x % P, tl = threadIdx.x / P;
     // Workgroups are dispatched round-robin over the 8 XCDs: blocks b and
     // b + 8 of each group of 16 take adjacent panels, so the two halves of a
@@ -136,8 +139,8 @@ __global__ __launch_bounds__((col_threads<L, MODE>())) void k_fft_cols(
     constexpr int T = fftc::threads_per_line(L);
     const long step = (long)T * es;
     float2 *col = data + (long)blockIdx.y * bstride + (ok ? j : 0) + (long)tl * es;
-    auto lld = [&](int, int, int i) { return buf[i * PS + c]; };
-    auto lst = [&](int, int, int i, float2 v) { buf[i * PS + c] = v; };
+    auto lld = [&](int, int, int i) { return buf[fftc::lpad(i) * P + c]; };
+    auto lst = [&](int, int, int i, float2 v) { buf[fftc::lpad(i) * P + c] = v; };
     // i < L/2 exactly when m < E/2 (known per register)
     auto gld = [&](int b, int r, int) {
         const int m = fftc::first_reg<L>(b, r);
