@@ -10,6 +10,7 @@
 //             [-version] [-help]
 #include <sys/stat.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -50,8 +51,9 @@ void usage(FILE *f) {
                  "  -writeQ              write <idir>/Q_stone.file\n"
                  "  -reuseQ <dir>        read <dir>/Q_stone.file instead of computing Q\n"
                  "  -fd                  finite-difference regularizer (needs -fdp)\n"
-                 "  -fdp <lambda>        regularization weight\n"
-                 "  -tv / -tv_num <n>    total-variation outer iterations\n"
+                 "  -fdp <lambda>        regularization weight (enables -fd)\n"
+                 "  -tv / -tv_num <n>    total-variation outer iterations (-tv_num enables -tv)\n"
+                 "  -tv_update           export the brute-force image before every TV pass\n"
                  "  -reg                 sparse-matrix regularizer C^T W C, C from <idir>/c.rmtx\n"
                  "  -cpu / -nogpu / -mgpu  CPU and multi-GPU paths (not part of this engine)\n"
                  "  -version, -help\n");
@@ -114,15 +116,30 @@ int parse(int argc, char **argv, Options &o) {
         } else if (!std::strcmp(a, "-fd")) {
             o.fd = true;
         } else if (!std::strcmp(a, "-fdp")) {
-            if (!need_arg(i, argc, a)) return 1;
+            // setting the weight enables the finite-difference term (main.cpp:292-301)
+            if (!need_arg(i, argc, a) || argv[i + 1][0] == '-') {
+                std::fprintf(stderr, "\nError: Missing finite difference penalty.\n");
+                return 1;
+            }
             o.fdp = (float)std::atof(argv[++i]);
             o.fdp_given = true;
+            o.fd = true;
         } else if (!std::strcmp(a, "-tv")) {
             o.tv = true;
         } else if (!std::strcmp(a, "-tv_num")) {
-            if (!need_arg(i, argc, a)) return 1;
+            // setting the pass count enables TV (main.cpp:306-319)
+            if (!need_arg(i, argc, a) || argv[i + 1][0] == '-') {
+                std::fprintf(stderr, "\nError: Missing TV iteration number.\n");
+                return 1;
+            }
             o.tv_num = std::atoi(argv[++i]);
+            if (o.tv_num == 0) {
+                std::fprintf(stderr, "\nError: Zero or wrong TV iteration number format.\n");
+                return 1;
+            }
+            o.tv = true;
         } else if (!std::strcmp(a, "-tv_update")) {
+            o.tv = true;
             o.tv_update = true;
         } else if (!std::strcmp(a, "-reg")) {
             o.reg = true;
@@ -316,6 +333,26 @@ int run_toeplitz(const Options &o) {
     return IMP_OK;
 }
 
+// output_gpu_{r,i}.dat: the first n values of xr / xi as text "%f\n"
+// (exportDataCpu, structures.cpp:1166-1175)
+int export_text(const std::string &fr, const std::string &fi, const std::vector<float> &xr,
+                const std::vector<float> &xi, size_t n) {
+    FILE *f1 = std::fopen(fr.c_str(), "wt"), *f2 = std::fopen(fi.c_str(), "wt");
+    if (!f1 || !f2) {
+        std::fprintf(stderr, "Error: cannot write %s\n", (f1 ? fi : fr).c_str());
+        if (f1) std::fclose(f1);
+        if (f2) std::fclose(f2);
+        return IMP_EIO;
+    }
+    for (size_t i = 0; i < n; ++i) {
+        std::fprintf(f1, "%f\n", xr[i]);
+        std::fprintf(f2, "%f\n", xi[i]);
+    }
+    std::fclose(f1);
+    std::fclose(f2);
+    return IMP_OK;
+}
+
 // The brute-force driver of main_mri.cu:652-1025 (GPU part).
 int run_bruteforce(const Options &o) {
     const char *names[] = {"kx", "ky", "ix", "iy", "t", "fm", "kdata_r", "kdata_i",
@@ -359,6 +396,24 @@ int run_bruteforce(const Options &o) {
         c.J = rj.data();
         c.V = rv.data();
     }
+    mkdir(o.odir.c_str(), 0755);
+    const std::string fr = o.odir + "/output_gpu_r.dat", fi = o.odir + "/output_gpu_i.dat";
+    // -tv_update: before every TV pass the image is exported (bruteForceGpu.cu:452-463);
+    // the files hold every slice, this slice's current image in its place
+    struct Update {
+        std::vector<float> *xr, *xi;
+        size_t offset, need_x;
+        const std::string *fr, *fi;
+        int st;
+    } upd{&xr, &xi, 0, need_x, &fr, &fi, IMP_OK};
+    const imp_tv_pass_hook hook = [](void *user, int, const float *x_r, const float *x_i,
+                                     int n) {
+        Update *u = (Update *)user;
+        std::vector<float> r(*u->xr), im(*u->xi);
+        std::copy(x_r, x_r + n, r.begin() + u->offset);
+        std::copy(x_i, x_i + n, im.begin() + u->offset);
+        if (export_text(*u->fr, *u->fi, r, im, u->need_x) != IMP_OK) u->st = IMP_EIO;
+    };
     for (int sl = 0; sl < nslices && st == IMP_OK; ++sl) {
         if (nslices > 1) std::printf("  GPU: Processing slice %d.\n", sl);
         float *x_r = xr.data() + (size_t)sl * num_i, *x_i = xi.data() + (size_t)sl * num_i;
@@ -367,30 +422,18 @@ int run_bruteforce(const Options &o) {
         const float *fm = a[5].v.data() + (size_t)sl * num_i;
         const float *s_r = ncoils > 1 ? a[10].v.data() : nullptr,
                     *s_i = ncoils > 1 ? a[11].v.data() : nullptr;
+        upd.offset = (size_t)sl * num_i;
         // TV passes as bruteForceGpu runs them (bruteForceGpu.cu:431-757)
-        st = imp_bruteForceGpu_ex(x_r, x_i, y_r, y_i, a[0].v.data(), a[1].v.data(),
+        st = imp_bruteForceGpu_tv(x_r, x_i, y_r, y_i, a[0].v.data(), a[1].v.data(),
                                   a[2].v.data(), a[3].v.data(), fm, a[4].v.data(), o.cg_num,
                                   num_k, num_i, lambda, s_r, s_i, ncoils, o.reg ? &c : nullptr,
-                                  o.tv ? 1 : 0, o.tv_num);
+                                  o.tv ? 1 : 0, o.tv_num, o.tv_update ? hook : nullptr, &upd);
+        if (st == IMP_OK && upd.st != IMP_OK) st = upd.st;
     }
     if (report(st, "bruteForceGpu")) return st;
-    mkdir(o.odir.c_str(), 0755);
-    // output_gpu_{r,i}.dat, text "%f\n" (exportDataCpu, structures.cpp:1166-1175)
-    const std::string fr = o.odir + "/output_gpu_r.dat", fi = o.odir + "/output_gpu_i.dat";
-    FILE *f1 = std::fopen(fr.c_str(), "wt"), *f2 = std::fopen(fi.c_str(), "wt");
-    if (!f1 || !f2) {
-        std::fprintf(stderr, "Error: cannot write %s\n", o.odir.c_str());
-        if (f1) std::fclose(f1);
-        if (f2) std::fclose(f2);
-        return IMP_EIO;
-    }
+    // output_gpu_{r,i}.dat, text "%f\n" (exportDataCpu, structures.cpp:1166-1175):
     // every slice, num_slices * num_i values (main_mri.cu:917-918)
-    for (size_t i = 0; i < need_x; ++i) {
-        std::fprintf(f1, "%f\n", xr[i]);
-        std::fprintf(f2, "%f\n", xi[i]);
-    }
-    std::fclose(f1);
-    std::fclose(f2);
+    if (export_text(fr, fi, xr, xi, need_x) != IMP_OK) return IMP_EIO;
     std::printf("  GPU: Exporting results to %s\n", o.odir.c_str());
     return IMP_OK;
 }

@@ -7,6 +7,7 @@
 #include "toeplitz.h"
 
 #include <cmath>
+#include <vector>
 
 namespace imp {
 
@@ -91,7 +92,7 @@ int bruteforce_cg(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *k
                   const float *ky, const float *t, const float *x, const float *y,
                   const float *fm, const float *d_r, const float *d_i, const float *s_r,
                   const float *s_i, int cg_num, float lambda, const SparseReg *sreg,
-                  int tv_num, float *x_r, float *x_i) {
+                  int tv_num, float *x_r, float *x_i, imp_tv_pass_hook hook, void *hook_user) {
     if (!ctx || num_k <= 0 || num_x <= 0 || ncoils <= 0 || !kx || !ky || !t || !x || !y ||
         !d_r || !d_i || !x_r || !x_i || (s_r == nullptr) != (s_i == nullptr))
         return fail(IMP_EINVAL, "imp_bruteforce_cg: bad arguments");
@@ -151,7 +152,16 @@ int bruteforce_cg(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *k
     // TV passes (bruteForceGpu.cu:431-757): pass 0 with W = 1, each later
     // pass with W = 0.5 / sqrt(|D_h x|^2 + |D_v x|^2 + 1e-6) of the current
     // image, and the CG restarted from it; one pass without TV
+    std::vector<float> hx;
     for (int tv_i = 0; tv_i < passes; ++tv_i) {
+    if (hook) {
+        // the image this pass starts from, handed to the caller on the host
+        // (the -tv_update export of bruteForceGpu.cu:452-463)
+        hx.resize(2 * nx);
+        IMP_HIP(hipMemcpyAsync(hx.data(), X, 2 * nx * sizeof(float), hipMemcpyDeviceToHost, st));
+        IMP_HIP(hipStreamSynchronize(st));
+        hook(hook_user, tv_i, hx.data(), hx.data() + nx, num_x);
+    }
     if (tv_i > 0 && reg) {
         hipLaunchKernelGGL(k_split_pair_to_c, dim3(ceil_div(nx, 256)), dim3(256), 0, st, X, nx,
                            cb);
@@ -214,7 +224,7 @@ extern "C" int imp_bruteforce_cg(imp_ctx *ctx, int num_k, int num_x, int ncoils,
                                  const float *s_i, int cg_num, float lambda, float *x_r,
                                  float *x_i) {
     return imp::bruteforce_cg(ctx, num_k, num_x, ncoils, kx, ky, t, x, y, fm, d_r, d_i, s_r,
-                              s_i, cg_num, lambda, nullptr, 1, x_r, x_i);
+                              s_i, cg_num, lambda, nullptr, 1, x_r, x_i, nullptr, nullptr);
 }
 
 extern "C" int imp_bruteforce_cg_ex(imp_ctx *ctx, int num_k, int num_x, int ncoils,
@@ -227,8 +237,10 @@ extern "C" int imp_bruteforce_cg_ex(imp_ctx *ctx, int num_k, int num_x, int ncoi
     const imp::SparseReg *R = imp::sparse_reg_of(reg);
     if (reg && !R)
         return imp::fail(IMP_EINVAL, "imp_bruteforce_cg_ex: regularizer has no context");
-    if (R && (!ctx || imp::sparse_reg_ctx(reg)->device != ctx->device))
-        return imp::fail(IMP_EINVAL, "imp_bruteforce_cg_ex: regularizer is on another device");
+    // the regularizer's arrays are used on ctx's stream: it must belong to ctx
+    if (R && (!ctx || imp::sparse_reg_ctx(reg) != ctx))
+        return imp::fail(IMP_EINVAL, "imp_bruteforce_cg_ex: regularizer belongs to another "
+                                     "context");
     return imp::bruteforce_cg(ctx, num_k, num_x, ncoils, kx, ky, t, x, y, fm, d_r, d_i, s_r,
-                              s_i, cg_num, lambda, R, tv_num, x_r, x_i);
+                              s_i, cg_num, lambda, R, tv_num, x_r, x_i, nullptr, nullptr);
 }

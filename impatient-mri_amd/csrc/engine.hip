@@ -587,9 +587,12 @@ extern "C" int imp_toep_set_sparse_reg(imp_toep *op, const imp_sparse_reg *reg) 
     if (!op) return fail(IMP_EINVAL, "imp_toep_set_sparse_reg: NULL operator");
     const SparseReg *R = sparse_reg_of(reg);
     if (reg && !R) return fail(IMP_EINVAL, "imp_toep_set_sparse_reg: regularizer has no context");
-    if (R && (!op->ctx || sparse_reg_ctx(reg)->device != op->ctx->device))
-        return fail(IMP_EINVAL, "imp_toep_set_sparse_reg: regularizer and operator are on "
-                                "different devices");
+    // the regularizer must live on the operator's own context: its CSR
+    // arrays and work vector are used on that context's stream, and only
+    // that context detaches the operator when it is destroyed
+    if (R && (!op->ctx || sparse_reg_ctx(reg) != op->ctx))
+        return fail(IMP_EINVAL, "imp_toep_set_sparse_reg: regularizer and operator belong to "
+                                "different contexts");
     if (R) {
         const long ndim = op->P.N3 > 1 ? 3 : 2;
         if ((long)R->cols != op->P.nvox || (long)R->rows != ndim * op->P.nvox)
@@ -603,7 +606,7 @@ extern "C" int imp_toep_set_sparse_reg(imp_toep *op, const imp_sparse_reg *reg) 
 
 extern "C" int imp_toep_apply(imp_toep *op, const float *p_r, const float *p_i, float *out_r,
                               float *out_i) {
-    if (!op || !p_r || !p_i || !out_r || !out_i)
+    if (!op || !op->ctx || !p_r || !p_i || !out_r || !out_i)
         return fail(IMP_EINVAL, "imp_toep_apply: bad arguments");
     ToepPlan &P = op->P;
     imp_ctx *ctx = op->ctx;
@@ -616,7 +619,7 @@ extern "C" int imp_toep_apply(imp_toep *op, const float *p_r, const float *p_i, 
 extern "C" int imp_toep_cg(imp_toep *op, const float *b_r, const float *b_i, float *x_r,
                            float *x_i, int max_iter, double tol, int flags,
                            imp_cg_stats *stats) {
-    if (!op || !b_r || !b_i || !x_r || !x_i || max_iter < 0)
+    if (!op || !op->ctx || !b_r || !b_i || !x_r || !x_i || max_iter < 0)
         return fail(IMP_EINVAL, "imp_toep_cg: bad arguments");
     ToepPlan &P = op->P;
     imp_ctx *ctx = op->ctx;

@@ -101,11 +101,16 @@ int bruteforce_cg(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *k
                   const float *ky, const float *t, const float *x, const float *y,
                   const float *fm, const float *d_r, const float *d_i, const float *s_r,
                   const float *s_i, int cg_num, float lambda, const SparseReg *sreg,
-                  int tv_num, float *x_r, float *x_i);
+                  int tv_num, float *x_r, float *x_i, imp_tv_pass_hook hook = nullptr,
+                  void *hook_user = nullptr);
 
 // internal CG flag: track the best iterate in ToepPlan::vbest but leave the
 // last iterate in x
 constexpr int CG_KEEP_LAST = 0x100;
+// internal CG flag (plain CG only): start with the residual norm and best
+// residual the previous solve on this plan ended with, instead of |b - A x0|^2
+// (the restarts of toeplitz_recon, recon.cu:2043-2048)
+constexpr int CG_CARRY_SCALARS = 0x200;
 
 }  // namespace imp
 

@@ -166,14 +166,19 @@ enum {
     SC_N = 9
 };
 
-__global__ void k_cg_init(double *sc, const double *partials, int np) {
-    // r = b (x0 = 0) or r = b - A x0: partials hold |r|^2; bnorm set separately
+__global__ void k_cg_init(double *sc, const double *partials, int np, int carry) {
+    // r = b (x0 = 0) or r = b - A x0: partials hold |r|^2; bnorm set separately.
+    // carry: keep beta, rho and the best residual of the previous solve on
+    // this plan (the restarts of recon.cu:2043-2048 set them only when
+    // restart == 0)
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         double s = 0.0;
         for (int i = 0; i < np; ++i) s += partials[i];
-        sc[SC_BETA] = s;
-        sc[SC_BEST] = s;
-        sc[SC_RHO] = s;
+        if (!carry) {
+            sc[SC_BETA] = s;
+            sc[SC_BEST] = s;
+            sc[SC_RHO] = s;
+        }
         sc[SC_IMPROVED] = 0.0;
         sc[SC_BREAK] = 0.0;
     }
@@ -514,7 +519,9 @@ int toep_cg(imp_ctx *ctx, ToepPlan &P, const float2 *b, float2 *x, int max_iter,
     IMP_HIP(hipMemcpyAsync(P.vbest, x, n * sizeof(float2), hipMemcpyDeviceToDevice, ctx->stream));
     hipLaunchKernelGGL(k_dot_re, dim3(nb), dim3(VT), 0, ctx->stream, P.vr, P.vr, n, partials);
     IMP_CHECK_LAUNCH();
-    hipLaunchKernelGGL(k_cg_init, dim3(1), dim3(1), 0, ctx->stream, sc, partials, nb);
+    const bool carry = (flags & CG_CARRY_SCALARS) != 0 && !pc;
+    hipLaunchKernelGGL(k_cg_init, dim3(1), dim3(1), 0, ctx->stream, sc, partials, nb,
+                       carry ? 1 : 0);
     IMP_CHECK_LAUNCH();
     if (pc) {  // z = M^-1 r, p = z, rho = Re(r^H z)
         IMP_TRY(precond_apply(ctx, P, P.vr, P.vz));

@@ -386,8 +386,12 @@ extern "C" int imp_toeplitz_recon(int argc, char **argv, int enable_tv, int enab
             return rs;
         }
     }
-    // x stays the last iterate across TV passes; reconBest is the output
-    // (recon.cu:2013-2126, 2150-2162)
+    // x stays the last iterate across TV passes and restarts; reconBest is
+    // the output (recon.cu:2013-2126, 2150-2162).  On restart 0 every pass
+    // starts its best residual (and beta) from its own |b - A x|^2; later
+    // restarts carry both from the previous pass, so reconBest changes only
+    // where an iterate beats every earlier one (recon.cu:2043-2048, 2107-2110).
+    // W stays the last TV weight across restarts (pass 0 does not reset it).
     std::vector<float> best(2 * nvox, 0.f);
     int st = IMP_OK;
     const int passes = enable_tv ? (tv_num > 0 ? tv_num : 1) : 1;
@@ -400,11 +404,12 @@ extern "C" int imp_toeplitz_recon(int argc, char **argv, int enable_tv, int enab
             }
             imp_cg_stats stats{};
             ToepPlan &P = op->P;
+            const int carry = restart > 0 ? CG_CARRY_SCALARS : 0;
             if (st == IMP_OK) st = split_to_c(ctx, db.p, db.p + nvox, P.vb, nvox);
             if (st == IMP_OK) st = split_to_c(ctx, dx.p, dx.p + nvox, P.vx, nvox);
             if (st == IMP_OK)
                 st = toep_cg(ctx, P, P.vb, P.vx, numIterMax, 1.0e-16,
-                             IMP_CG_USE_X0 | IMP_CG_BEST_ITERATE | CG_KEEP_LAST, &stats);
+                             IMP_CG_USE_X0 | IMP_CG_BEST_ITERATE | CG_KEEP_LAST | carry, &stats);
             if (st == IMP_OK) st = c_to_split(ctx, P.vx, dx.p, dx.p + nvox, nvox);
             if (st == IMP_OK && hipStreamSynchronize(ctx->stream) != hipSuccess)
                 st = fail(IMP_EHIP, "imp_toeplitz_recon: CG stream");
@@ -433,7 +438,7 @@ int bforce_host(float *x_r, float *x_i, const float *y_r, const float *y_i, cons
                 const float *ky, const float *ix, const float *iy, const float *fm,
                 const float *t, int cg_num, int num_k, int num_i, float fd_penalizer,
                 const float *s_r, const float *s_i, int ncoils, const imp_coo_matrix *c,
-                int tv_num) {
+                int tv_num, imp_tv_pass_hook hook = nullptr, void *hook_user = nullptr) {
     if (!x_r || !x_i || !y_r || !y_i || !kx || !ky || !ix || !iy || !t || num_k <= 0 ||
         num_i <= 0 || ncoils <= 0 || (s_r == nullptr) != (s_i == nullptr))
         return fail(IMP_EINVAL, "imp_bruteForceGpu: bad arguments");
@@ -465,7 +470,7 @@ int bforce_host(float *x_r, float *x_i, const float *y_r, const float *y_i, cons
     IMP_TRY(a[11].up(x_i, num_i));
     IMP_TRY(bruteforce_cg(ctx, num_k, num_i, ncoils, a[0].p, a[1].p, a[2].p, a[3].p, a[4].p,
                           a[5].p, a[6].p, a[7].p, a[8].p, a[9].p, cg_num, fd_penalizer,
-                          sparse_reg_of(reg), tv_num, a[10].p, a[11].p));
+                          sparse_reg_of(reg), tv_num, a[10].p, a[11].p, hook, hook_user));
     IMP_HIP(hipMemcpy(x_r, a[10].p, num_i * sizeof(float), hipMemcpyDeviceToHost));
     IMP_HIP(hipMemcpy(x_i, a[11].p, num_i * sizeof(float), hipMemcpyDeviceToHost));
     return IMP_OK;
@@ -489,4 +494,17 @@ extern "C" int imp_bruteForceGpu_ex(float *x_r, float *x_i, const float *y_r, co
                                     const imp_coo_matrix *c, int enable_tv, int tv_num) {
     return bforce_host(x_r, x_i, y_r, y_i, kx, ky, ix, iy, fm, t, cg_num, num_k, num_i,
                        fd_penalizer, s_r, s_i, ncoils, c, enable_tv ? tv_num : 1);
+}
+
+extern "C" int imp_bruteForceGpu_tv(float *x_r, float *x_i, const float *y_r, const float *y_i,
+                                    const float *kx, const float *ky, const float *ix,
+                                    const float *iy, const float *fm, const float *t,
+                                    int cg_num, int num_k, int num_i, float fd_penalizer,
+                                    const float *s_r, const float *s_i, int ncoils,
+                                    const imp_coo_matrix *c, int enable_tv, int tv_num,
+                                    imp_tv_pass_hook hook, void *hook_user) {
+    // the export runs only inside the TV loop (bruteForceGpu.cu:440-463)
+    return bforce_host(x_r, x_i, y_r, y_i, kx, ky, ix, iy, fm, t, cg_num, num_k, num_i,
+                       fd_penalizer, s_r, s_i, ncoils, c, enable_tv ? tv_num : 1,
+                       enable_tv ? hook : nullptr, hook_user);
 }

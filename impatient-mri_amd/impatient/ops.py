@@ -267,11 +267,15 @@ class SparseReg:
     as 0-based COO (rows, cols, I, J, V)."""
 
     def __init__(self, ctx: Context, num_rows, num_cols, I, J, V):
-        self.ctx = ctx
-        self.rows, self.cols = int(num_rows), int(num_cols)
         I = np.ascontiguousarray(I, np.int32)
         J = np.ascontiguousarray(J, np.int32)
         V = np.ascontiguousarray(V, np.float32)
+        # the library reads len(V) entries from each array
+        if I.ndim != 1 or J.ndim != 1 or V.ndim != 1 or not (I.size == J.size == V.size):
+            raise ValueError(f"SparseReg: I, J, V must be 1-D of one length "
+                             f"(got {I.shape}, {J.shape}, {V.shape})")
+        self.ctx = ctx
+        self.rows, self.cols = int(num_rows), int(num_cols)
         h = C.c_void_p()
         call("imp_sparse_reg_create", ctx.h, C.c_int(self.rows), C.c_int(self.cols),
              C.c_int(len(V)), I.ctypes.data_as(C.c_void_p), J.ctypes.data_as(C.c_void_p),

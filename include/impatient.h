@@ -195,7 +195,8 @@ int imp_toep_set_weight(imp_toep *op, const float *w);
 /* Use lambda C^T W C (a sparse-matrix regularizer, below) as R instead of
  * the finite-difference stencil: the -reg branch of Computeq
  * (toeplitz/recon.cu:1543-1640).  C must be (ndim N) x N, ndim = 2 for
- * N3 = 1 and 3 otherwise; reg is not copied.  Destroying reg while it is
+ * N3 = 1 and 3 otherwise, and created on the operator's context
+ * (IMP_EINVAL otherwise); reg is not copied.  Destroying reg while it is
  * set makes the operator's applies fail (IMP_EINVAL) until it is set again;
  * NULL restores the stencil. */
 struct imp_sparse_reg;
@@ -325,8 +326,8 @@ int imp_bruteforce_cg(imp_ctx *ctx, int num_k, int num_x, int ncoils,
                       int cg_num, float lambda,
                       float *x_r, float *x_i);
 /* The same with the brute force's other options (bruteForceGpu.cu:143-156):
- * reg != NULL: R = C^T W C from a sparse-matrix regularizer (the -reg
- * branch, bruteForceGpu.cu:513-541, 579-629; C is (2 num_x) x num_x);
+ * reg != NULL: R = C^T W C from a sparse-matrix regularizer created on ctx
+ * (the -reg branch, bruteForceGpu.cu:513-541, 579-629; C is (2 num_x) x num_x);
  * tv_num > 1: total-variation passes (bruteForceGpu.cu:431-757), each
  * restarting the CG from the current image with W = 0.5 / sqrt(|D_h x|^2 +
  * |D_v x|^2 + 1e-6) of it (W = 1 in the first pass); tv_num <= 1: one pass. */
@@ -428,6 +429,21 @@ int imp_bruteForceGpu_ex(float *x_r, float *x_i, const float *y_r, const float *
                          int cg_num, int num_k, int num_i, float fd_penalizer,
                          const float *s_r, const float *s_i, int ncoils,
                          const imp_coo_matrix *c, int enable_tv, int tv_num);
+/* The same with bruteForceGpu's enable_tv_update (bruteForceGpu.cu:143-156,
+ * 452-463): with enable_tv, hook (if not NULL) is called on the calling
+ * thread at the start of every TV pass, tv_pass = 0 .. tv_num - 1, with the
+ * image that pass starts from (host arrays of num_x floats, valid during the
+ * call).  The reference writes that image to its output files there; the
+ * mriSolver -tv_update option does the same from the hook. */
+typedef void (*imp_tv_pass_hook)(void *user, int tv_pass, const float *x_r, const float *x_i,
+                                 int num_x);
+int imp_bruteForceGpu_tv(float *x_r, float *x_i, const float *y_r, const float *y_i,
+                         const float *kx, const float *ky, const float *ix,
+                         const float *iy, const float *fm, const float *t,
+                         int cg_num, int num_k, int num_i, float fd_penalizer,
+                         const float *s_r, const float *s_i, int ncoils,
+                         const imp_coo_matrix *c, int enable_tv, int tv_num,
+                         imp_tv_pass_hook hook, void *hook_user);
 
 #ifdef __cplusplus
 }

@@ -303,6 +303,52 @@ def toeplitz_cg(apply, b, max_iter, tol=1e-16, best_iterate=True, vec_dtype=np.c
     return (xbest if best_iterate else x), it, hist
 
 
+def toeplitz_recon(make_apply, b, shape, num_iter, enable_tv=False, tv_num=1, restarts=1,
+                   tol=1e-16, vec_dtype=np.complex128):
+    """The restart / TV / CG loops of toeplitz_recon (toeplitz/recon.cu:2011-2133).
+    make_apply(w) returns v -> A v with the TV weight w (None: W = 1) in the
+    lambda D^T W D term.  x starts at 0 and stays the last iterate across
+    passes; the pass tv_i > 0 first sets w = tv_weight(x) (recon.cu:2022-2034);
+    each pass recomputes r = b - A x; on restart 0 the pass resets the best
+    residual and beta2 to |r|^2 (recon.cu:2043-2048), later restarts carry
+    both; reconBest (the output, :2107-2110) is replaced by every iterate
+    whose |r|^2 is below the best.  Without TV only the first pass runs.
+    Returns (reconBest, per-pass residual histories)."""
+    st, dot, sd = _vec_ops(vec_dtype)
+    b = st(np.asarray(b).reshape(-1))
+    normb = dot(b, b)
+    x = st(np.zeros_like(b))
+    best_x = x.copy()
+    res_best = beta2 = None
+    w = None
+    hists = []
+    for restart in range(restarts):
+        for tv_i in range(tv_num if enable_tv else 1):
+            if tv_i != 0:
+                w = tv_weight(x.reshape(shape))
+            A = make_apply(w)
+            r = st(b - st(A(x)))
+            if restart == 0:
+                res_best = beta2 = dot(r, r)
+            p = r.copy()
+            hist = []
+            for _ in range(num_iter):
+                q = st(A(p))
+                beta = beta2
+                alpha = sd(beta / dot(p, q))
+                x = st(x + alpha * p)
+                r = st(r - alpha * q)
+                beta2 = dot(r, r)
+                p = st(r + sd(beta2 / beta) * p)
+                hist.append(np.sqrt(beta2 / normb))
+                if beta2 < res_best:
+                    res_best, best_x = beta2, x.copy()
+                if np.sqrt(beta2 / normb) < tol:
+                    break
+            hists.append(hist)
+    return best_x, hists
+
+
 def chan_circulant(t, dims):
     """T. Chan's optimal circulant of a multilevel Toeplitz matrix whose
     kernel t is given on the 2N grid (axis d index m = delta + N_d, delta in

@@ -172,3 +172,89 @@ def test_cli_bruteforce_tv(data_dir, ctx):
                             None, 5, 0.01, x0, tv_num=3)
     assert np.max(np.abs(xr - ref.real)) <= 5e-7 + 1e-6 * np.abs(ref.real).max()
     assert np.max(np.abs(xi - ref.imag)) <= 5e-7 + 1e-6 * np.abs(ref.imag).max()
+
+
+def _cli(args):
+    r = subprocess.run([str(CLI), *args], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-800:], r.stderr[-800:])
+    return r
+
+
+def test_cli_option_implications(tmp_path, data_dir):
+    """As main.cpp:292-322: '-fdp x' alone enables the finite-difference term,
+    '-tv_num n' and '-tv_update' enable TV; the images equal those of the
+    explicit flags, on both drivers."""
+    runs = {}
+    for name, extra in (("fdp", ["-fdp", "0.01"]), ("fd_fdp", ["-fd", "-fdp", "0.01"]),
+                        ("none", []), ("tvnum", ["-tv_num", "3", "-fdp", "0.01"]),
+                        ("tv_tvnum", ["-tv", "-tv_num", "3", "-fdp", "0.01"])):
+        d = tmp_path / name
+        shutil.copytree(data_dir, d)
+        assert _cli(["-idir", str(d), "-odir", str(d / "o"), "-cg_num", "5", *extra]).returncode == 0
+        runs[name] = np.loadtxt(d / "o" / "output_gpu_r.dat")
+        if name in ("fdp", "fd_fdp"):
+            assert _cli(["-idir", str(d), "-toeplitzDirect", "-ntime_segs", "3", "-cg_num", "5",
+                         *extra]).returncode == 0
+            runs[name + "_toep"] = np.fromfile(d / "out.file", dtype=np.float32)
+    assert np.array_equal(runs["fdp"], runs["fd_fdp"])
+    assert np.array_equal(runs["fdp_toep"], runs["fd_fdp_toep"])
+    assert np.array_equal(runs["tvnum"], runs["tv_tvnum"])
+    # the weight does act: without it the solve is unregularized
+    assert rel_l2(runs["fdp"], runs["none"]) > 1e-4
+    # bad values are rejected as in the reference
+    assert _cli(["-idir", str(data_dir), "-tv_num", "0", "-fdp", "0.01"]).returncode != 0
+    assert _cli(["-idir", str(data_dir), "-tv_num", "-fdp", "0.01"]).returncode != 0
+
+
+def test_cli_bruteforce_tv_update(tmp_path, data_dir):
+    """-tv_update exports the image before every TV pass; the files left at
+    the end hold the final image, equal to the run without it."""
+    a, b = tmp_path / "a", tmp_path / "b"
+    shutil.copytree(data_dir, a)
+    shutil.copytree(data_dir, b)
+    common = ["-cg_num", "4", "-tv_num", "3", "-fdp", "0.01"]
+    assert _cli(["-idir", str(a), "-odir", str(a / "o"), *common, "-tv_update"]).returncode == 0
+    assert _cli(["-idir", str(b), "-odir", str(b / "o"), *common]).returncode == 0
+    for part in ("r", "i"):
+        assert np.array_equal(np.loadtxt(a / "o" / f"output_gpu_{part}.dat"),
+                              np.loadtxt(b / "o" / f"output_gpu_{part}.dat"))
+
+
+def test_bruteforce_tv_pass_hook(data_dir, ctx):
+    """imp_bruteForceGpu_tv calls the hook before every TV pass with the image
+    that pass starts from: the initial estimate, then each earlier pass count's
+    result (checked against runs with tv_num = 1 and 2)."""
+    from impatient import _lib, files, ops
+    lib = _lib.load()
+    rd = lambda n: files.read(data_dir / f"{n}.dat")
+    keep = {n: np.ascontiguousarray(rd(n), np.float32)
+            for n in ("kdata_r", "kdata_i", "kx", "ky", "ix", "iy", "fm", "t")}
+    seen = []
+    HOOK = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                       C.c_int)
+
+    def on_pass(user, tv_pass, xr, xi, n):
+        seen.append((tv_pass, np.ctypeslib.as_array(xr, (n,)).copy()
+                     + 1j * np.ctypeslib.as_array(xi, (n,)).copy()))
+    hook = HOOK(on_pass)
+    ni = keep["ix"].size
+    x0 = rd("idata_r") + 1j * rd("idata_i")
+    xr = np.ascontiguousarray(x0.real, np.float32)
+    xi = np.ascontiguousarray(x0.imag, np.float32)
+    P = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    lib.imp_bruteForceGpu_tv.argtypes = None
+    _lib.check(lib.imp_bruteForceGpu_tv(
+        P(xr), P(xi), P(keep["kdata_r"]), P(keep["kdata_i"]), P(keep["kx"]), P(keep["ky"]),
+        P(keep["ix"]), P(keep["iy"]), P(keep["fm"]), P(keep["t"]), C.c_int(4),
+        C.c_int(keep["kx"].size), C.c_int(ni), C.c_float(0.01), None, None, C.c_int(1), None,
+        C.c_int(1), C.c_int(3), hook, None), "imp_bruteForceGpu_tv")
+    assert [s[0] for s in seen] == [0, 1, 2]
+    assert np.array_equal(seen[0][1], x0.astype(np.complex64))
+    d = (keep["kdata_r"] + 1j * keep["kdata_i"])[None, :]
+    for k in (1, 2):
+        ref = ops.bruteforce_cg(ctx, keep["kx"], keep["ky"], keep["t"], keep["ix"], keep["iy"],
+                                keep["fm"], d, None, 4, 0.01, x0, tv_num=k)
+        assert np.array_equal(seen[k][1], ref.astype(np.complex64)), k
+    final = ops.bruteforce_cg(ctx, keep["kx"], keep["ky"], keep["t"], keep["ix"], keep["iy"],
+                              keep["fm"], d, None, 4, 0.01, x0, tv_num=3)
+    assert np.array_equal(xr + 1j * xi, final.astype(np.complex64))

@@ -574,3 +574,66 @@ def test_toeplitz_operator_equals_apply(shape):
     ref2 = orc.toeplitz_apply(v2, q, 1e-3, 2e-4, fm, sens, 0.3)
     got2 = A(v2.reshape(-1)).reshape(img)
     assert np.linalg.norm(got2 - ref2) / np.linalg.norm(ref2) < 1e-12
+
+
+def _dense_fd(shape, w):
+    """Dense sum_d D_d^T diag(w) D_d with periodic first differences."""
+    blocks = []
+    for ax, na in enumerate(shape):
+        if na == 1:
+            continue
+        mats = [np.eye(s) for s in shape]
+        mats[ax] = _periodic_D(na)
+        K = mats[0]
+        for m_ in mats[1:]:
+            K = np.kron(K, m_)
+        blocks.append(K)
+    return sum(B.T @ np.diag(np.asarray(w).reshape(-1)) @ B for B in blocks)
+
+
+@pytest.mark.parametrize("shape", [(5, 6), (3, 4, 5)])
+def test_toeplitz_recon_tv_passes_solve_reweighted_systems(shape):
+    """Converged TV passes of toeplitz_recon: pass k solves
+    (G + lam D^T W(x_{k-1}) D) x_k = b with W = 0.5/sqrt(sum |D x|^2 + 1e-6)
+    of the previous pass's image (W = 1 in pass 0), checked against dense
+    solves; dropping the reweighting changes the answer."""
+    orc = _orc()
+    n = int(np.prod(shape))
+    G = _spd(n, 31)
+    lam = 40.0
+    rng = np.random.default_rng(32)
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+
+    def make_apply(w):
+        return lambda v: G @ v + orc.fd_reg(np.asarray(v).reshape(shape), lam, w).reshape(-1)
+    x, hists = orc.toeplitz_recon(make_apply, b, shape, 400, enable_tv=True, tv_num=3,
+                                  tol=1e-14)
+    assert len(hists) == 3
+    xk = np.linalg.solve(G + lam * _dense_fd(shape, np.ones(n)), b)
+    for _ in range(2):
+        w = orc.tv_weight(xk.reshape(shape))
+        xk = np.linalg.solve(G + lam * _dense_fd(shape, w), b)
+    x1 = np.linalg.solve(G + lam * _dense_fd(shape, np.ones(n)), b)
+    assert rel_l2(x, xk) < 1e-9
+    assert rel_l2(x1, xk) > 1e-3
+    # without TV only one pass runs and it is the plain CG of toeplitz_cg
+    x0, h0 = orc.toeplitz_recon(make_apply, b, shape, 7, enable_tv=False, tv_num=3)
+    xc, _, _ = orc.toeplitz_cg(make_apply(None), b, 7)
+    assert len(h0) == 1 and rel_l2(x0, xc) < 1e-12
+
+
+def test_toeplitz_recon_restarts_carry_the_best_residual():
+    """Restarts after the first keep reconBest unless an iterate beats every
+    earlier one: the returned image has the smallest residual of the whole run
+    (restart 0's passes each start their own best, recon.cu:2043-2048)."""
+    orc = _orc()
+    n = 24
+    G = _spd(n, 33)
+    b = np.linspace(1, 2, n) + 0.5j
+    A = lambda v: G @ v
+    x, hists = orc.toeplitz_recon(lambda w: A, b, (4, 6), 3, restarts=3)
+    # candidates: the zero start (relative residual 1) and every iterate
+    allres = np.concatenate([[1.0]] + [np.asarray(h) for h in hists])
+    true = np.linalg.norm(b - A(x)) / np.linalg.norm(b)
+    assert len(hists) == 3
+    assert abs(true - allres.min()) < 1e-9 * allres.min()
