@@ -6,7 +6,7 @@
 //   mriSolver -idir D [-odir O] [-gpu_id n] [-cg_num K]
 //             [-toeplitzDirect | -toeplitzGridding] [-ntime_segs L]
 //             [-writeQ] [-reuseQ dir] [-fd] [-fdp lambda] [-tv] [-tv_num n]
-//             [-reg] [-cpu] [-nogpu] [-mgpu] [-gridOS_Q f] [-gridOS_FHD f]
+//             [-reg] [-cpu] [-nogpu] [-mgpu] [-gridOS_Q f] [-gridOS_FH f]
 //             [-version] [-help]
 #include <sys/stat.h>
 
@@ -46,7 +46,8 @@ void usage(FILE *f) {
                  "  -gpu_id <n>          GPU device (default 0)\n"
                  "  -cg_num <n>          CG iterations (default 8)\n"
                  "  -toeplitzDirect      Toeplitz method, direct Q and F^H d\n"
-                 "  -toeplitzGridding    Toeplitz method with gridding (not supported)\n"
+                 "  -toeplitzGridding    Toeplitz method, Q and F^H d by gridding\n"
+                 "  -gridOS_Q f / -gridOS_FH f  gridding oversampling in [1, 2] (default 1.125 / 1.5)\n"
                  "  -ntime_segs <L>      time segments (default 12), after the method flag\n"
                  "  -writeQ              write <idir>/Q_stone.file\n"
                  "  -reuseQ <dir>        read <dir>/Q_stone.file instead of computing Q\n"
@@ -97,12 +98,26 @@ int parse(int argc, char **argv, Options &o) {
             }
             if (!need_arg(i, argc, a)) return 1;
             o.ntime_segs = (float)std::atof(argv[++i]);
-        } else if (!std::strcmp(a, "-gridOS_Q")) {
-            if (!need_arg(i, argc, a)) return 1;
-            o.gridOS_Q = (float)std::atof(argv[++i]);
-        } else if (!std::strcmp(a, "-gridOS_FHD")) {
-            if (!need_arg(i, argc, a)) return 1;
-            o.gridOS_FHD = (float)std::atof(argv[++i]);
+            if (o.ntime_segs < 1.0f) {
+                std::fprintf(stderr, "\nError: number of time segments has to be a natural "
+                                     "number in floating format.\n");
+                return 1;
+            }
+        } else if (!std::strcmp(a, "-gridOS_Q") || !std::strcmp(a, "-gridOS_FH")) {
+            // gridding oversampling of Q / of F^H d, after -toeplitzGridding,
+            // in [1, 2] (main.cpp:256-290)
+            if (!o.toeplitz_gridding || !need_arg(i, argc, a) || argv[i + 1][0] == '-') {
+                std::fprintf(stderr, "\nError: -toeplitzGridding flag is not found or invalid "
+                                     "gridding OS factor.\n");
+                return 1;
+            }
+            const float v = (float)std::atof(argv[++i]);
+            if (v < 1.0f || v > 2.0f) {
+                std::fprintf(stderr, "\nError: gridding oversampling factor should be in "
+                                     "[1.0,2.0].\n");
+                return 1;
+            }
+            (a[8] == 'Q' ? o.gridOS_Q : o.gridOS_FHD) = v;
         } else if (!std::strcmp(a, "-reuseQ")) {
             if (!o.toeplitz_direct && !o.toeplitz_gridding) {
                 std::fprintf(stderr, "Error: -reuseQ must follow the Toeplitz method flag\n");
@@ -207,10 +222,8 @@ int write_like(const std::string &path, float version, int nx, int ny, int nz, i
 
 // The Toeplitz driver of main_mri.cu:207-630.
 int run_toeplitz(const Options &o) {
-    if (o.toeplitz_gridding && !o.toeplitz_direct) {
-        std::fprintf(stderr, "Error: -toeplitzGridding is not supported by this engine\n");
-        return IMP_EUNSUPPORTED;
-    }
+    // direct evaluation when -toeplitzDirect is given, else gridding (main_mri.cu:207-630)
+    const int direct = o.toeplitz_direct ? 1 : 0, grid = direct ? 0 : 1;
     Arr kx, ix, iy, kd;
     if (load(o.idir + "/kx.dat", kx) || load(o.idir + "/ix.dat", ix) ||
         load(o.idir + "/iy.dat", iy) || load(o.idir + "/kdata_r.dat", kd))
@@ -289,7 +302,7 @@ int run_toeplitz(const Options &o) {
         std::printf("======== Step 1. Compute Q matrices: Q[%d] ========\n", L + 1);
         const std::string qfile = o.idir + "/Q_stone.file";
         if (report(imp_toeplitz_computeQ_GPU(o.idir.c_str(), o.ntime_segs, Nx, Ny, Nz,
-                                             qfile.c_str(), pr.data(), pi.data(), 1, 0,
+                                             qfile.c_str(), pr.data(), pi.data(), direct, grid,
                                              o.gridOS_Q, o.writeQ ? 1 : 0),
                    "computeQ"))
             return IMP_EHIP;
@@ -298,7 +311,7 @@ int run_toeplitz(const Options &o) {
     std::vector<float> fr(num_i), fi(num_i);
     const std::string fhd = o.idir + "/FhD.file";
     if (report(imp_toeplitz_computeFH_GPU(o.idir.c_str(), o.ntime_segs, Nx, Ny, Nz, fhd.c_str(),
-                                          fr.data(), fi.data(), 1, 0, o.gridOS_FHD),
+                                          fr.data(), fi.data(), direct, grid, o.gridOS_FHD),
                "computeFH"))
         return IMP_EHIP;
     std::printf("======== Step 3. Start CG solver ========\n");

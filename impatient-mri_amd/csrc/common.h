@@ -73,6 +73,7 @@ enum WsSlot {
     WS_QS_PART,    // separable Q: K-block partial sums and job table
     WS_STACK,      // stacked-trajectory detection flags
     WS_QSTACK,     // stacked Q: one partition's 2D planes and the z factor
+    WS_GRID,       // gridding: binned samples, grids and line-transform tables
     WS_COUNT
 };
 

@@ -97,6 +97,15 @@ int toeplitz_q(imp_ctx *ctx, int num_k, const float *kx, const float *ky,
                const float *gx, const float *gy, const float *gz, int ntime_segs,
                int flags, const float *t_host_first_last, float *q_r, float *q_i);
 
+// gridding path (grid.hip): Q_l on the standard 2x grid, F^H d on [Ny][Nx][Nz]
+int gridding_q(imp_ctx *ctx, int num_k, const float *kx, const float *ky, const float *kz,
+               const float *t, int Nx, int Ny, int Nz, int ntime_segs, float gridOS, float *q_r,
+               float *q_i);
+int gridding_fhd(imp_ctx *ctx, int num_k, int ncoils, const float *kx, const float *ky,
+                 const float *kz, const float *t, int Nx, int Ny, int Nz, const float *fm,
+                 const float *d_r, const float *d_i, const float *s_r, const float *s_i,
+                 int ntime_segs, float gridOS, float *out_r, float *out_i);
+
 int bruteforce_cg(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *kx,
                   const float *ky, const float *t, const float *x, const float *y,
                   const float *fm, const float *d_r, const float *d_i, const float *s_r,

@@ -87,6 +87,164 @@ void segment_timing(const std::vector<float> &t, int nk, float ntime_segs, float
     t0 = t[0];
 }
 
+// 2D arrays in the reference's y-fastest file order ([x][y], the MATLAB
+// meshgrid order of mriData/64x64x1) <-> the engine's [y][x] (x fastest).
+// `to_engine` converts file order to engine order; planes of n0 x n1 with
+// n0 = Nx, n1 = Ny.
+void transpose_plane(std::vector<float> &v, size_t off, int Nx, int Ny, bool to_engine) {
+    std::vector<float> tmp(v.begin() + off, v.begin() + off + (size_t)Nx * Ny);
+    for (int y = 0; y < Ny; ++y)
+        for (int x = 0; x < Nx; ++x) {
+            const size_t e = (size_t)y * Nx + x, f = (size_t)x * Ny + y;
+            v[off + (to_engine ? e : f)] = tmp[to_engine ? f : e];
+        }
+}
+
+// the samples must lie in the window [-N/2, N/2] on every axis gridded
+// (computeQ.cmem.cu:527-534)
+int check_window(const HostArray &kx, const HostArray &ky, const HostArray &kz, int nk, int Nx,
+                 int Ny, int Nz) {
+    for (int i = 0; i < nk; ++i)
+        if (std::fabs(kx.v[i]) > Nx / 2.0f || std::fabs(ky.v[i]) > Ny / 2.0f ||
+            (Nz > 1 && std::fabs(kz.v[i]) > Nz / 2.0f))
+            return fail(IMP_EINVAL, "gridding: k-space sample %d outside the window [-N/2, N/2]",
+                        i);
+    return IMP_OK;
+}
+
+// the -toeplitzGridding branch of toeplitz_computeQ_GPU (computeQ.cmem.cu:401-754)
+int computeQ_gridding(const char *dir, float ntime_segs, int Nx, int Ny, int Nz,
+                      const char *Q_file, float **Qr, float **Qi, float gridOS, int writeQ) {
+    const std::string d(dir);
+    HostArray kx, ky, kz, t, gx, gy;
+    IMP_TRY(read_file(d + "/kx.dat", kx));
+    IMP_TRY(read_file(d + "/ky.dat", ky));
+    IMP_TRY(read_file(d + "/kz.dat", kz));
+    IMP_TRY(read_file(d + "/t.dat", t));
+    const int nk = kx.info.per_coil;
+    if (nk <= 0 || Nx <= 0 || Ny <= 0 || Nz <= 0 || (int)ky.v.size() < nk ||
+        (int)kz.v.size() < nk || (int)t.v.size() < nk)
+        return fail(IMP_EFORMAT, "imp_toeplitz_computeQ_GPU: inconsistent file sizes in %s", dir);
+    IMP_TRY(check_window(kx, ky, kz, nk, Nx, Ny, Nz));
+    // the Q-grid order of the caller's coordinate files (H2): 2D grids whose
+    // x is constant over the first entries hold y fastest
+    bool y_fast = false;
+    if (Nz == 1 && file_exists(d + "/ixQ.dat") && file_exists(d + "/iyQ.dat")) {
+        IMP_TRY(read_file(d + "/ixQ.dat", gx));
+        IMP_TRY(read_file(d + "/iyQ.dat", gy));
+        y_fast = gx.v.size() > 1 && gx.v[1] == gx.v[0] && gy.v[1] != gy.v[0];
+    }
+    imp_ctx *ctx = nullptr;
+    IMP_TRY(default_ctx(&ctx));
+    const int L = (int)ntime_segs;
+    const long nseg = L + 1;
+    const long nq = 4L * Nx * Ny * (Nz > 1 ? 2L * Nz : 1L);
+    Dev dkx, dky, dkz, dt, dq;
+    IMP_TRY(dkx.up(kx.v, nk));
+    IMP_TRY(dky.up(ky.v, nk));
+    IMP_TRY(dkz.up(kz.v, nk));
+    IMP_TRY(dt.up(t.v, nk));
+    IMP_TRY(dq.alloc(2 * nseg * nq));
+    IMP_TRY(gridding_q(ctx, nk, dkx.p, dky.p, Nz > 1 ? dkz.p : nullptr, dt.p, Nx, Ny, Nz, L,
+                       gridOS, dq.p, dq.p + nseg * nq));
+    IMP_HIP(hipStreamSynchronize(ctx->stream));
+    std::vector<float> h(2 * nseg * nq);
+    IMP_HIP(hipMemcpy(h.data(), dq.p, h.size() * sizeof(float), hipMemcpyDeviceToHost));
+    for (long l = 0; l < nseg; ++l) {
+        if (!Qr[l] || !Qi[l]) return fail(IMP_EINVAL, "imp_toeplitz_computeQ_GPU: Qr[%ld] NULL", l);
+        if (y_fast) {
+            transpose_plane(h, l * nq, 2 * Nx, 2 * Ny, false);
+            transpose_plane(h, (nseg + l) * nq, 2 * Nx, 2 * Ny, false);
+        }
+        std::memcpy(Qr[l], h.data() + l * nq, nq * sizeof(float));
+        std::memcpy(Qi[l], h.data() + (nseg + l) * nq, nq * sizeof(float));
+    }
+    if (writeQ && Q_file) {
+        // (L+1) x [Re(Q_l), Im(Q_l)] float32 (computeQ.cmem.cu:280-300)
+        FILE *f = std::fopen(Q_file, "wb");
+        if (!f) return fail(IMP_EIO, "cannot open %s for writing", Q_file);
+        for (long l = 0; l < nseg; ++l)
+            if (std::fwrite(Qr[l], sizeof(float), nq, f) != (size_t)nq ||
+                std::fwrite(Qi[l], sizeof(float), nq, f) != (size_t)nq) {
+                std::fclose(f);
+                return fail(IMP_EIO, "%s: write failed", Q_file);
+            }
+        std::fclose(f);
+    }
+    return IMP_OK;
+}
+
+// the -toeplitzGridding branch of toeplitz_computeFH_GPU
+// (parcomputeAH_GPU_Grid, computeFH.cmem.cu:1032-1117, 1330-1336)
+int computeFH_gridding(const char *dir, float ntime_segs, int Nx, int Ny, int Nz,
+                       const char *FhD_file, float *outR, float *outI, float gridOS) {
+    const std::string d(dir);
+    HostArray kx, ky, kz, t, ix, iy, fm, sr, si, dr, di;
+    IMP_TRY(read_file(d + "/kx.dat", kx));
+    IMP_TRY(read_file(d + "/ky.dat", ky));
+    IMP_TRY(read_file(d + "/kz.dat", kz));
+    IMP_TRY(read_file(d + "/t.dat", t));
+    IMP_TRY(read_file(d + "/ix.dat", ix));
+    IMP_TRY(read_file(d + "/iy.dat", iy));
+    IMP_TRY(read_file(d + "/fm.dat", fm));
+    IMP_TRY(read_file(d + "/kdata_r.dat", dr));
+    IMP_TRY(read_file(d + "/kdata_i.dat", di));
+    const int nk = kx.info.per_coil;
+    const long nv = (long)Nx * Ny * Nz;
+    const int nc = dr.info.ncoils > 0 ? dr.info.ncoils : 1;
+    if (nk <= 0 || nv <= 0 || (long)ix.info.per_coil != nv || (long)fm.v.size() < nv ||
+        (long)dr.v.size() < (long)nk * nc || (long)di.v.size() < (long)nk * nc ||
+        (int)t.v.size() < nk || (int)kz.v.size() < nk)
+        return fail(IMP_EFORMAT, "imp_toeplitz_computeFH_GPU: inconsistent file sizes in %s", dir);
+    IMP_TRY(check_window(kx, ky, kz, nk, Nx, Ny, Nz));
+    const bool sense = nc > 1;   // single-coil data: S = 1 (computeFH.cmem.cu:305-310)
+    if (sense) {
+        IMP_TRY(read_file(d + "/sensi_r.dat", sr));
+        IMP_TRY(read_file(d + "/sensi_i.dat", si));
+        if ((long)sr.v.size() < nv * nc || (long)si.v.size() < nv * nc)
+            return fail(IMP_EFORMAT, "imp_toeplitz_computeFH_GPU: sensitivity files too short");
+    }
+    // image order of the coordinate files (H2): the gridded image is [y][x]
+    const bool y_fast = Nz == 1 && nv > 1 && ix.v[1] == ix.v[0] && iy.v[1] != iy.v[0];
+    if (y_fast) {
+        transpose_plane(fm.v, 0, Nx, Ny, true);
+        for (int c = 0; sense && c < nc; ++c) {
+            transpose_plane(sr.v, (size_t)c * nv, Nx, Ny, true);
+            transpose_plane(si.v, (size_t)c * nv, Nx, Ny, true);
+        }
+    }
+    imp_ctx *ctx = nullptr;
+    IMP_TRY(default_ctx(&ctx));
+    Dev a[10];
+    IMP_TRY(a[0].up(kx.v, nk));
+    IMP_TRY(a[1].up(ky.v, nk));
+    IMP_TRY(a[2].up(kz.v, nk));
+    IMP_TRY(a[3].up(t.v, nk));
+    IMP_TRY(a[4].up(fm.v, nv));
+    IMP_TRY(a[5].up(dr.v, (size_t)nk * nc));
+    IMP_TRY(a[6].up(di.v, (size_t)nk * nc));
+    if (sense) {
+        IMP_TRY(a[7].up(sr.v, (size_t)nv * nc));
+        IMP_TRY(a[8].up(si.v, (size_t)nv * nc));
+    }
+    IMP_TRY(a[9].alloc(2 * (size_t)nv));
+    IMP_TRY(gridding_fhd(ctx, nk, nc, a[0].p, a[1].p, Nz > 1 ? a[2].p : nullptr, a[3].p, Nx, Ny,
+                         Nz, a[4].p, a[5].p, a[6].p, a[7].p, a[8].p, (int)ntime_segs, gridOS,
+                         a[9].p, a[9].p + nv));
+    IMP_HIP(hipStreamSynchronize(ctx->stream));
+    std::vector<float> h(2 * nv);
+    IMP_HIP(hipMemcpy(h.data(), a[9].p, h.size() * sizeof(float), hipMemcpyDeviceToHost));
+    if (y_fast) {
+        transpose_plane(h, 0, Nx, Ny, false);
+        transpose_plane(h, nv, Nx, Ny, false);
+    }
+    std::memcpy(outR, h.data(), nv * sizeof(float));
+    std::memcpy(outI, h.data() + nv, nv * sizeof(float));
+    // FhD.file: Re then Im float32 (computeFH.cmem.cu:313-321)
+    if (FhD_file) IMP_TRY(write_raw(FhD_file, {outR, outI}, nv));
+    return IMP_OK;
+}
+
 }  // namespace
 
 extern "C" int imp_set_compat(int flags) {
@@ -109,11 +267,12 @@ extern "C" int imp_toeplitz_computeQ_GPU(const char *dir, float ntime_segs, int 
                                          int Nz, const char *Q_file, float **Qr, float **Qi,
                                          int enable_direct, int enable_gridding, float gridOS,
                                          int enable_writeQ) {
-    (void)gridOS;
     if (!dir || !Qr || !Qi || ntime_segs < 0)
         return fail(IMP_EINVAL, "imp_toeplitz_computeQ_GPU: bad arguments");
-    if (!enable_direct || enable_gridding)
-        return fail(IMP_EUNSUPPORTED, "imp_toeplitz_computeQ_GPU: only direct evaluation");
+    if (!enable_direct && !enable_gridding)
+        return fail(IMP_EINVAL, "imp_toeplitz_computeQ_GPU: neither direct nor gridding");
+    if (!enable_direct) return computeQ_gridding(dir, ntime_segs, Nx, Ny, Nz, Q_file, Qr, Qi,
+                                                 gridOS, enable_writeQ);
     const std::string d(dir);
     HostArray kx, ky, kz, t, gx, gy, gz, pr, pi;
     IMP_TRY(read_file(d + "/kx.dat", kx));
@@ -186,14 +345,12 @@ extern "C" int imp_toeplitz_computeFH_GPU(const char *dir, float ntime_segs, int
                                           int Nz, const char *FhD_file, float *outR,
                                           float *outI, int enable_direct, int enable_gridding,
                                           float gridOS) {
-    (void)gridOS;
-    (void)ntime_segs;
-    (void)Nx;
-    (void)Ny;
     if (!dir || !outR || !outI)
         return fail(IMP_EINVAL, "imp_toeplitz_computeFH_GPU: bad arguments");
-    if (!enable_direct || enable_gridding)
-        return fail(IMP_EUNSUPPORTED, "imp_toeplitz_computeFH_GPU: only direct evaluation");
+    if (!enable_direct && !enable_gridding)
+        return fail(IMP_EINVAL, "imp_toeplitz_computeFH_GPU: neither direct nor gridding");
+    if (!enable_direct)
+        return computeFH_gridding(dir, ntime_segs, Nx, Ny, Nz, FhD_file, outR, outI, gridOS);
     const std::string d(dir);
     HostArray kx, ky, kz, t, ix, iy, iz, fm, pr, pi, sr, si, dr, di;
     IMP_TRY(read_file(d + "/kx.dat", kx));

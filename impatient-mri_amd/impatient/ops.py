@@ -129,6 +129,38 @@ def toeplitz_q(ctx: Context, kx, ky, kz, t, Nx, Ny, Nz, ntime_segs, phi=None, fl
     return _cplx(qr, qi, (nseg, nq))
 
 
+def toeplitz_q_gridding(ctx: Context, kx, ky, kz, t, Nx, Ny, Nz, ntime_segs, gridOS):
+    """Q_l by Kaiser-Bessel gridding on the standard 2x grid; [(L+1), G]."""
+    nk = len(kx)
+    a = [ctx.to_device(_f32(v)) for v in (kx, ky, t)]
+    akz = _opt(ctx, kz)
+    nq = 4 * Nx * Ny * (2 * Nz if Nz > 1 else 1)
+    nseg = ntime_segs + 1
+    qr, qi = _pair(ctx, nseg * nq)
+    call("imp_toeplitz_q_gridding", ctx.h, C.c_int(nk), ptr(a[0]), ptr(a[1]), ptr(akz),
+         ptr(a[2]), C.c_int(Nx), C.c_int(Ny), C.c_int(Nz), C.c_int(ntime_segs),
+         C.c_float(gridOS), ptr(qr), ptr(qi))
+    return _cplx(qr, qi, (nseg, nq))
+
+
+def fhd_gridding(ctx: Context, kx, ky, kz, t, Nx, Ny, Nz, fm, d, sens, ntime_segs, gridOS):
+    """F^H d by gridding with time segmentation; d is [C, M]; returns [N]."""
+    d = np.atleast_2d(np.asarray(d))
+    ncoils, nk = d.shape
+    a = [ctx.to_device(_f32(v)) for v in (kx, ky, t)]
+    akz, afm = _opt(ctx, kz), _opt(ctx, fm)
+    dr, di = _split(ctx, d.reshape(-1))
+    sr = si = None
+    if sens is not None:
+        sr, si = _split(ctx, np.asarray(sens).reshape(-1))
+    nv = Nx * Ny * Nz
+    orr, oi = _pair(ctx, nv)
+    call("imp_fhd_gridding", ctx.h, C.c_int(nk), C.c_int(ncoils), ptr(a[0]), ptr(a[1]),
+         ptr(akz), ptr(a[2]), C.c_int(Nx), C.c_int(Ny), C.c_int(Nz), ptr(afm), ptr(dr),
+         ptr(di), ptr(sr), ptr(si), C.c_int(ntime_segs), C.c_float(gridOS), ptr(orr), ptr(oi))
+    return _cplx(orr, oi)
+
+
 def fd_reg(ctx: Context, p, lam, w=None):
     """lambda * sum_d D_d^T W D_d p for p of shape (N1, N2) or (N1, N2, N3)."""
     p = np.asarray(p)

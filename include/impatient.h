@@ -158,6 +158,30 @@ int imp_toeplitz_q(imp_ctx *ctx, int num_k,
                    int ntime_segs, int flags,
                    float *q_r, float *q_i);
 
+/* Q_l by Kaiser-Bessel gridding (the -toeplitzGridding path:
+ * computeQ_GPU_Grid, toeplitz/computeQ.cmem.cu:401-754, called per l from
+ * toeplitz_computeQ_GPU): the samples at 2k, weights a_l(t) (phi = 1), on
+ * the 2N grid of the standard Q layout (as imp_toeplitz_q with NULL
+ * coordinates), q_r/q_i [(ntime_segs+1) * 4NxNy*M3].  Kernel width 4,
+ * Beatty's beta, oversampling gridOS rounded as the reference rounds it (x/y
+ * grid even in 2D, z grid a multiple of 4 in 3D), no wrap at the grid edge,
+ * deapodization by the kernel's transform.  kz may be NULL in 2D. */
+int imp_toeplitz_q_gridding(imp_ctx *ctx, int num_k, const float *kx, const float *ky,
+                            const float *kz, const float *t, int Nx, int Ny, int Nz,
+                            int ntime_segs, float gridOS, float *q_r, float *q_i);
+
+/* F^H d by gridding with time segmentation (parcomputeAH_GPU_Grid /
+ * computeAH_GPU_Grid / computeFH_GPU_Grid, toeplitz/computeFH.cmem.cu:470-802,
+ * 1032-1117):
+ *   out = sum_c conj(S_c) sum_l e^{+i fm (t0 + l tau)} grid(a_l(t) d_c)
+ * on the image grid x = (i - N/2)/N, out [Ny][Nx][Nz] (x fastest in 2D, z in
+ * 3D); d_r/d_i [ncoils*num_k] coil-major; s_r/s_i [ncoils*Nx*Ny*Nz] or
+ * NULL (S = 1); fm [Nx*Ny*Nz] or NULL (0). */
+int imp_fhd_gridding(imp_ctx *ctx, int num_k, int ncoils, const float *kx, const float *ky,
+                     const float *kz, const float *t, int Nx, int Ny, int Nz, const float *fm,
+                     const float *d_r, const float *d_i, const float *s_r, const float *s_i,
+                     int ntime_segs, float gridOS, float *out_r, float *out_i);
+
 /* Periodic first-difference regularizer (DHWD2dGpu / DHWD3dGpu,
  * toeplitz/DHWD2dGpu_toeplitz.cu:132-387, toeplitz/DHWD3dGpu_toeplitz.cu:170-580):
  *   s = lambda * sum_d D_d^T W D_d p,  (D_d p)(i) = p(i) - p(i + e_d mod N_d)

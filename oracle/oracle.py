@@ -303,6 +303,170 @@ def toeplitz_cg(apply, b, max_iter, tol=1e-16, best_iterate=True, vec_dtype=np.c
     return (xbest if best_iterate else x), it, hist
 
 
+GRID_KERNEL_WIDTH = 4.0   # kernelWidth (computeQ.cmem.cu:455, computeFH.cmem.cu:495)
+
+
+def gridding_params(img, gridOS):
+    """Grid set-up of computeQ_GPU_Grid / computeFH_GPU_Grid
+    (toeplitz/computeQ.cmem.cu:433-498, computeFH.cmem.cu:495-543) for a
+    transform of image size img = (nx, ny, nz) (nz = 1 in 2D): gridOS is
+    rounded in float32 so the grid is even in x/y (2D) or a multiple of 4 in
+    z (3D); beta is Beatty's (Eqn. 5; the GPU path, :456-463); grid sizes
+    are ceil(gridOS n), x and y bumped to even.  Returns (os, beta, grid)."""
+    f = np.float32
+    nx, ny, nz = (int(v) for v in img)
+    os_ = f(gridOS)
+    if nz == 1:
+        os_ = f(f(2.0) * f(np.ceil(f(os_ * f(nx)) / f(2.0))) / f(nx))
+    else:
+        os_ = f(f(4.0) * f(np.ceil(f(os_ * f(nz)) / f(4.0))) / f(nz))
+    W = GRID_KERNEL_WIDTH
+    o = float(os_)
+    beta = np.pi * np.sqrt((o - 0.5) ** 2 * (W * W * 4.0) / (o * o) - 0.8)
+    g = [int(np.ceil(f(os_ * f(nx)))), int(np.ceil(f(os_ * f(ny))))]
+    g = [v + (v % 2) for v in g]
+    g.append(1 if nz == 1 else int(np.ceil(f(os_ * f(nz)))))
+    return o, float(beta), tuple(g)
+
+
+def kb_kernel(dist, beta, width=GRID_KERNEL_WIDTH):
+    """Kaiser-Bessel weight I0(beta sqrt(1 - (2 d / W)^2)) / W for d <= W/2,
+    else 0 (gridding_GPU_2D / gridding_Gold_2D, GPU_kernels.cu:290-313,
+    CPU_kernels.cpp:160-171); d in units of the unoversampled grid."""
+    d = np.asarray(dist, np.float64)
+    arg = 1.0 - (2.0 * d / width) ** 2
+    return np.where(d <= width / 2, np.i0(beta * np.sqrt(np.clip(arg, 0.0, None))) / width, 0.0)
+
+
+def kb_deapod(u, beta, width=GRID_KERNEL_WIDTH):
+    """Fourier transform of kb_kernel at image position u (cycles per grid
+    sample, u in [-1/2, 1/2)): sin(sqrt(pi^2 W^2 u^2 - beta^2)) / sqrt(...),
+    sinh(...) below zero (Deapodization2d_kernel, gridding_utils.cu:60-110)."""
+    e = (np.pi * width * np.asarray(u, np.float64)) ** 2 - beta * beta
+    out = np.empty_like(e)
+    pos = e >= 0
+    out[pos] = np.sinc(np.sqrt(e[pos]) / np.pi)           # sin(s)/s
+    s = np.sqrt(-e[~pos])
+    out[~pos] = np.sinh(s) / s
+    return out
+
+
+def gridding_adjoint(k, data, img, gridOS, beta=None):
+    """Gridding estimate of sum_j data_j e^{+i 2 pi k_j . x} on the image
+    grid x = (i - n/2)/n per axis, as computeFH_GPU_Grid and
+    computeQ_GPU_Grid evaluate it (toeplitz/computeFH.cmem.cu:470-802,
+    computeQ.cmem.cu:401-754; CUDA_interface.cu, GPU_kernels.cu:190-581,
+    gridding_utils.cu):
+      1. convolve the samples with the separable Kaiser-Bessel kernel onto
+         the oversampled grid, sample j at s = os (k_j + n/2) per axis,
+         grid points g in [0, G) within W/2 (unoversampled units), no wrap;
+      2. ifftshift, unnormalised inverse FFT, fftshift, crop the centre n;
+      3. divide by the kernel's transform per axis and by os per axis.
+    k: [M, d] (d = 2 or 3) in cycles of the transform's image; data [M]
+    complex (already time-weighted); img = (nx, ny, nz).  beta overrides
+    Beatty's value (gridding_Gold_2D/3D use 18.5547, CPU_kernels.cpp:110).
+    Returns [ny, nx] or [ny, nx, nz] complex128 (y outer, x, z inner)."""
+    os_, beta_b, G = gridding_params(img, gridOS)
+    beta = beta_b if beta is None else beta
+    W = GRID_KERNEL_WIDTH
+    d3 = img[2] > 1
+    dims = 3 if d3 else 2
+    n = [int(img[0]), int(img[1]), int(img[2])]
+    grid = gridding_grid(k, data, img, gridOS, beta)
+    # ifftshift, unnormalised inverse DFT, fftshift, centre crop
+    X = np.fft.ifftn(np.fft.ifftshift(grid)) * grid.size
+    X = np.fft.fftshift(X)
+    sl = tuple(slice(G[a] // 2 - n[a] // 2, G[a] // 2 - n[a] // 2 + n[a]) for a in range(dims))
+    X = X[sl]
+    # deapodization, separable, and 1 / os per axis
+    for a in range(dims):
+        u = (np.arange(n[a]) - n[a] // 2) / n[a]
+        c = kb_deapod(u, beta, W) * os_
+        shape = [1] * dims
+        shape[a] = n[a]
+        X = X / c.reshape(shape)
+    return np.swapaxes(X, 0, 1)
+
+
+def gridding_grid(k, data, img, gridOS, beta=None):
+    """Step 1 of gridding_adjoint alone: the oversampled grid, axes (x, y[, z])
+    (gridding_Gold_2D / gridding_Gold_3D before their reorder,
+    CPU_kernels.cpp:89-330; gridding_GPU_2D/3D, GPU_kernels.cu:190-581)."""
+    os_, beta_b, G = gridding_params(img, gridOS)
+    beta = beta_b if beta is None else beta
+    W = GRID_KERNEL_WIDTH
+    k = np.asarray(k, np.float64)
+    data = np.asarray(data, np.complex128).reshape(-1)
+    d3 = img[2] > 1
+    dims = 3 if d3 else 2
+    n = [int(img[0]), int(img[1]), int(img[2])]
+    grid = np.zeros(G[:dims], np.complex128)
+    half = W * os_ / 2.0
+    r = int(np.ceil(half)) + 1
+    offs = np.arange(-r, r + 1)
+    idx, wts = [], []
+    for a in range(dims):
+        s = os_ * (k[:, a] + n[a] / 2.0)
+        g = np.floor(s)[:, None] + offs[None, :]
+        dist = np.abs(s[:, None] - g) / os_
+        ok = (g >= 0) & (g < G[a]) & (dist <= W / 2)
+        idx.append(np.where(ok, g, 0).astype(np.int64))
+        wts.append(np.where(ok, kb_kernel(dist, beta, W), 0.0))
+    if dims == 2:
+        w = wts[0][:, :, None] * wts[1][:, None, :]
+        ix = np.broadcast_to(idx[0][:, :, None], w.shape)
+        iy = np.broadcast_to(idx[1][:, None, :], w.shape)
+        np.add.at(grid, (ix.reshape(-1), iy.reshape(-1)), (w * data[:, None, None]).reshape(-1))
+    else:
+        w = wts[0][:, :, None, None] * wts[1][:, None, :, None] * wts[2][:, None, None, :]
+        ix = np.broadcast_to(idx[0][:, :, None, None], w.shape)
+        iy = np.broadcast_to(idx[1][:, None, :, None], w.shape)
+        iz = np.broadcast_to(idx[2][:, None, None, :], w.shape)
+        np.add.at(grid, (ix.reshape(-1), iy.reshape(-1), iz.reshape(-1)),
+                  (w * data[:, None, None, None]).reshape(-1))
+    return grid
+
+
+def gridding_q(kx, ky, kz, t, Nx, Ny, Nz, ntime_segs, gridOS, beta=None):
+    """Q_l by gridding (computeQ_GPU_Grid, toeplitz/computeQ.cmem.cu:401-754,
+    called per l from toeplitz_computeQ_GPU :826-1028): the samples, with
+    weight a_l(t) (and phi = 1: unit data, :518-525), at 2k on the 2N grid.
+    Returns [(L+1), G] complex128 on the standard Q grid (toeplitz_q's)."""
+    kx, ky, kz, t = _as64(kx, ky, kz, t)
+    d3 = Nz > 1
+    k = np.stack([2 * kx, 2 * ky] + ([2 * kz] if d3 else []), axis=1)
+    img = (2 * Nx, 2 * Ny, 2 * Nz if d3 else 1)
+    tau, t0 = segment_timing(t, ntime_segs)
+    out = [gridding_adjoint(k, hanning(t, tau, l, t0), img, gridOS, beta).reshape(-1)
+           for l in range(ntime_segs + 1)]
+    return np.stack(out)
+
+
+def gridding_fhd(kx, ky, kz, t, fm, d, sens, Nx, Ny, Nz, ntime_segs, gridOS):
+    """F^H d by gridding with time segmentation (parcomputeAH_GPU_Grid /
+    computeAH_GPU_Grid, toeplitz/computeFH.cmem.cu:1032-1117):
+    sum_c conj(S_c) sum_l e^{+i w (t0 + l tau)} grid(a_l(t) d_c), the image on
+    x = (i - N/2)/N.  d: [C, M]; sens [C, N] or None.  Returns [N] complex128
+    in the [Ny][Nx][Nz] order."""
+    kx, ky, kz, t = _as64(kx, ky, kz, t)
+    d = np.atleast_2d(np.asarray(d, np.complex128))
+    d3 = Nz > 1
+    k = np.stack([kx, ky] + ([kz] if d3 else []), axis=1)
+    img = (Nx, Ny, Nz)
+    tau, t0 = segment_timing(t, ntime_segs)
+    nv = Nx * Ny * Nz
+    f = np.zeros(nv) if fm is None else np.asarray(fm, np.float64).reshape(-1)
+    S = np.ones((d.shape[0], nv)) if sens is None else np.asarray(sens, np.complex128).reshape(-1, nv)
+    out = np.zeros(nv, np.complex128)
+    for c in range(d.shape[0]):
+        acc = np.zeros(nv, np.complex128)
+        for l in range(ntime_segs + 1):
+            g = gridding_adjoint(k, hanning(t, tau, l, t0) * d[c], img, gridOS).reshape(-1)
+            acc += np.exp(1j * f * (tau * l + t0)) * g
+        out += np.conj(S[c]) * acc
+    return out
+
+
 def toeplitz_recon(make_apply, b, shape, num_iter, enable_tv=False, tv_num=1, restarts=1,
                    tol=1e-16, vec_dtype=np.complex128):
     """The restart / TV / CG loops of toeplitz_recon (toeplitz/recon.cu:2011-2133).

@@ -231,3 +231,88 @@ void bruteforce_cpu(float *x_r, float *x_i, const float *y_r, const float *y_i,
     free(af_r); free(af_i); free(q_r); free(q_i); free(r_r); free(r_i);
     free(g_r); free(g_i); free(d_r); free(d_i); free(c_r); free(c_i);
 }
+
+/* ---------------------------------------------------------------------------
+ * Gold CPU gridding: gridding_Gold_2D / gridding_Gold_3D
+ * (toeplitz/CPU_kernels.cpp:89-330), the reference's CPU version of the
+ * gridding step of computeQ_GPU_Grid / computeFH_GPU_Grid.  FP32 throughout,
+ * the fixed beta = 18.5547 of the gold path (CPU_kernels.cpp:110), I0 by the
+ * polynomial approximations of Abramowitz & Stegun 9.8.1-9.8.2, samples
+ * scaled by `kscale` (2 for Q, 1 for F^H d) and weighted by a_l(t) of
+ * segment l, window indices clamped to [0, gridOS n - 1] (no wrap).
+ * Output in the reference's order after its reorder: 2D [x][y] (y fastest),
+ * 3D [y][x][z] (z fastest); grid sizes g[3] (g[2] = 1 in 2D).  Data re/im of
+ * NULL means unit data (the Q case).  Single-threaded, as the reference. */
+static float gold_i0(float x) {
+    float ax = fabsf(x), y;
+    if (ax < 3.75f) {
+        y = x / 3.75f;
+        y *= y;
+        return 1.0f + y * (3.5156229f + y * (3.0899424f + y * (1.2067492f + y * (0.2659732f +
+                     y * (0.0360768f + y * 0.0045813f)))));
+    }
+    y = 3.75f / ax;
+    return (expf(ax) / sqrtf(ax)) * (0.39894228f + y * (0.01328592f + y * (0.00225319f +
+           y * (-0.00157565f + y * (0.00916281f + y * (-0.02057706f + y * (0.02635537f +
+           y * (-0.01647633f + y * 0.00392377f))))))));
+}
+
+static float gold_hanning(float tm, float tau, float l, float t0) {
+    const float x = tm - tau * l - t0;
+    return fabsf(x) < tau ? 0.5f + 0.5f * cosf(3.14159265359f * x / tau) : 0.0f;
+}
+
+static float gold_kb(float s, int g, float os, float width, float beta) {
+    const float dist = fabsf(s - (float)g) / os;
+    const float u = 2.0f * dist / width;
+    const float v = gold_i0(beta * sqrtf(1.0f - u * u)) / width;
+    return v != v ? 0.0f : v;   /* NaN -> 0 as CPU_kernels.cpp:162-163 */
+}
+
+void gridding_gold(int m, const float *kx, const float *ky, const float *kz, float kscale,
+                   const float *re, const float *im, const float *t, float l, float tau,
+                   int nx, int ny, int nz, float gridOS, float beta, const int *g,
+                   float *grid_r, float *grid_i) {
+    const float width = 4.0f, half = width * gridOS / 2.0f;
+    const int d3 = nz > 1;
+    const long ng = (long)g[0] * g[1] * g[2];
+    float *ar = (float *)calloc(ng, sizeof(float)), *ai = (float *)calloc(ng, sizeof(float));
+    const float t0 = t[0];
+    for (int i = 0; i < m; i++) {
+        const float atm = gold_hanning(t[i], tau, l, t0);
+        const float vr = re ? re[i] : 1.0f, vi = re ? im[i] : 0.0f;
+        const float sx = gridOS * (kscale * kx[i] + (float)nx / 2.0f);
+        const float sy = gridOS * (kscale * ky[i] + (float)ny / 2.0f);
+        const float sz = d3 ? gridOS * (kscale * kz[i] + (float)nz / 2.0f) : 0.0f;
+        const int xl = (int)fmaxf(0.0f, ceilf(sx - half));
+        const int xh = (int)fminf(gridOS * (float)nx - 1.0f, floorf(sx + half));
+        const int yl = (int)fmaxf(0.0f, ceilf(sy - half));
+        const int yh = (int)fminf(gridOS * (float)ny - 1.0f, floorf(sy + half));
+        const int zl = d3 ? (int)fmaxf(0.0f, ceilf(sz - half)) : 0;
+        const int zh = d3 ? (int)fminf(gridOS * (float)nz - 1.0f, floorf(sz + half)) : 0;
+        for (int z = zl; z <= zh; z++) {
+            const float kbz = d3 ? gold_kb(sz, z, gridOS, width, beta) : 1.0f;
+            for (int y = yl; y <= yh; y++) {
+                const float kby = gold_kb(sy, y, gridOS, width, beta);
+                for (int x = xl; x <= xh; x++) {
+                    const float w = gold_kb(sx, x, gridOS, width, beta) * kby * kbz;
+                    const long idx = x + (long)y * g[0] + (long)z * g[0] * g[1];
+                    ar[idx] += w * vr * atm;
+                    ai[idx] += w * vi * atm;
+                }
+            }
+        }
+    }
+    /* reorder x->y->z to the reference's output order (CPU_kernels.cpp:183-197, 310-325) */
+    for (int x = 0; x < g[0]; x++)
+        for (int y = 0; y < g[1]; y++)
+            for (int z = 0; z < g[2]; z++) {
+                const long src = x + (long)y * g[0] + (long)z * g[0] * g[1];
+                const long dst = d3 ? z + (long)x * g[2] + (long)y * g[0] * g[2]
+                                    : y + (long)x * g[0];
+                grid_r[dst] = ar[src];
+                grid_i[dst] = ai[src];
+            }
+    free(ar);
+    free(ai);
+}

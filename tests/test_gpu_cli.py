@@ -258,3 +258,44 @@ def test_bruteforce_tv_pass_hook(data_dir, ctx):
     final = ops.bruteforce_cg(ctx, keep["kx"], keep["ky"], keep["t"], keep["ix"], keep["iy"],
                               keep["fm"], d, None, 4, 0.01, x0, tv_num=3)
     assert np.array_equal(xr + 1j * xi, final.astype(np.complex64))
+
+
+def test_cli_toeplitz_gridding(data_dir):
+    """The reference's demo command (run_all_demos.sh:4: -toeplitzGridding
+    -ntime_segs 8 -gridOS_Q 1.250 -gridOS_FH 1.375) on its 64x64x1 input:
+    Q_stone.file and FhD.file against the FP64 restatement of the
+    reference's gridding (these files hold images and Q grids y fastest, the
+    order of their coordinate files), and the image against the oracle CG
+    on those Q_l and F^H d."""
+    from impatient import files
+    from oracle import oracle as orc
+    r = _cli(["-idir", str(data_dir), "-cg_num", "10", "-toeplitzGridding", "-ntime_segs", "8",
+              "-gpu_id", "0", "-gridOS_Q", "1.250", "-gridOS_FH", "1.375", "-writeQ"])
+    assert r.returncode == 0
+    rd = lambda n: files.read(data_dir / f"{n}.dat").astype(np.float64)
+    kx, ky, t, fm = rd("kx"), rd("ky"), rd("t"), rd("fm")
+    d = (rd("kdata_r") + 1j * rd("kdata_i"))[None, :]
+    n, L = 64, 8
+    to_file = lambda v, a, b: np.asarray(v).reshape(b, a).T.reshape(-1)   # [y][x] -> [x][y]
+    qref = orc.gridding_q(kx, ky, None, t, n, n, 1, L, 1.25)
+    q = np.fromfile(data_dir / "Q_stone.file", dtype=np.float32).reshape(L + 1, 2, -1)
+    q = q[:, 0] + 1j * q[:, 1]
+    qref_file = np.stack([to_file(qref[l], 2 * n, 2 * n) for l in range(L + 1)])
+    eq = rel_l2(q, qref_file)
+    fm_eng = fm.reshape(n, n).T.reshape(-1)                               # file -> engine order
+    bref = orc.gridding_fhd(kx, ky, None, t, fm_eng, d, None, n, n, 1, L, 1.375)
+    fhd = np.fromfile(data_dir / "FhD.file", dtype=np.float32).reshape(2, -1)
+    b = fhd[0] + 1j * fhd[1]
+    eb = rel_l2(b, to_file(bref, n, n))
+    # the solve: recon on the gridded Q_l and F^H d (lambda 0: no -fd given)
+    img = np.fromfile(data_dir / "out.file", dtype=np.float32).reshape(2, -1)
+    x = img[0] + 1j * img[1]
+    tau, t0 = orc.segment_timing(t, L)
+    qe = np.stack([q[l].reshape(2 * n, 2 * n).T.reshape(-1) for l in range(L + 1)])  # engine
+    A = lambda v: orc.toeplitz_apply(v.reshape(n, n), qe, tau, t0, fm_eng, None, 0.0).reshape(-1)
+    b_eng = b.reshape(n, n).T.reshape(-1).astype(np.complex128)
+    x32, _, _ = orc.toeplitz_cg(A, b_eng, 10, vec_dtype=np.complex64)
+    ex = rel_l2(x.reshape(n, n).T.reshape(-1), x32)
+    print(f"CLI gridding: Q vs FP64 gridding {eq:.2e}, F^H d {eb:.2e}, image vs FP32-storage "
+          f"oracle CG {ex:.2e}")
+    assert eq < 5e-6 and eb < 5e-6 and ex < 5e-3

@@ -1,0 +1,88 @@
+"""The gridding path (-toeplitzGridding): Q_l and F^H d by Kaiser-Bessel
+gridding on the GPU against the FP64 restatement of the reference's
+gridding (oracle.gridding_q / gridding_fhd), and against the direct
+operators (the method's own approximation error, "around 1%",
+README:151-154)."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+GRID_TOL = 5e-6   # FP32 gridding + FFT against the FP64 restatement
+
+
+@pytest.mark.parametrize("N,Nz,os_", [(32, 1, 1.375), (32, 1, 1.25), (24, 1, 2.0), (16, 8, 1.5)])
+def test_q_gridding_matches_oracle(ctx, N, Nz, os_):
+    """Grids of odd factor 11 (88 = 11 x 8), 5 (80 = 5 x 16), a power of
+    two (96) and 3D (48 x 48 x 24)."""
+    from impatient import ops, synth
+    from oracle import oracle as orc
+    p = synth.tiny(N=N, Nz=Nz, ncoils=1, J=8, S=256, ntime_segs=3)
+    q = ops.toeplitz_q_gridding(ctx, p.kx, p.ky, p.kz, p.t, N, N, Nz, 3, os_)
+    ref = orc.gridding_q(p.kx, p.ky, p.kz, p.t, N, N, Nz, 3, os_)
+    img = (2 * N, 2 * N, 2 * Nz if Nz > 1 else 1)
+    e = rel_l2(q, ref)
+    print(f"Q gridding {N}x{N}x{Nz} os {os_} grid {orc.gridding_params(img, os_)[2]}: "
+          f"vs FP64 {e:.2e}")
+    assert e < GRID_TOL
+
+
+@pytest.mark.parametrize("N,Nz,ncoils,os_", [(32, 1, 4, 1.375), (16, 8, 2, 1.5), (16, 1, 34, 1.25)])
+def test_fhd_gridding_matches_oracle(ctx, N, Nz, ncoils, os_):
+    """Time-segmented F^H d with field map and coils (34 coils: two coil
+    groups of the gather), 2D and 3D."""
+    from impatient import ops, synth
+    from oracle import oracle as orc
+    p = synth.tiny(N=N, Nz=Nz, ncoils=ncoils, J=8, S=256, ntime_segs=3)
+    rng = np.random.default_rng(61)
+    d = (rng.standard_normal((ncoils, p.nk)) + 1j * rng.standard_normal((ncoils, p.nk)))
+    got = ops.fhd_gridding(ctx, p.kx, p.ky, p.kz, p.t, N, N, Nz, p.fm, d.astype(np.complex64),
+                           p.sens, 3, os_)
+    ref = orc.gridding_fhd(p.kx, p.ky, p.kz, p.t, p.fm, d, p.sens, N, N, Nz, 3, os_)
+    e = rel_l2(got, ref)
+    print(f"F^H d gridding {N}x{N}x{Nz}, {ncoils} coils, os {os_}: vs FP64 {e:.2e}")
+    assert e < GRID_TOL
+
+
+def test_gridding_is_deterministic(ctx):
+    """Samples are gathered in a fixed order (no atomics on the grid): two
+    calls give identical bits."""
+    from impatient import ops, synth
+    p = synth.tiny(N=32, ncoils=4, J=8, S=256, ntime_segs=3)
+    rng = np.random.default_rng(62)
+    d = (rng.standard_normal((4, p.nk)) + 1j * rng.standard_normal((4, p.nk))).astype(np.complex64)
+    a = ops.fhd_gridding(ctx, p.kx, p.ky, None, p.t, 32, 32, 1, p.fm, d, p.sens, 3, 1.375)
+    b = ops.fhd_gridding(ctx, p.kx, p.ky, None, p.t, 32, 32, 1, p.fm, d, p.sens, 3, 1.375)
+    assert np.array_equal(a, b)
+
+
+def test_gridding_against_direct_at_config3_size(ctx):
+    """512^2, 48 x 8192 spiral, L = 10 (the 32-coil workload's trajectory):
+    the gridded Q_l against the direct (separable GEMM) Q_l, and the gridded
+    F^H d of 4 coils against the direct time-segmented one, at the reference's
+    demo oversampling (gridOS_Q 1.25, gridOS_FHD 1.375; run_all_demos.sh:4)."""
+    from impatient import ops, synth
+    from oracle import oracle as orc
+    p = synth.config(3)
+    L = p.ntime_segs
+    qg = ops.toeplitz_q_gridding(ctx, p.kx, p.ky, None, p.t, p.Nx, p.Ny, 1, L, 1.25)
+    qd = ops.toeplitz_q(ctx, p.kx, p.ky, None, p.t, p.Nx, p.Ny, 1, L)
+    eq = rel_l2(qg, qd)
+    rng = np.random.default_rng(63)
+    d = (rng.standard_normal((4, p.nk)) + 1j * rng.standard_normal((4, p.nk))).astype(np.complex64)
+    sens = p.sens[:4]
+    bg = ops.fhd_gridding(ctx, p.kx, p.ky, None, p.t, p.Nx, p.Ny, 1, p.fm, d, sens, L, 1.375)
+    # the direct F^H d with the same time segmentation
+    tau, t0 = orc.segment_timing(p.t, L)
+    bd = np.zeros(p.nvox, np.complex128)
+    for l in range(L + 1):
+        a = orc.hanning(p.t, tau, l, t0).astype(np.float32)
+        part = ops.nudft_adjoint(ctx, p.kx, p.ky, None, p.t, p.x, p.y, None, None,
+                                 d * a[None, :], sens)
+        bd += np.exp(1j * p.fm.astype(np.float64) * (tau * l + t0)) * part
+    ef = rel_l2(bg, bd)
+    print(f"config-3 size: gridded Q (os 1.25) vs direct {eq:.2e}; gridded F^H d (os 1.375) "
+          f"vs direct time-segmented {ef:.2e}")
+    assert eq < 0.03 and ef < 0.03

@@ -152,8 +152,11 @@ def test_file_level_wrappers_reject_bad_arguments():
     assert lib.imp_toeplitz_recon(18, argv18, 0, 1, 0, 0, None, P, P, None, None) == -3
     assert lib.imp_toeplitz_computeQ_GPU(b"/nonexistent", C.c_float(2), 8, 8, 1, None,
                                          None, None, 1, 0, C.c_float(1.0), 0) == -1
+    # gridding reads its input files first: a missing directory is IMP_EIO
     assert lib.imp_toeplitz_computeFH_GPU(b"/nonexistent", C.c_float(2), 8, 8, 1, None,
-                                          P, P, 0, 1, C.c_float(1.0)) == -8
+                                          P, P, 0, 1, C.c_float(1.0)) == -3
+    assert lib.imp_toeplitz_computeFH_GPU(b"/nonexistent", C.c_float(2), 8, 8, 1, None,
+                                          P, P, 0, 0, C.c_float(1.0)) == -1
 
 
 def test_context_without_gpu_fails_cleanly():

@@ -637,3 +637,120 @@ def test_toeplitz_recon_restarts_carry_the_best_residual():
     true = np.linalg.norm(b - A(x)) / np.linalg.norm(b)
     assert len(hists) == 3
     assert abs(true - allres.min()) < 1e-9 * allres.min()
+
+
+def _interior_samples(rng, n, M, dims):
+    """Samples at least 3 grid cells inside the k-space window on every axis."""
+    return np.stack([rng.uniform(-n[a] / 2 + 3, n[a] / 2 - 3, M) for a in range(dims)], axis=1)
+
+
+@pytest.mark.parametrize("img,oss", [((20, 24, 1), (1.25, 1.5, 2.0)), ((16, 12, 8), (1.5, 2.0))])
+def test_gridding_is_the_adjoint_nudft_away_from_the_grid_edge(img, oss):
+    """Gridding (KB width 4, Beatty beta) of samples away from the edge of
+    the oversampled grid is the adjoint NUDFT on x = (i - n/2)/n: to 1e-6 at
+    os = 2 and to a few 1e-3 at os = 1.5, in 2D and 3D; the error falls with
+    os.  A sign, index, shift or scale mistake gives O(1) errors."""
+    orc = _orc()
+    rng = np.random.default_rng(51)
+    nx, ny, nz = img
+    dims = 3 if nz > 1 else 2
+    k = _interior_samples(rng, (nx, ny, nz), 700, dims)
+    d = rng.standard_normal(700) + 1j * rng.standard_normal(700)
+    iy, ix, iz = np.meshgrid(np.arange(ny), np.arange(nx), np.arange(nz), indexing="ij")
+    xs = [((ix - nx // 2) / nx).ravel(), ((iy - ny // 2) / ny).ravel(), ((iz - nz // 2) / nz).ravel()]
+    ph = 2 * np.pi * sum(np.outer(k[:, a], xs[a]) for a in range(dims))
+    exact = (d @ np.exp(1j * ph)).reshape(ny, nx, nz) if dims == 3 else (d @ np.exp(1j * ph)).reshape(ny, nx)
+    errs = [rel_l2(orc.gridding_adjoint(k, d, img, os_), exact) for os_ in oss]
+    assert errs[-1] < 1e-6
+    assert errs[-2] < 1e-2
+    assert all(a > b for a, b in zip(errs, errs[1:]))
+
+
+def test_gridding_grid_edge_is_not_wrapped():
+    """As the reference (no periodic wrap: window indices clamped to the
+    grid, CPU_kernels.cpp:137-141, GPU_kernels.cu:211-221), a sample at
+    k = -n/2 keeps only the part of its kernel inside the grid: the image
+    centre is well below the exact 1 (0.72 at os = 2), which a wrapped grid
+    would reproduce; one cell further in, the whole kernel fits."""
+    orc = _orc()
+    n = 32
+    k = np.array([[-n / 2, 0.0]])
+    g = orc.gridding_adjoint(k, np.ones(1), (n, n, 1), 2.0)
+    ratio = np.abs(g[n // 2, n // 2])       # exact value 1 at x = 0
+    assert 0.55 < ratio < 0.85
+    g2 = orc.gridding_adjoint(np.array([[-n / 2 + 2, 0.0]]), np.ones(1), (n, n, 1), 2.0)
+    assert abs(np.abs(g2[n // 2, n // 2]) - 1.0) < 1e-6
+
+
+def test_gridding_params_follow_the_reference_rounding():
+    """gridOS rounding and grid sizes of computeQ_GPU_Grid / computeFH_GPU_Grid:
+    2D grids even (os rounded on x), 3D z a multiple of 4 (os rounded on z),
+    Beatty's beta."""
+    orc = _orc()
+    os_, beta, G = orc.gridding_params((1024, 1024, 1), 1.375)
+    assert (os_, G) == (1.375, (1408, 1408, 1))
+    assert abs(beta - np.pi * np.sqrt(0.875 ** 2 * 64 / 1.375 ** 2 - 0.8)) < 1e-12
+    os3, _, G3 = orc.gridding_params((128, 128, 32), 1.375)
+    assert os3 == 1.375 and G3 == (176, 176, 44)
+    os5, _, G5 = orc.gridding_params((128, 128, 12), 1.375)     # z rounded up to 20
+    assert abs(os5 - 20 / 12) < 1e-6 and G5[2] == 20 and G5[0] % 2 == 0
+    os4, _, G4 = orc.gridding_params((64, 64, 1), 1.3)
+    assert G4[0] % 2 == 0 and abs(os4 * 64 - G4[0]) < 1e-4
+
+
+def test_gridding_q_and_fhd_against_direct():
+    """Q_l and the time-segmented F^H d by gridding against the direct sums
+    on a spiral (samples out to |k| = N/2, so the unwrapped edge costs a
+    little): the reference's 'around 1%' at its default os."""
+    orc = _orc()
+    import sys
+    sys.path.insert(0, str(REPO / "impatient-mri_amd"))
+    from impatient import synth
+    p = synth.tiny(N=32, ncoils=2, J=8, S=256, ntime_segs=3)
+    q = orc.toeplitz_q(p.kx, p.ky, None, p.t, 32, 32, 1, 3)
+    qg = orc.gridding_q(p.kx, p.ky, None, p.t, 32, 32, 1, 3, 1.375)
+    assert rel_l2(qg, q) < 0.02
+    rng = np.random.default_rng(52)
+    d = rng.standard_normal((2, p.nk)) + 1j * rng.standard_normal((2, p.nk))
+    tau, t0 = orc.segment_timing(p.t, 3)
+    # with the field map the exact reference is the time-segmented sum
+    ts = sum(np.exp(1j * p.fm.astype(np.float64) * (tau * l + t0)) *
+             orc.nudft_adjoint(p.kx, p.ky, None, p.t, p.x, p.y, None, None,
+                               d * orc.hanning(p.t, tau, l, t0)[None, :], p.sens)
+             for l in range(4))
+    bg = orc.gridding_fhd(p.kx, p.ky, None, p.t, p.fm, d, p.sens, 32, 32, 1, 3, 2.0)
+    assert rel_l2(bg, ts) < 0.05
+
+
+@pytest.mark.parametrize("img,kscale", [((32, 32, 1), 1.0), ((16, 16, 8), 2.0)])
+def test_c_gold_gridding_agrees_with_fp64(img, kscale):
+    """gridding_Gold_2D / 3D restated in FP32 C (oracle/ref_cpu.c
+    gridding_gold, the timed CPU gridding) against the FP64 grid of
+    oracle.gridding_grid with the gold path's beta = 18.5547, in the
+    reference's output order ([x][y] in 2D, [y][x][z] in 3D), with Hanning
+    weights of one segment."""
+    orc = _orc()
+    lib = _cpu_lib()
+    rng = np.random.default_rng(53)
+    nx, ny, nz = img
+    d3 = nz > 1
+    M = 500
+    lim = [n / (2 * kscale) for n in img]
+    k = np.stack([rng.uniform(-lim[a], lim[a], M) for a in range(3 if d3 else 2)], axis=1)
+    t = np.linspace(0.0, 5e-3, M)
+    v = rng.standard_normal(M) + 1j * rng.standard_normal(M)
+    L, l = 3, 1
+    tau, t0 = orc.segment_timing(t, L)
+    os_, _, G = orc.gridding_params(img, 1.5)
+    ref = orc.gridding_grid(kscale * k, v * orc.hanning(t.astype(np.float32), tau, l, t0), img,
+                            1.5, beta=18.5547)
+    ref = np.transpose(ref, (1, 0, 2)).reshape(-1) if d3 else ref.reshape(-1)
+    keep = [_fp(a) for a in (k[:, 0], k[:, 1], k[:, 2] if d3 else np.zeros(M), v.real, v.imag, t)]
+    gr = np.zeros(int(np.prod(G)), np.float32)
+    gi = np.zeros(int(np.prod(G)), np.float32)
+    g = (C.c_int * 3)(*G)
+    P = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+    lib.gridding_gold(C.c_int(M), keep[0][1], keep[1][1], keep[2][1], C.c_float(kscale),
+                      keep[3][1], keep[4][1], keep[5][1], C.c_float(l), C.c_float(tau), C.c_int(nx),
+                      C.c_int(ny), C.c_int(nz), C.c_float(os_), C.c_float(18.5547), g, P(gr), P(gi))
+    assert rel_l2(gr + 1j * gi, ref) < 2e-5
