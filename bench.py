@@ -47,6 +47,9 @@ MFMA_FLOPS_PER_PAIR = 24.0
 # 256 MiB device copy calibrates (profiles/traffic_512x512x1_32coil_pmc.md);
 # re-measure when the kernel changes
 NUDFT_TRAFFIC_BYTES = 2.0 * 3436252.2 * 1024 + 1.0 * 242554.8 * 1024
+NUDFT_TRAFFIC_SOURCE = ("constant from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of "
+                        "bench.py --pmc-mode (profiles/traffic_512x512x1_32coil_pmc.md), "
+                        "not measured in this run")
 
 
 def hip_events():
@@ -108,9 +111,27 @@ def cpu_baseline(prob, d_host, seconds_target=15.0):
     pairs = float(m) * nv
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": pairs / dt / 1e9, "unit": "GSample*vox/s", "cores": cores,
-            "kind": "port",
+            "kind": "port", **host_cpu(),
             "sample": f"iftCpu restatement (FP32, OpenMP inner reduction), 1 coil x {m} "
                       f"samples x {nv} voxels = {pairs:.3e} pairs in {dt:.2f} s"}
+
+
+def host_cpu():
+    """CPU model (from /proc/cpuinfo) and the logical CPUs this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": avail}
 
 
 def cpu_bruteforce_cfg0(ctx):
@@ -169,21 +190,44 @@ def learned_solve(args, ctx, h, prob, dbr, dbi, dxr, dxi, fhd_ms, t_q, t_plan, l
         log(f"no trained model at {MODEL}: learned start skipped")
         return f"no model file ({MODEL.name})"
     params, gains, meta = learn.load_model(MODEL)
+    params_pc = learn.load_precond_params(MODEL) if meta.get("use_netw") else None
     net = ops.CNN(ctx, prob.Ny, prob.Nx, seed=1)
     net.set_params(params)
     gh = np.ascontiguousarray(gains, np.float32)
+    # the preconditioner network's weights depend on the trajectory (through
+    # the Chan weights of Q_l), not on the data: computed once per
+    # trajectory, like Q_l, and counted in the time to 1e-5
+    dw = None
+    t_pcnet = 0.0
+    if params_pc is not None:
+        net_pc = ops.CNN(ctx, prob.Ny, prob.Nx, seed=1)
+        net_pc.set_params(params_pc)
+        for _ in range(2):   # the first call loads the kernels' code; the second is timed
+            call("imp_toep_set_precond", h, None)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            chan = ctx.alloc((prob.nvox,))
+            call("imp_toep_get_precond", h, ptr(chan))
+            w = learn.network_weights(net_pc, chan.get().reshape(prob.Ny, prob.Nx))
+            dw = ctx.to_device(np.ascontiguousarray(w.reshape(-1), np.float32))
+            ctx.synchronize()
+            t_pcnet = time.perf_counter() - t0
+        net_pc.close()
     flags = (impatient.IMP_CG_BEST_ITERATE | impatient.IMP_CG_TRUE_RESIDUAL |
              impatient.IMP_CG_PRECOND | impatient.IMP_CG_USE_X0 | impatient.IMP_CG_FALLBACK_ZERO)
     zr, zi = ctx.alloc((prob.nvox,)), ctx.alloc((prob.nvox,))
     out = {"model": MODEL.name, "trained_on": f"{meta['draws']} draws, {meta['epochs']} epochs",
-           "preconditioner": "learned radial gains" if meta.get("use_gains") else "Chan"}
+           "preconditioner": ("network weights" if dw is not None else
+                              "learned radial gains" if meta.get("use_gains") else "Chan")}
     for warm in (True, False):   # first pass warms the kernels up, second is timed
         call("imp_toep_set_precond", h, None)
         ctx.synchronize()
         t0 = time.perf_counter()
         call("imp_toep_precond_apply", h, ptr(dbr), ptr(dbi), ptr(zr), ptr(zi))
         net.forward_device(zr, zi, dxr, dxi)
-        if meta.get("use_gains", False):
+        if dw is not None:
+            call("imp_toep_set_precond", h, ptr(dw))
+        elif meta.get("use_gains", False):
             call("imp_toep_set_precond_radial", h, gh.ctypes.data_as(C.c_void_p),
                  C.c_int(len(gh)))
         ctx.synchronize()
@@ -202,7 +246,124 @@ def learned_solve(args, ctx, h, prob, dbr, dbi, dxr, dxi, fhd_ms, t_q, t_plan, l
                 "start_rel_residual": s["start_rel_residual"], "start_used": s["start_used"],
                 "solve_s": round(t_total, 4), "true_rel_residual": s["true_rel_residual"],
                 "converged": s["converged"],
-                "time_to_1e-5_s": round(fhd_ms * 1e-3 + t_q + t_plan + t_total, 4)})
+                "precond_network_s": round(t_pcnet, 4),
+                "time_to_1e-5_s": round(fhd_ms * 1e-3 + t_q + t_plan + t_pcnet + t_total, 4)})
+    return out
+
+
+def batch_leg(ctx, log):
+    """The 48-slice batch of uneven sizes ({128, 256, 384}^2, 8 coils, L = 8,
+    +-150 Hz): one persistent F^H d launch, Q_l per size class, per slice the
+    learned start (plain-CG fallback) and graph-captured PCG to 1e-5.  Two
+    passes with one autotuner; the second is reported (the first includes the
+    autotuner's trial launches)."""
+    from impatient import batch, learn
+    t0 = time.perf_counter()
+    sl = batch.make_slices(48, ctx=ctx)
+    t_syn = time.perf_counter() - t0
+    params = learn.load_model(MODEL)[0] if MODEL.exists() else None
+    tuner = batch.Autotuner(ctx)
+    quiet = lambda m: None
+    first = batch.reconstruct(ctx, sl, tol=1e-5, model=params, tuner=tuner, log=quiet)
+    res = batch.reconstruct(ctx, sl, tol=1e-5, model=params, tuner=tuner, log=quiet)
+    recs = res["slices"]
+    by = {}
+    for r in recs:
+        by.setdefault(r["N"], []).append(r)
+    out = {
+        "workload": "48 2D slices from {128^2, 256^2, 384^2} (seeded draw), 8 coils, L=8, "
+                    "+-150 Hz, spiral scaled per size; learned start with plain-CG fallback, "
+                    "graph-captured PCG to 1e-5",
+        "slices": len(recs), "synthesis_s": round(t_syn, 2),
+        "first_pass_s": round(first["total_s"], 4), "total_s": round(res["total_s"], 4),
+        "slices_per_s": round(len(recs) / res["total_s"], 2),
+        "fhd_batch_s": round(res["fhd_s"], 4), "q_s": round(res["q_s"], 4),
+        "converged": int(sum(r["converged"] for r in recs)),
+        "max_true_rel_residual": max(r["true_rel_residual"] for r in recs),
+        "model": MODEL.name if params is not None else None,
+        "per_size": {str(N): {"count": len(v),
+                              "mean_iterations": round(float(np.mean([r["iterations"] for r in v])), 1),
+                              "mean_slice_s": round(float(np.mean([r["slice_s"] for r in v])), 4),
+                              "start_used": int(sum(r["start_used"] for r in v))}
+                     for N, v in sorted(by.items())},
+    }
+    log(f"batch: {out['slices']} slices in {out['total_s']:.3f} s ({out['slices_per_s']} "
+        f"slices/s), max true residual {out['max_true_rel_residual']:.2e}")
+    return out
+
+
+def gridding_leg(ctx, prob, dkx, dky, dt_, dfm, ddr, ddi, dsr, dsi, d_host, log):
+    """The gridding path (-toeplitzGridding) on the same workload at the
+    reference's demo oversampling (gridOS_Q 1.25, gridOS_FH 1.375;
+    run_all_demos.sh:4): Q_l (L + 1 planes on the 1024^2 Q grid) and the
+    32-coil F^H d on the GPU, each timed after one untimed call, beside the
+    C restatement of the reference's CPU gold gridding step
+    (gridding_Gold_2D, CPU_kernels.cpp:89-200, single-threaded as there):
+    Q's 11 segment grids in full, F^H d's grids for 1 coil (x 32 coils,
+    extrapolated)."""
+    from impatient._lib import call, ptr
+    from oracle import oracle as orc  # checker / baseline only
+    L, nv, nk, C_ = prob.ntime_segs, prob.nvox, prob.nk, prob.ncoils
+    nq = 4 * nv
+    qr, qi = ctx.alloc(((L + 1) * nq,)), ctx.alloc(((L + 1) * nq,))
+    br, bi = ctx.alloc((nv,)), ctx.alloc((nv,))
+
+    def q():
+        call("imp_toeplitz_q_gridding", ctx.h, C.c_int(nk), ptr(dkx), ptr(dky), None, ptr(dt_),
+             C.c_int(prob.Nx), C.c_int(prob.Ny), C.c_int(1), C.c_int(L), C.c_float(1.25),
+             ptr(qr), ptr(qi))
+
+    def fhd():
+        call("imp_fhd_gridding", ctx.h, C.c_int(nk), C.c_int(C_), ptr(dkx), ptr(dky), None,
+             ptr(dt_), C.c_int(prob.Nx), C.c_int(prob.Ny), C.c_int(1), ptr(dfm), ptr(ddr),
+             ptr(ddi), ptr(dsr), ptr(dsi), C.c_int(L), C.c_float(1.375), ptr(br), ptr(bi))
+    out = {"gridOS_Q": 1.25, "gridOS_FH": 1.375}
+    for name, fn in (("q", q), ("fhd", fhd)):
+        fn()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        ctx.synchronize()
+        out[name + "_s"] = round(time.perf_counter() - t0, 4)
+    log(f"gridding on the GPU: Q_l {out['q_s'] * 1e3:.1f} ms, F^H d {out['fhd_s'] * 1e3:.1f} ms")
+    so = REPO / "oracle" / "liboracle_cpu.so"
+    if so.exists():
+        lib = C.CDLL(str(so))
+        P = lambda a: a.ctypes.data_as(C.POINTER(C.c_float))
+        f32 = lambda a: np.ascontiguousarray(a, np.float32)
+        kx, ky, t = f32(prob.kx), f32(prob.ky), f32(prob.t)
+        tau, t0 = orc.segment_timing(prob.t, L)
+        lib.gridding_gold.argtypes = [C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                      C.c_void_p, C.c_float, C.c_void_p, C.c_void_p,
+                                      C.POINTER(C.c_float), C.c_float, C.c_float, C.c_int,
+                                      C.c_int, C.c_int, C.c_float, C.c_float,
+                                      C.POINTER(C.c_int), C.POINTER(C.c_float),
+                                      C.POINTER(C.c_float)]
+        timings = {}
+        for name, img, kscale, os_, data in (("q", (2 * prob.Nx, 2 * prob.Ny, 1), 2.0, 1.25, None),
+                                            ("fhd_1coil", (prob.Nx, prob.Ny, 1), 1.0, 1.375,
+                                             (f32(d_host[0].real), f32(d_host[0].imag)))):
+            o, beta, g = orc.gridding_params(img, os_)
+            gi = np.asarray(g, np.int32)
+            gr_ = np.zeros(int(np.prod(g)), np.float32)
+            gim = np.zeros_like(gr_)
+            t1 = time.perf_counter()
+            for l in range(L + 1):
+                lib.gridding_gold(nk, P(kx), P(ky), None, kscale,
+                                  None if data is None else data[0].ctypes.data,
+                                  None if data is None else data[1].ctypes.data, P(t), float(l),
+                                  float(tau), img[0], img[1], 1, float(o), float(beta),
+                                  gi.ctypes.data_as(C.POINTER(C.c_int)), P(gr_), P(gim))
+            timings[name] = time.perf_counter() - t1
+        out["cpu_gold"] = {
+            "what": "gridding step of gridding_Gold_2D (C restatement, 1 thread), all L+1 "
+                    "segments; no FFT or deapodization (the reference runs those on the GPU)",
+            "q_s": round(timings["q"], 3),
+            "fhd_1coil_s": round(timings["fhd_1coil"], 3),
+            "fhd_32coil_s_extrapolated": round(timings["fhd_1coil"] * C_, 2),
+            "threads": 1}
+        log(f"gridding, CPU gold step: Q {timings['q']:.2f} s, F^H d {timings['fhd_1coil']:.2f} s "
+            f"per coil")
     return out
 
 
@@ -219,6 +380,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-recon", action="store_true", help="skip Q / CG (profiling runs)")
     ap.add_argument("--cg-max-iter", type=int, default=400)
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the 48-slice uneven-size batch")
     ap.add_argument("--pmc-mode", action="store_true",
                     help="counter runs: one F^H d, one Toeplitz apply and one 256 MiB "
                          "device copy (the calibration kernel), nothing else")
@@ -307,6 +470,7 @@ def main():
     roofline = {"bound": "mfma", "achieved": round(achieved, 1), "peak": PEAK_FP16_MFMA_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP16_MFMA_TFLOPS, 4),
                 "traffic": round(NUDFT_TRAFFIC_BYTES),
+                "traffic_source": NUDFT_TRAFFIC_SOURCE,
                 "kernel": "k_nudft_mfma<2,false> (F^H d, 32 coils in one pass)",
                 "kernel_ms": round(k_ms, 3),
                 "work": "24 fp16 MFMA flops per (coil, sample, voxel) pair"}
@@ -323,7 +487,8 @@ def main():
                   "(plain CG) reported in recon",
         "value": round(value, 2), "unit": "GSample*vox/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32-accurate (3x fp16-split MFMA products, FP32 accumulate, FP64 phase)",
         "data": "synthetic (seeded Shepp-Logan, spiral, Gaussian field map and coils)",
         "config": {"workload": "512x512x1 32-coil spiral 48x8192, L=10, +-200 Hz (F^H d)",
                    "nvox": nv, "nsamples": nk, "ncoils": ncoils, "pairs": pairs},
@@ -364,18 +529,24 @@ def main():
         tp0 = time.perf_counter()
         plan()
         t_plan = time.perf_counter() - tp0
-        # one apply, timed
+        # the apply, timed with the kernel timer off (its per-launch events
+        # would be inside the region), then 3 more with it on for the passes
         dxr, dxi = ctx.alloc((nv,)), ctx.alloc((nv,))
+        call("imp_profile_enable", ctx.h, C.c_int(0))
         call("imp_toep_apply", h, ptr(dbr), ptr(dbi), ptr(dxr), ptr(dxi))
         ctx.synchronize()
-        call("imp_profile_reset", ctx.h)
         e0.record(stream)
-        for _ in range(3):
+        for _ in range(10):
             call("imp_toep_apply", h, ptr(dbr), ptr(dbi), ptr(dxr), ptr(dxi))
         e1.record(stream)
         e1.sync()
-        t_apply = e1.ms_since(e0) / 3
-        log(f"Toeplitz plan {t_plan:.3f} s, apply {t_apply:.2f} ms")
+        t_apply = e1.ms_since(e0) / 10
+        call("imp_profile_enable", ctx.h, C.c_int(1))
+        call("imp_profile_reset", ctx.h)
+        for _ in range(3):
+            call("imp_toep_apply", h, ptr(dbr), ptr(dbi), ptr(dxr), ptr(dxi))
+        ctx.synchronize()
+        log(f"Toeplitz plan {t_plan:.3f} s, apply {t_apply:.2f} ms (timer off)")
         # HBM roofline of the FFT passes: algorithmic bytes per launch (one
         # segment l, all coils; complex64 = 8 B; the padded grid keeps the
         # N1 x 2N2 rows that hold data)
@@ -404,7 +575,7 @@ def main():
                                       C.c_size_t(nbytes), C.c_int(3), C.c_void_p(stream)) == 0
             ctx.synchronize()
             call("imp_toep_destroy", h)
-            log("pmc mode: F^H d, Q, 4 applies, one 256 MiB copy done")
+            log("pmc mode: F^H d, Q, 14 applies, one 256 MiB copy done")
             print(json.dumps(result), flush=True)
             return
         st = _lib.CGStats()
@@ -429,6 +600,7 @@ def main():
             f"{s2['rel_residual']:.2e}, true {s2['true_rel_residual']:.2e}")
         learned = learned_solve(args, ctx, h, prob, dbr, dbi, dxr, dxi, ms, t_q, t_plan, log)
         call("imp_toep_destroy", h)
+        gridding = gridding_leg(ctx, prob, dkx, dky, dt_, dfm, ddr, ddi, dsr, dsi, d, log)
         result["recon"] = {
             "fhd_s": round(ms * 1e-3, 4), "q_s": round(t_q, 4),
             "q_direct_equiv_gsample_vox_s": round(q_pairs / t_q / 1e9, 1),
@@ -442,11 +614,18 @@ def main():
             "cg_chan_true_rel_residual": s2["true_rel_residual"],
             "time_to_1e-5_chan_s": round(ms * 1e-3 + t_q + t_plan + t_pcg, 4),
             "learned": learned,
+            "gridding": gridding,
         }
         if isinstance(learned, dict) and "time_to_1e-5_s" in learned:
             result["recon"]["speedup_learned_vs_plain_cg"] = round(t_cg / learned["solve_s"], 2)
             result["recon"]["speedup_learned_vs_plain_time_to_1e-5"] = round(
                 result["recon"]["time_to_1e-5_plain_s"] / learned["time_to_1e-5_s"], 2)
+            result["recon"]["speedup_learned_vs_chan_cg"] = round(t_pcg / learned["solve_s"], 2)
+            result["recon"]["speedup_learned_vs_chan_time_to_1e-5"] = round(
+                result["recon"]["time_to_1e-5_chan_s"] / learned["time_to_1e-5_s"], 2)
+
+    if not args.no_recon and not args.no_batch:
+        result["batch"] = batch_leg(ctx, log)
 
     if not args.no_cpu_baseline:
         log("cpu baseline ...")

@@ -56,6 +56,7 @@ constexpr int MF_BPAD = 2;        // MFMA: zero steps packed after the last (loa
 constexpr int MF_COLS = 16;       // MFMA: right-hand sides per 32-column tile (Re + Im)
 constexpr int W_EXP = 12;         // W is scaled so max |W| < 2^12
 constexpr int MF_WAVE_PTS = 64;   // MFMA: points per wave (two 32-point M tiles)
+constexpr int MF_PIPE_VALU = 5;  // MFMA: VALU instructions scheduled after each MFMA
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
@@ -184,7 +185,11 @@ __global__ __launch_bounds__(NU_THREADS) void k_nudft_valu(
 // [cos of A | sin of A] and [cos of B | sin of B], i.e. the A operands with
 // K rows (Re|Im, sample) that k_pack_mfma's B operands expect.  B operands
 // are fetched two steps ahead into two alternating register sets (bpack
-// holds MF_BPAD zero steps after the last).  tot stays in registers and the two-sum
+// holds MF_BPAD zero steps after the last).  The single-job launch
+// software-pipelines the steps: the MFMAs of step s are interleaved with the
+// phase, sin/cos and split work of step s + 1 (one MFMA, then MF_PIPE_VALU
+// VALU instructions), so each wave keeps the matrix core and the VALU
+// issuing together.  tot stays in registers and the two-sum
 // compensation in LDS.  Output planes: partial[chunk][TILES*32][n_lane]
 // (plane = tile*32 + part*16 + rhs%16).  `one` is 1.0f, passed at run time
 // so the lo split compiles to v_fma_mix.
@@ -247,8 +252,12 @@ __device__ __forceinline__ void mfma_body(const float4 *__restrict__ lanes, int 
         for (int j = 0; j < NPOS; ++j) pos[j] = B[j];
     }
 
-    // one step of 8 samples with B operands b
-    auto step = [&](int st, const BPair(&b)[TILES]) {
+    struct AOp {
+        half8 hA, hB, lA, lB;
+    };
+    // A operands of the step whose positions are in pos; pos is then
+    // reloaded with step `next` (clamped inside the chunk)
+    auto operands = [&](int next, AOp &o) {
         float cv[MF_K], sv[MF_K];
 #pragma unroll
         for (int j = 0; j < MF_K; ++j) {
@@ -261,13 +270,9 @@ __device__ __forceinline__ void mfma_body(const float4 *__restrict__ lanes, int 
             sv[j] = __builtin_amdgcn_sinf(r);
         }
         {
-            // next step (the last step reloads itself: stays inside spos)
-            const double *B = spos + (size_t)min(st + 1, s1 - 1) * NPOS;
+            const double *B = spos + (size_t)min(next, s1 - 1) * NPOS;
 #pragma unroll
             for (int j = 0; j < NPOS; ++j) pos[j] = B[j];
-            // keep the loads here: the scheduler would otherwise sink them
-            // below the MFMAs
-            __builtin_amdgcn_sched_barrier(0);
         }
         u32x4 chu, clu, shu, slu;
         split8(cv, one, chu, clu);
@@ -282,45 +287,72 @@ __device__ __forceinline__ void mfma_body(const float4 *__restrict__ lanes, int 
             lA[i] = rl[0];
             lB[i] = rl[1];
         }
-        const half8 AhA = __builtin_bit_cast(half8, hA), AhB = __builtin_bit_cast(half8, hB);
-        const half8 AlA = __builtin_bit_cast(half8, lA), AlB = __builtin_bit_cast(half8, lB);
+        o.hA = __builtin_bit_cast(half8, hA);
+        o.hB = __builtin_bit_cast(half8, hB);
+        o.lA = __builtin_bit_cast(half8, lA);
+        o.lB = __builtin_bit_cast(half8, lB);
+    };
+    auto mfmas = [&](const AOp &o, const BPair(&b)[TILES]) {
 #pragma unroll
         for (int t = 0; t < TILES; ++t) {
-            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(AhA, b[t].hi, acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(o.hA, b[t].hi, acc[t], 0, 0, 0);
             acc[TILES + t] =
-                __builtin_amdgcn_mfma_f32_32x32x16_f16(AhB, b[t].hi, acc[TILES + t], 0, 0, 0);
+                __builtin_amdgcn_mfma_f32_32x32x16_f16(o.hB, b[t].hi, acc[TILES + t], 0, 0, 0);
         }
 #pragma unroll
         for (int t = 0; t < TILES; ++t) {
-            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(AhA, b[t].lo, acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(o.hA, b[t].lo, acc[t], 0, 0, 0);
             acc[TILES + t] =
-                __builtin_amdgcn_mfma_f32_32x32x16_f16(AhB, b[t].lo, acc[TILES + t], 0, 0, 0);
+                __builtin_amdgcn_mfma_f32_32x32x16_f16(o.hB, b[t].lo, acc[TILES + t], 0, 0, 0);
         }
 #pragma unroll
         for (int t = 0; t < TILES; ++t) {
-            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(AlA, b[t].hi, acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(o.lA, b[t].hi, acc[t], 0, 0, 0);
             acc[TILES + t] =
-                __builtin_amdgcn_mfma_f32_32x32x16_f16(AlB, b[t].hi, acc[TILES + t], 0, 0, 0);
+                __builtin_amdgcn_mfma_f32_32x32x16_f16(o.lB, b[t].hi, acc[TILES + t], 0, 0, 0);
         }
+    };
+    // step st: its MFMAs (operands cur, B operands b) interleaved with the
+    // VALU work of step st + 1 (into nxt)
+    auto pstep = [&](int st, const AOp &cur, AOp &nxt, const BPair(&b)[TILES]) {
+        operands(st + 2, nxt);
+        mfmas(cur, b);
+#pragma unroll
+        for (int k = 0; k < 6 * TILES; ++k) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x002, MF_PIPE_VALU, 0);
+        }
+    };
+    auto step = [&](int st, const BPair(&b)[TILES]) {
+        AOp o;
+        operands(st + 1, o);
+        mfmas(o, b);
     };
     auto refill = [&](BPair(&b)[TILES], int st) {
 #pragma unroll
         for (int t = 0; t < TILES; ++t) b[t] = bpack[((size_t)st * TILES + t) * 64 + l];
     };
 
+    AOp op0, op1;
+    if constexpr (PAIRS) {
+        // prologue: operands of s0; pos then holds step s0 + 1
+        if (s0 < s1) operands(s0 + 1, op0);
+    }
     for (int f0 = s0; f0 < s1; f0 += MF_FOLD) {
         const int f1 = min(s1, f0 + MF_FOLD);
         if constexpr (PAIRS) {
             int st = f0;
-            // invariant at the top: b0 holds step st, b1 step st + 1
+            // invariant at the top: op0 and b0 hold step st, b1 step st + 1,
+            // pos step st + 1
             for (; st + 1 < f1; st += 2) {
-                step(st, b0);
+                pstep(st, op0, op1, b0);
                 refill(b0, st + 2);
-                step(st + 1, b1);
+                pstep(st + 1, op1, op0, b1);
                 refill(b1, st + 3);
             }
             if (st < f1) {   // odd tail of a chunk: one step, then shift the sets
-                step(st, b0);
+                pstep(st, op0, op1, b0);
+                op0 = op1;
 #pragma unroll
                 for (int t = 0; t < TILES; ++t) b0[t] = b1[t];
                 refill(b1, st + 2);
@@ -806,7 +838,36 @@ struct Job {
     const char *prof = "k_nudft_mfma";   // kernel-timer name of the MFMA launch
 };
 
-int run_job(imp_ctx *ctx, const Job &J) {
+// Where one call's device scratch comes from: the context's growable
+// workspaces (base == nullptr), or a fixed region carved front to back in
+// 256-byte units (the allocation-free entry point).
+struct Scratch {
+    imp_ctx *ctx;
+    char *base = nullptr;
+    size_t cap = 0, used = 0;
+    static size_t unit(size_t bytes) { return (bytes + 255) & ~(size_t)255; }
+    int get(int slot, size_t bytes, void **out) {
+        if (!base) return workspace(ctx, slot, bytes, out);
+        if (unit(bytes) > cap - used)
+            return fail(IMP_EUNSUPPORTED, "NUDFT: the fixed scratch region (%zu bytes) is %zu "
+                                          "bytes short", cap, unit(bytes) - (cap - used));
+        *out = base + used;
+        used += unit(bytes);
+        return IMP_OK;
+    }
+    // chunks whose partials still fit (fixed region), at most `want`
+    int fit_chunks(int want, size_t per_chunk) const {
+        if (!base) return want;
+        const size_t n = (cap - used) / unit(per_chunk);
+        return (int)std::min<size_t>((size_t)want, n);
+    }
+    // the kernel timer records HIP events; the fixed-region path records none
+    int prof(const char *name) const { return base ? -1 : prof_begin(ctx, name); }
+};
+
+__global__ void k_zero_u32(unsigned int *p) { *p = 0u; }
+
+int run_job(imp_ctx *ctx, const Job &J, Scratch &S) {
     const hipStream_t st = ctx->stream;
     const int n = J.n_lane;
     if (J.nrhs >= MFMA_MIN_RHS) {
@@ -815,27 +876,30 @@ int run_job(imp_ctx *ctx, const Job &J) {
             return fail(IMP_EUNSUPPORTED, "NUDFT: %d right-hand sides (at most 32)", J.nrhs);
         const int tiles = total_tiles;
         const int n_steps = ceil_div(J.n_stream, MF_K);
-        // max |W| for the power-of-two scale
+        // max |W| for the power-of-two scale (cleared by a kernel, not a
+        // memset node, so the sequence can be captured in a graph)
         unsigned int *maxbits = nullptr;
         void *mb = nullptr, *bp = nullptr, *part = nullptr;
-        IMP_TRY(workspace(ctx, WS_MAXBITS, 256, &mb));
+        IMP_TRY(S.get(WS_MAXBITS, 256, &mb));
         maxbits = (unsigned int *)mb;
-        IMP_HIP(hipMemsetAsync(maxbits, 0, sizeof(unsigned int), st));
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, st, maxbits);
         hipLaunchKernelGGL(k_absmax, dim3(512), dim3(256), 0, st, J.W,
                            (size_t)J.nrhs * J.n_stream, maxbits);
         IMP_CHECK_LAUNCH();
         // MF_BPAD zero steps after the last: the kernel fetches up to step st + 2
         const size_t nb = (size_t)(n_steps + MF_BPAD) * tiles * 64;
-        IMP_TRY(workspace(ctx, WS_BPACK, nb * sizeof(BPair), &bp));
+        IMP_TRY(S.get(WS_BPACK, nb * sizeof(BPair), &bp));
         hipLaunchKernelGGL(k_pack_mfma, dim3(ceil_div((long)nb, 256)), dim3(256), 0, st,
                            J.n_stream, J.nrhs, tiles, n_steps + MF_BPAD, J.sign, J.W, maxbits,
                            (BPair *)bp);
         IMP_CHECK_LAUNCH();
         const long wgs = ceil_div(n, MF_WAVE_PTS * (NU_THREADS / 64));
-        const int chunks = pick_chunks(ctx, wgs, n_steps, 4 * MF_FOLD);
+        const size_t per_chunk = (size_t)tiles * 32 * n * sizeof(float);
+        const int chunks = S.fit_chunks(pick_chunks(ctx, wgs, n_steps, 4 * MF_FOLD), per_chunk);
+        if (chunks < 1) return fail(IMP_EUNSUPPORTED, "NUDFT: no room for one chunk's partials");
         const int spc = ceil_div(n_steps, chunks);
-        IMP_TRY(workspace(ctx, WS_PARTIAL, (size_t)chunks * tiles * 32 * n * sizeof(float), &part));
-        const int rec = prof_begin(ctx, J.prof);
+        IMP_TRY(S.get(WS_PARTIAL, (size_t)chunks * per_chunk, &part));
+        const int rec = S.prof(J.prof);
         hipLaunchKernelGGL(pick_mfma(tiles, J.d3), dim3((unsigned)wgs, chunks), dim3(NU_THREADS),
                            0, st, J.lanes, n, J.spos, (const BPair *)bp, n_steps, spc,
                            (float *)part, 1.0f);
@@ -856,18 +920,20 @@ int run_job(imp_ctx *ctx, const Job &J) {
     const int G = J.nrhs <= 1 ? 1 : (J.nrhs <= 2 ? 2 : 4);
     const int groups = ceil_div(J.nrhs, G);
     void *ws = nullptr, *part = nullptr;
-    IMP_TRY(workspace(ctx, WS_BPACK, (size_t)groups * J.n_stream * 2 * G * sizeof(float), &ws));
+    IMP_TRY(S.get(WS_BPACK, (size_t)groups * J.n_stream * 2 * G * sizeof(float), &ws));
     hipLaunchKernelGGL(k_pack_valu, dim3(ceil_div(J.n_stream, 256), groups), dim3(256), 0, st,
                        J.n_stream, J.nrhs, G, J.W, (float *)ws);
     IMP_CHECK_LAUNCH();
     const int lane_blocks = ceil_div(n, NU_THREADS);
-    const int chunks = pick_chunks(ctx, (long)lane_blocks * groups, J.n_stream, 4 * NU_BLK);
-    int chunk_len = ceil_div(ceil_div(J.n_stream, chunks), NU_BLK) * NU_BLK;
     const bool combine = J.out == Out::Combine;
-    const size_t nparts = combine ? (size_t)groups * chunks : (size_t)chunks * J.nrhs;
-    IMP_TRY(workspace(ctx, WS_PARTIAL, nparts * n * sizeof(float2), &part));
+    const size_t per_chunk = (combine ? (size_t)groups : (size_t)J.nrhs) * n * sizeof(float2);
+    const int chunks = S.fit_chunks(
+        pick_chunks(ctx, (long)lane_blocks * groups, J.n_stream, 4 * NU_BLK), per_chunk);
+    if (chunks < 1) return fail(IMP_EUNSUPPORTED, "NUDFT: no room for one chunk's partials");
+    int chunk_len = ceil_div(ceil_div(J.n_stream, chunks), NU_BLK) * NU_BLK;
+    IMP_TRY(S.get(WS_PARTIAL, (size_t)chunks * per_chunk, &part));
     VFn fn = combine ? pick_valu<EPI_COMBINE>(G, J.d3) : pick_valu<EPI_PERCOIL>(G, J.d3);
-    const int rec = prof_begin(ctx, "k_nudft_valu");
+    const int rec = S.prof("k_nudft_valu");
     hipLaunchKernelGGL(fn, dim3(lane_blocks, chunks, groups), dim3(NU_THREADS), 0, st, J.lanes,
                        n, J.spos, (const float *)ws, J.n_stream, chunk_len, J.sign, J.nrhs,
                        J.s_r, J.s_i, (float2 *)part);
@@ -885,17 +951,31 @@ int run_job(imp_ctx *ctx, const Job &J) {
     return IMP_OK;
 }
 
-int build_spos(imp_ctx *ctx, int n, const float *x, const float *y, const float *z,
+int run_job(imp_ctx *ctx, const Job &J) {
+    Scratch S{ctx};
+    return run_job(ctx, J, S);
+}
+
+size_t spos_bytes(int n, bool d3) {
+    return (size_t)(ceil_div(n, MF_K)) * pos_stride(d3) * sizeof(double);
+}
+
+int build_spos(imp_ctx *ctx, Scratch &S, int n, const float *x, const float *y, const float *z,
                const float *w, double wscale, const double **out) {
     void *p = nullptr;
     const int n_pad = ceil_div(n, MF_K) * MF_K;
-    const size_t doubles = (size_t)(n_pad / MF_K) * pos_stride(z != nullptr);
-    IMP_TRY(workspace(ctx, WS_SPOS, doubles * sizeof(double), &p));
+    IMP_TRY(S.get(WS_SPOS, spos_bytes(n, z != nullptr), &p));
     hipLaunchKernelGGL(k_spos, dim3(ceil_div(n_pad, 256)), dim3(256), 0, ctx->stream, n, n_pad,
                        x, y, z, w, wscale, (double *)p);
     IMP_CHECK_LAUNCH();
     *out = (const double *)p;
     return IMP_OK;
+}
+
+int build_spos(imp_ctx *ctx, int n, const float *x, const float *y, const float *z,
+               const float *w, double wscale, const double **out) {
+    Scratch S{ctx};
+    return build_spos(ctx, S, n, x, y, z, w, wscale, out);
 }
 
 // ---- stacks of 2D trajectories (F^H d separable in z) -----------------------
@@ -1077,28 +1157,35 @@ int run_stack_adjoint(imp_ctx *ctx, int num_k, int K, int ncoils, const float4 *
 
 // ---- operator entry points (device pointers) --------------------------------
 
-int nudft_adjoint(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *kx,
-                  const float *ky, const float *kz, const float *t, const float *x,
-                  const float *y, const float *z, const float *fm,
-                  const float *phi_r, const float *phi_i, const float *d_r,
-                  const float *d_i, const float *s_r, const float *s_i,
-                  float *out_r, float *out_i) {
-    if (!ctx || num_k <= 0 || num_x <= 0 || ncoils <= 0 || !kx || !ky || !t ||
-        !x || !y || !d_r || !d_i || !out_r || !out_i || (s_r && !s_i) ||
-        (phi_r && !phi_i))
-        return fail(IMP_EINVAL, "nudft_adjoint: bad arguments");
+namespace {
+
+bool adjoint_args_ok(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *kx,
+                     const float *ky, const float *t, const float *x, const float *y,
+                     const float *phi_r, const float *phi_i, const float *d_r, const float *d_i,
+                     const float *s_r, const float *s_i, const float *out_r, const float *out_i) {
+    return ctx && num_k > 0 && num_x > 0 && ncoils > 0 && kx && ky && t && x && y && d_r &&
+           d_i && out_r && out_i && !(s_r && !s_i) && !(phi_r && !phi_i);
+}
+
+// F^H d with the scratch of S; stacked trajectories take the separable form
+// only when allow_stack (its detection synchronises with the host).
+int adjoint_impl(imp_ctx *ctx, Scratch &S, bool allow_stack, int num_k, int num_x, int ncoils,
+                 const float *kx, const float *ky, const float *kz, const float *t,
+                 const float *x, const float *y, const float *z, const float *fm,
+                 const float *phi_r, const float *phi_i, const float *d_r, const float *d_i,
+                 const float *s_r, const float *s_i, float *out_r, float *out_i) {
     const bool d3 = kz && z;
     int K = 0;
-    if (d3) IMP_TRY(detect_stack(ctx, num_k, ncoils, num_x, kx, ky, kz, t, &K));
+    if (d3 && allow_stack) IMP_TRY(detect_stack(ctx, num_k, ncoils, num_x, kx, ky, kz, t, &K));
     const double *spos = nullptr;
-    if (K == 0) IMP_TRY(build_spos(ctx, num_k, kx, ky, d3 ? kz : nullptr, t, 1.0, &spos));
+    if (K == 0) IMP_TRY(build_spos(ctx, S, num_k, kx, ky, d3 ? kz : nullptr, t, 1.0, &spos));
     void *lanes = nullptr, *W = nullptr;
-    IMP_TRY(workspace(ctx, WS_LANES, (size_t)num_x * sizeof(float4), &lanes));
+    IMP_TRY(S.get(WS_LANES, (size_t)num_x * sizeof(float4), &lanes));
     hipLaunchKernelGGL(k_lanes_from_arrays, dim3(ceil_div(num_x, 256)), dim3(256), 0,
                        ctx->stream, num_x, x, y, d3 ? z : nullptr, fm, 1.0 / (2.0 * M_PI),
                        (float4 *)lanes);
     IMP_CHECK_LAUNCH();
-    IMP_TRY(workspace(ctx, WS_W, (size_t)num_k * ncoils * sizeof(float2), &W));
+    IMP_TRY(S.get(WS_W, (size_t)num_k * ncoils * sizeof(float2), &W));
     hipLaunchKernelGGL(k_w_adjoint, dim3(ceil_div((long)num_k * ncoils, 256)), dim3(256), 0,
                        ctx->stream, num_k, ncoils, phi_r, phi_i, d_r, d_i, (float2 *)W);
     IMP_CHECK_LAUNCH();
@@ -1107,8 +1194,25 @@ int nudft_adjoint(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *k
                                  kz, t, z, (const float2 *)W, s_r, s_i, out_r, out_i);
     Job J{(const float4 *)lanes, num_x, spos, num_k, (const float2 *)W, ncoils, d3, +1.f,
           Out::Combine, s_r, s_i, out_r, out_i};
-    return run_job(ctx, J);
+    return run_job(ctx, J, S);
 }
+
+}  // namespace
+
+int nudft_adjoint(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *kx,
+                  const float *ky, const float *kz, const float *t, const float *x,
+                  const float *y, const float *z, const float *fm,
+                  const float *phi_r, const float *phi_i, const float *d_r,
+                  const float *d_i, const float *s_r, const float *s_i,
+                  float *out_r, float *out_i) {
+    if (!adjoint_args_ok(ctx, num_k, num_x, ncoils, kx, ky, t, x, y, phi_r, phi_i, d_r, d_i,
+                         s_r, s_i, out_r, out_i))
+        return fail(IMP_EINVAL, "nudft_adjoint: bad arguments");
+    Scratch S{ctx};
+    return adjoint_impl(ctx, S, true, num_k, num_x, ncoils, kx, ky, kz, t, x, y, z, fm, phi_r,
+                        phi_i, d_r, d_i, s_r, s_i, out_r, out_i);
+}
+
 
 // F^H d of several images (different sizes, trajectories, coil maps) with
 // one persistent MFMA launch over all of them.  Every job needs 3..32 coils

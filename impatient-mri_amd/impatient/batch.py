@@ -107,8 +107,11 @@ def _plan(ctx, p, qr, qi):
     return h, (fm, sr, si)
 
 
-def reconstruct(ctx: Context, slices, tol=1e-5, max_iter=500, model=None, tuner=None, log=print):
-    """Batched reconstruction; returns per-slice records and totals."""
+def reconstruct(ctx: Context, slices, tol=1e-5, max_iter=500, model=None, tuner=None, log=print,
+                keep_images=False):
+    """Batched reconstruction; returns per-slice records and totals
+    (keep_images: each record also holds its F^H d "b" and image "x" as
+    host arrays, copied after the slice's timing ends)."""
     tuner = tuner or Autotuner(ctx)
     t_all = time.perf_counter()
     # 1. F^H d of every slice in one persistent launch (autotuned chunking)
@@ -172,6 +175,8 @@ def reconstruct(ctx: Context, slices, tol=1e-5, max_iter=500, model=None, tuner=
         call("imp_toep_destroy", h)
         r = st.as_dict()
         r.update(N=N, slice_s=t_slice, cpg=cpg)
+        if keep_images:
+            r.update(b=np.asarray(b), x=xr.get() + 1j * xi.get())
         recs.append(r)
     t_total = time.perf_counter() - t_all
     for cl in classes.values():

@@ -267,15 +267,114 @@ def probe_pairs(geo: Geometry, n: int, seed0: int, log=print):
     return pairs
 
 
+# The preconditioner network: a second instance of the same CNN, run on the
+# frequency plane.  Input z (real, Im = 0): log w_chan(k), mean removed,
+# fftshifted so DC sits at the image centre and the sampled disk is one
+# connected region.  Output x = z + s net(z / s); the weights are
+#   w(k) = w_chan(k) exp(clip(Re x(k) - z(k), -PC_CLIP, PC_CLIP)),
+# real and positive, so M = F^H diag(1/w) F stays Hermitian positive
+# definite and PCG stays valid whatever the network outputs.
+PC_CLIP = 3.0
+
+
+def precond_input(chan):
+    """Network input from the Chan weights (natural DFT order, image shape):
+    (z as complex64, the removed mean)."""
+    lz = np.log(np.asarray(chan, np.float64))
+    mu = float(lz.mean())
+    return np.fft.fftshift(lz - mu).astype(np.complex64), mu
+
+
+def precond_from_output(x, z, chan):
+    """Weights from the network output x for input z (both fftshifted)."""
+    d = np.clip(np.asarray(x).real - np.asarray(z).real, -PC_CLIP, PC_CLIP)
+    return (np.asarray(chan, np.float64) * np.exp(np.fft.ifftshift(d))).astype(np.float32)
+
+
+def precond_targets(geo: Geometry, n: int, probes: int, seed0: int, log=print):
+    """Training targets of the preconditioner network, one per draw of a
+    random field map: the per-frequency least-squares weight of
+    M^-1 A v ~ v over `probes` white probes v,
+      w(k) = sum Re(conj(AV(k)) V(k)) / sum |AV(k)|^2   (V = DFT v, AV = DFT A v),
+    which is 1/lambda(k) for a circulant A; returned as the target image
+    z + log(w / w_chan) (fftshifted), clipped to +-PC_CLIP.  Noisy per draw
+    (a few probes per frequency): trained with a squared loss the network
+    learns their mean, and its 3x3 stencils smooth it over frequency."""
+    prob = geo.prob
+    N1, N2 = prob.Ny, prob.Nx
+    X = prob.x.reshape(N1, N2).astype(np.float64)
+    Y = prob.y.reshape(N1, N2).astype(np.float64)
+    chan = geo.chan.get().reshape(N1, N2).astype(np.float64)
+    z, _ = precond_input(chan)
+    yr, yi = geo.ctx.alloc((geo.nv,)), geo.ctx.alloc((geo.nv,))
+    vr, vi = geo.ctx.alloc((geo.nv,)), geo.ctx.alloc((geo.nv,))
+    out = []
+    t0 = time.perf_counter()
+    for k in range(n):
+        rng = np.random.default_rng(seed0 + k)
+        geo.set_fieldmap(synth.field_map(rng, X, Y, 4, prob.meta["max_hz"]))
+        num = np.zeros((N1, N2))
+        den = np.zeros((N1, N2))
+        for _ in range(probes):
+            v = (rng.standard_normal(geo.nv) + 1j * rng.standard_normal(geo.nv)).astype(np.complex64)
+            vr.upload(v.real)
+            vi.upload(v.imag)
+            call("imp_toep_apply", geo.op, ptr(vr), ptr(vi), ptr(yr), ptr(yi))
+            av = (yr.get() + 1j * yi.get()).reshape(N1, N2)
+            V = np.fft.fft2(v.reshape(N1, N2))
+            AV = np.fft.fft2(av)
+            num += (np.conj(AV) * V).real
+            den += np.abs(AV) ** 2
+        w = np.maximum(num, 1e-30 * den.max()) / np.maximum(den, 1e-300)
+        d = np.clip(np.log(w / chan), -PC_CLIP, PC_CLIP)
+        out.append((z + np.fft.fftshift(d)).astype(np.complex64))
+        if (k + 1) % max(1, n // 5) == 0 or k == n - 1:
+            log(f"preconditioner targets {k + 1}/{n} ({probes} probes each), "
+                f"{time.perf_counter() - t0:.1f} s")
+    return z, out
+
+
+def train_precond(net: ops.CNN, z, targets, epochs, lr=1e-3, seed=0, log=print):
+    """Adam (the network's own kernels) on (z, target) for `epochs` passes."""
+    ctx = net.ctx
+    zr = ctx.to_device(np.ascontiguousarray(z.real.reshape(-1), np.float32))
+    zi = ctx.to_device(np.ascontiguousarray(z.imag.reshape(-1), np.float32))
+    tr, ti = ctx.alloc((z.size,)), ctx.alloc((z.size,))
+    rng = np.random.default_rng(seed)
+    total, step, hist = epochs * len(targets), 0, []
+    for ep in range(epochs):
+        losses = []
+        for i in rng.permutation(len(targets)):
+            tr.upload(np.ascontiguousarray(targets[i].real.reshape(-1), np.float32))
+            ti.upload(np.ascontiguousarray(targets[i].imag.reshape(-1), np.float32))
+            lr_t = lr * 0.5 * (1 + np.cos(np.pi * step / total))
+            losses.append(net.train_step_device(zr, zi, tr, ti, max(lr_t, lr * 1e-3)))
+            step += 1
+        hist.append(float(np.mean(losses)))
+    log(f"preconditioner network: {epochs} epochs over {len(targets)} targets, loss "
+        f"{hist[0]:.3e} -> {hist[-1]:.3e}")
+    return hist
+
+
+def network_weights(net: ops.CNN, chan):
+    """The preconditioner network's weights for an operator whose Chan
+    weights are `chan` (natural DFT order, image shape)."""
+    z, _ = precond_input(chan)
+    return precond_from_output(net.forward(z), z, chan)
+
+
 def learned_start(geo: Geometry, net: ops.CNN, xr, xi):
     """x0 = z0 + s * net(z0 / s) with z0 = M^-1 b (current preconditioner)."""
     geo.precond(geo.br, geo.bi, geo.rr, geo.ri)
     net.forward_device(geo.rr, geo.ri, xr, xi)
 
 
-def evaluate(geo: Geometry, net: ops.CNN, gains, n, seed0, snr_db=40.0, tol=1e-5, log=print):
+def evaluate(geo: Geometry, net: ops.CNN, gains, n, seed0, snr_db=40.0, tol=1e-5, log=print,
+             pc_weights=None):
     """Held-out draws: iterations / wall time to tol for plain CG, Chan PCG,
-    and the learned start with Chan or learned-gain preconditioning."""
+    and the learned start with Chan or learned-gain preconditioning; with
+    pc_weights (the preconditioner network's weights), PCG with those from
+    zero ("netw") and from the learned start ("netw_learned")."""
     prob = geo.prob
     X = prob.x.reshape(prob.Ny, prob.Nx).astype(np.float64)
     Y = prob.y.reshape(prob.Ny, prob.Nx).astype(np.float64)
@@ -318,20 +417,37 @@ def evaluate(geo: Geometry, net: ops.CNN, gains, n, seed0, snr_db=40.0, tol=1e-5
             r["gains_learned"] = geo.solve(xr, xi, tol, flags=base | IMP_CG_PRECOND | START)
             r["gains_learned"]["start_s"] = t_start
             call("imp_toep_set_precond", geo.op, None)
+        if pc_weights is not None:
+            dw = geo.ctx.to_device(np.ascontiguousarray(np.asarray(pc_weights).reshape(-1),
+                                                        np.float32))
+            call("imp_toep_set_precond", geo.op, ptr(dw))
+            r["netw"] = geo.solve(xr, xi, tol, flags=base | IMP_CG_PRECOND)
+            call("imp_toep_set_precond", geo.op, None)
+            learned_start(geo, net, xr, xi)
+            call("imp_toep_set_precond", geo.op, ptr(dw))
+            r["netw_learned"] = geo.solve(xr, xi, tol, flags=base | IMP_CG_PRECOND | START)
+            call("imp_toep_set_precond", geo.op, None)
         rows.append(r)
         log("draw %d: " % k + ", ".join(f"{m} {v['iterations']} it" for m, v in r.items()))
     return rows
 
 
-def save_model(path, params, gains, meta):
+def save_model(path, params, gains, meta, params_pc=None):
+    extra = {} if params_pc is None else {"params_pc": np.asarray(params_pc, np.float32)}
     np.savez(path, params=np.asarray(params, np.float32), gains=np.asarray(gains, np.float32),
-             meta=np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8))
+             meta=np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8), **extra)
 
 
 def load_model(path):
     with np.load(path, allow_pickle=False) as z:
         meta = json.loads(bytes(z["meta"]).decode())
         return z["params"].astype(np.float32), z["gains"].astype(np.float32), meta
+
+
+def load_precond_params(path):
+    """The preconditioner network's parameters of a saved model, or None."""
+    with np.load(path, allow_pickle=False) as z:
+        return z["params_pc"].astype(np.float32) if "params_pc" in z.files else None
 
 
 def main(argv=None):
@@ -345,6 +461,12 @@ def main(argv=None):
     ap.add_argument("--pair-tol", type=float, default=1e-6,
                     help="relative residual the training solutions x* are solved to")
     ap.add_argument("--out", required=True)
+    ap.add_argument("--precond-from", default=None, metavar="MODEL",
+                    help="keep MODEL's warm-start network and gains; train only the "
+                         "preconditioner network and write MODEL plus it to --out")
+    ap.add_argument("--precond-draws", type=int, default=64)
+    ap.add_argument("--precond-probes", type=int, default=4)
+    ap.add_argument("--precond-epochs", type=int, default=40)
     args = ap.parse_args(argv)
 
     def log(m):
@@ -354,24 +476,69 @@ def main(argv=None):
     seed0 = args.seed if args.seed is not None else prob.meta["seed"] * 10
     ctx = Context(0)
     geo = Geometry(ctx, prob)
+    if args.precond_from:
+        return precond_main(args, ctx, geo, seed0, log)
     log(f"{prob.name}: {args.draws} training draws from seed {seed0}")
     pairs = make_pairs(geo, args.draws, seed0, tol=args.pair_tol, log=log)
     net = ops.CNN(ctx, prob.Ny, prob.Nx, seed=seed0)
     hist = train(net, pairs, args.epochs, args.lr, log=log)
     gains = fit_radial_gains(geo, probe_pairs(geo, 24, seed0 + 500000, log=log))
     log("radial gains: " + " ".join(f"{g:.3f}" for g in gains[::8]))
-    rows = evaluate(geo, net, gains, args.val, seed0 + args.draws + 1000, log=log)
+    # the preconditioner network
+    net_pc = ops.CNN(ctx, prob.Ny, prob.Nx, seed=seed0 + 7)
+    z, targets = precond_targets(geo, args.precond_draws, args.precond_probes, seed0 + 700000,
+                                 log=log)
+    hist_pc = train_precond(net_pc, z, targets, args.precond_epochs, log=log)
+    w_net = network_weights(net_pc, geo.chan.get().reshape(prob.Ny, prob.Nx))
+    rows = evaluate(geo, net, gains, args.val, seed0 + args.draws + 1000, log=log,
+                    pc_weights=w_net)
     val = {m: [r[m]["iterations"] for r in rows] for m in rows[0]}
     # keep the learned gains only if they beat the Chan weights on held-out draws
     use_gains = float(np.mean(val["gains_learned"])) < float(np.mean(val["chan_learned"]))
+    use_netw = float(np.mean(val["netw_learned"])) < min(float(np.mean(val["gains_learned"])),
+                                                         float(np.mean(val["chan_learned"])))
     log(f"validation mean iterations: " + ", ".join(f"{m} {np.mean(v):.1f}" for m, v in val.items())
-        + f"; learned gains {'kept' if use_gains else 'not used (Chan weights are better)'}")
+        + f"; learned gains {'kept' if use_gains else 'not used (Chan weights are better)'}; "
+        f"network weights {'kept' if use_netw else 'not used'}")
     meta = dict(config=prob.name, draws=args.draws, epochs=args.epochs, lr=args.lr,
                 pair_tol=args.pair_tol,
                 seed0=seed0, loss_history=hist, nbins=NBINS, use_gains=use_gains,
-                validation=val)
-    save_model(args.out, net.get_params(), gains, meta)
+                use_netw=use_netw, precond_draws=args.precond_draws,
+                precond_probes=args.precond_probes, precond_epochs=args.precond_epochs,
+                precond_loss_history=hist_pc, validation=val)
+    save_model(args.out, net.get_params(), gains, meta, net_pc.get_params())
     log(f"saved {args.out}")
+    net_pc.close()
+    net.close()
+    geo.close()
+    ctx.close()
+    return 0
+
+
+def precond_main(args, ctx, geo, seed0, log):
+    """--precond-from: add a preconditioner network to a trained model."""
+    prob = geo.prob
+    params, gains, meta = load_model(args.precond_from)
+    net = ops.CNN(ctx, prob.Ny, prob.Nx, seed=1)
+    net.set_params(params)
+    net_pc = ops.CNN(ctx, prob.Ny, prob.Nx, seed=seed0 + 7)
+    z, targets = precond_targets(geo, args.precond_draws, args.precond_probes, seed0 + 700000,
+                                 log=log)
+    hist_pc = train_precond(net_pc, z, targets, args.precond_epochs, log=log)
+    w_net = network_weights(net_pc, geo.chan.get().reshape(prob.Ny, prob.Nx))
+    rows = evaluate(geo, net, gains, args.val, seed0 + meta.get("draws", 0) + 1000, log=log,
+                    pc_weights=w_net)
+    val = {m: [r[m]["iterations"] for r in rows] for m in rows[0]}
+    use_netw = float(np.mean(val["netw_learned"])) < min(float(np.mean(val["gains_learned"])),
+                                                         float(np.mean(val["chan_learned"])))
+    log("validation mean iterations: " + ", ".join(f"{m} {np.mean(v):.1f}" for m, v in val.items())
+        + f"; network weights {'kept' if use_netw else 'not used'}")
+    meta.update(use_netw=use_netw, precond_draws=args.precond_draws,
+                precond_probes=args.precond_probes, precond_epochs=args.precond_epochs,
+                precond_loss_history=hist_pc, precond_validation=val)
+    save_model(args.out, params, gains, meta, net_pc.get_params())
+    log(f"saved {args.out}")
+    net_pc.close()
     net.close()
     geo.close()
     ctx.close()

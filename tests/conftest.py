@@ -17,6 +17,15 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def ctx():
+    # Some GPU tests check against FP64 restatements run with torch on the
+    # GPU (tests/fp64_torch.py).  torch's HIP runtime has to be initialised
+    # before libimpatient.so is loaded: once the library has brought in the
+    # system HIP runtime, torch finds no device.
+    try:
+        import torch
+        torch.cuda.is_available()
+    except ImportError:
+        pass
     from impatient import Context
     c = Context(int(os.environ.get("IMP_TEST_DEVICE", "0")))
     yield c

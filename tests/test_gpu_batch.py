@@ -34,3 +34,63 @@ def test_batch_slices_converge_and_match_single_slice_path(ctx):
         print(f"N={N}: single path {st['iterations']} it vs batch {recs[i]['iterations']} it")
         assert abs(st["iterations"] - recs[i]["iterations"]) <= 2
         op.close()
+
+
+def test_config4_48_slices_at_size(ctx):
+    """The 48-slice workload at its stated size: sizes drawn from
+    {128^2, 256^2, 384^2}, 8 coils, L = 8, +-150 Hz, the trained warm-start
+    model with the zero-start fallback, graph-captured PCG to 1e-5.
+
+    Every slice's true residual (recomputed on the GPU after the solve) is
+    below 1e-5, and so is its residual under the FP64 operator (the oracle's
+    Toeplitz operator in complex128 on the GPU's FP64 FFT, from the same Q_l);
+    the batched F^H d of the first slice of each size class matches the FP64
+    oracle at 16 sampled voxels."""
+    from pathlib import Path
+    import torch
+    from impatient import batch, learn, ops
+    from oracle import oracle as orc
+    import fp64_torch as ft
+    model = Path(__file__).resolve().parents[1] / "impatient-mri_amd" / "models" / "cfg3_warmstart.npz"
+    params = learn.load_model(model)[0]
+    sl = batch.make_slices(48, ctx=ctx)
+    sizes = sorted({s.prob.Nx for s in sl})
+    assert sizes == [128, 256, 384]
+    res = batch.reconstruct(ctx, sl, tol=1e-5, model=params, keep_images=True)
+    recs = res["slices"]
+    print(f"48 slices: {res['total_s']:.3f} s (first pass, autotuner trials included), "
+          f"F^H d batch {res['fhd_s'] * 1e3:.1f} ms, Q {res['q_s'] * 1e3:.1f} ms")
+    assert len(recs) == 48
+    assert all(r["converged"] == 1 for r in recs)
+    assert max(r["true_rel_residual"] for r in recs) < 1e-5
+    # FP64-operator residual of every image, per size class Q_l
+    qs, worst = {}, {}
+    for s, r in zip(sl, recs):
+        p = s.prob
+        N = p.Nx
+        if N not in qs:
+            qs[N] = np.asarray(ops.toeplitz_q(ctx, p.kx, p.ky, None, p.t, N, N, 1, p.ntime_segs))
+        tau, t0 = orc.segment_timing(p.t, p.ntime_segs)
+        A = ft.ToeplitzOperator(qs[N], tau, t0, p.shape, p.fm, p.sens, p.lam)
+        b = torch.as_tensor(r["b"].astype(np.complex128), device="cuda")
+        res64 = (torch.linalg.norm(A(r["x"].astype(np.complex128)) - b) /
+                 torch.linalg.norm(b)).item()
+        worst[N] = max(worst.get(N, 0.0), res64)
+        assert res64 < 1e-5, (N, res64)
+    for N in sizes:
+        v = [r for r in recs if r["N"] == N]
+        print(f"N={N}: {len(v)} slices, mean {np.mean([r['iterations'] for r in v]):.1f} it, "
+              f"learned start used {sum(r['start_used'] for r in v)}, "
+              f"max GPU true residual {max(r['true_rel_residual'] for r in v):.2e}, "
+              f"max FP64-operator residual {worst[N]:.2e}")
+    # batched F^H d of the first slice of each class against the FP64 oracle
+    rng = np.random.default_rng(48)
+    for N in sizes:
+        i = next(k for k, s in enumerate(sl) if s.prob.Nx == N)
+        p = sl[i].prob
+        idx = np.sort(rng.choice(p.nvox, 16, replace=False))
+        ref = orc.nudft_adjoint(p.kx, p.ky, None, p.t, p.x[idx], p.y[idx], None, p.fm[idx],
+                                sl[i].d, p.sens[:, idx])
+        e = rel_l2(recs[i]["b"][idx], ref)
+        print(f"N={N}: batched F^H d at 16 voxels vs FP64 {e:.2e}")
+        assert e < 2e-6

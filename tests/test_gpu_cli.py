@@ -187,7 +187,8 @@ def test_cli_option_implications(tmp_path, data_dir):
     runs = {}
     for name, extra in (("fdp", ["-fdp", "0.01"]), ("fd_fdp", ["-fd", "-fdp", "0.01"]),
                         ("none", []), ("tvnum", ["-tv_num", "3", "-fdp", "0.01"]),
-                        ("tv_tvnum", ["-tv", "-tv_num", "3", "-fdp", "0.01"])):
+                        ("tv_tvnum", ["-tv", "-tv_num", "3", "-fdp", "0.01"]),
+                        ("fdp_big", ["-fdp", "100"])):
         d = tmp_path / name
         shutil.copytree(data_dir, d)
         assert _cli(["-idir", str(d), "-odir", str(d / "o"), "-cg_num", "5", *extra]).returncode == 0
@@ -199,8 +200,10 @@ def test_cli_option_implications(tmp_path, data_dir):
     assert np.array_equal(runs["fdp"], runs["fd_fdp"])
     assert np.array_equal(runs["fdp_toep"], runs["fd_fdp_toep"])
     assert np.array_equal(runs["tvnum"], runs["tv_tvnum"])
-    # the weight does act: without it the solve is unregularized
-    assert rel_l2(runs["fdp"], runs["none"]) > 1e-4
+    # the weight acts (lambda is absolute: against F^H F of 3770 samples,
+    # 0.01 moves the image by about 1e-6 and 100 by several percent)
+    assert rel_l2(runs["fdp"], runs["none"]) > 0
+    assert rel_l2(runs["fdp_big"], runs["none"]) > 1e-2
     # bad values are rejected as in the reference
     assert _cli(["-idir", str(data_dir), "-tv_num", "0", "-fdp", "0.01"]).returncode != 0
     assert _cli(["-idir", str(data_dir), "-tv_num", "-fdp", "0.01"]).returncode != 0
@@ -298,4 +301,6 @@ def test_cli_toeplitz_gridding(data_dir):
     ex = rel_l2(x.reshape(n, n).T.reshape(-1), x32)
     print(f"CLI gridding: Q vs FP64 gridding {eq:.2e}, F^H d {eb:.2e}, image vs FP32-storage "
           f"oracle CG {ex:.2e}")
-    assert eq < 5e-6 and eb < 5e-6 and ex < 5e-3
+    # FP32 sample positions s = os (2k + n/2) on a 160-point grid (as the
+    # reference computes them) against the FP64 restatement: ~1e-5
+    assert eq < 2e-5 and eb < 2e-5 and ex < 5e-3

@@ -59,23 +59,38 @@ def test_gridding_is_deterministic(ctx):
 
 
 def test_gridding_against_direct_at_config3_size(ctx):
-    """512^2, 48 x 8192 spiral, L = 10 (the 32-coil workload's trajectory):
-    the gridded Q_l against the direct (separable GEMM) Q_l, and the gridded
-    F^H d of 4 coils against the direct time-segmented one, at the reference's
-    demo oversampling (gridOS_Q 1.25, gridOS_FHD 1.375; run_all_demos.sh:4)."""
+    """512^2, 48 x 8192 spiral, L = 10 (the 32-coil workload's trajectory) at
+    the reference's demo oversampling (gridOS_Q 1.25, gridOS_FHD 1.375;
+    run_all_demos.sh:4), against the direct operators: the method's own
+    approximation error, "around 1%" (README:151-154).
+
+    Q_l on the 2N grid is checked over its central half: at gridOS 1.25 the
+    Kaiser-Bessel aliasing concentrates in the outer half (3.5% there on a
+    64^2 oracle run, 0.3% in the centre), where Q_l only pairs voxels more
+    than N/2 apart.  What the solve sees is the Toeplitz apply, checked on
+    the phantom with 4 of the coils, and F^H d with the same time
+    segmentation."""
     from impatient import ops, synth
     from oracle import oracle as orc
     p = synth.config(3)
-    L = p.ntime_segs
-    qg = ops.toeplitz_q_gridding(ctx, p.kx, p.ky, None, p.t, p.Nx, p.Ny, 1, L, 1.25)
-    qd = ops.toeplitz_q(ctx, p.kx, p.ky, None, p.t, p.Nx, p.Ny, 1, L)
-    eq = rel_l2(qg, qd)
+    L, N = p.ntime_segs, p.Nx
+    qg = np.asarray(ops.toeplitz_q_gridding(ctx, p.kx, p.ky, None, p.t, p.Nx, p.Ny, 1, L, 1.25))
+    qd = np.asarray(ops.toeplitz_q(ctx, p.kx, p.ky, None, p.t, p.Nx, p.Ny, 1, L))
+    c = (slice(None), slice(N // 2, 3 * N // 2), slice(N // 2, 3 * N // 2))
+    eq_all = rel_l2(qg, qd)
+    eq_mid = rel_l2(qg.reshape(L + 1, 2 * N, 2 * N)[c], qd.reshape(L + 1, 2 * N, 2 * N)[c])
+    tau, t0 = orc.segment_timing(p.t, L)
+    sens = p.sens[::8]
+    opg = ops.Toeplitz(ctx, p.shape, qg, tau, t0, p.fm, sens, 0.0)
+    opd = ops.Toeplitz(ctx, p.shape, qd, tau, t0, p.fm, sens, 0.0)
+    rho = p.rho.astype(np.complex64)
+    ea = rel_l2(opg.apply(rho), opd.apply(rho))
+    opg.close()
+    opd.close()
     rng = np.random.default_rng(63)
     d = (rng.standard_normal((4, p.nk)) + 1j * rng.standard_normal((4, p.nk))).astype(np.complex64)
-    sens = p.sens[:4]
     bg = ops.fhd_gridding(ctx, p.kx, p.ky, None, p.t, p.Nx, p.Ny, 1, p.fm, d, sens, L, 1.375)
     # the direct F^H d with the same time segmentation
-    tau, t0 = orc.segment_timing(p.t, L)
     bd = np.zeros(p.nvox, np.complex128)
     for l in range(L + 1):
         a = orc.hanning(p.t, tau, l, t0).astype(np.float32)
@@ -83,6 +98,7 @@ def test_gridding_against_direct_at_config3_size(ctx):
                                  d * a[None, :], sens)
         bd += np.exp(1j * p.fm.astype(np.float64) * (tau * l + t0)) * part
     ef = rel_l2(bg, bd)
-    print(f"config-3 size: gridded Q (os 1.25) vs direct {eq:.2e}; gridded F^H d (os 1.375) "
-          f"vs direct time-segmented {ef:.2e}")
-    assert eq < 0.03 and ef < 0.03
+    print(f"config-3 size: gridded Q (os 1.25) vs direct {eq_all:.2e} (central half "
+          f"{eq_mid:.2e}); Toeplitz apply on the phantom {ea:.2e}; gridded F^H d (os 1.375) vs "
+          f"direct time-segmented {ef:.2e}")
+    assert eq_mid < 0.01 and ea < 0.01 and ef < 0.03

@@ -13,10 +13,11 @@ from conftest import difference_coo, rel_l2
 pytestmark = pytest.mark.gpu
 
 
-def _recon_dir(ctx, tmp_path, Nz, ncoils, L, seed):
+def _recon_dir(ctx, tmp_path, Nz, ncoils, L, seed, scale=1.0):
     from impatient import files, ops, synth
     from oracle import oracle as orc
     p = synth.tiny(N=16, Nz=Nz, ncoils=ncoils, ntime_segs=L, seed=seed)
+    p.rho = p.rho * scale
     d = orc.nudft_forward(p.kx, p.ky, p.kz, p.t, p.x, p.y, p.z, p.fm, p.rho, p.sens)
     d = synth.add_noise(d, 30.0, seed)
     files.write_problem(tmp_path, p, d)
@@ -49,7 +50,7 @@ def _run_recon(tmp_path, p, b, q, lam, iters, tv_num, enable_tv, restarts=1):
     return img[0] + 1j * img[1]
 
 
-def _oracle_recon(p, b, q, lam, iters, tv_num, enable_tv, restarts=1):
+def _oracle_recon(p, b, q, lam, iters, tv_num, enable_tv, restarts=1, vec=np.complex64):
     from oracle import oracle as orc
     L = q.shape[0] - 1
     tau, t0 = orc.segment_timing(p.t, L)
@@ -57,40 +58,79 @@ def _oracle_recon(p, b, q, lam, iters, tv_num, enable_tv, restarts=1):
     make = lambda w: orc.toeplitz_operator(q, tau, t0, p.shape, p.fm, p.sens, lam, w)
     x, hists = orc.toeplitz_recon(make, b.astype(np.complex128), shape, iters,
                                   enable_tv=enable_tv, tv_num=tv_num, restarts=restarts,
-                                  vec_dtype=np.complex64)
+                                  vec_dtype=vec)
     return x, hists
+
+
+# The TV weight w = 0.5 / sqrt(|D x|^2 + 1e-6) of the current image makes
+# the later passes sensitive to rounding where |D x|^2 is near the 1e-6
+# floor: on the unit-scale phantom below the oracle's own FP32-vector run
+# leaves its FP64 run by 1e-3 (3D) to 8e-2 (2D) after three passes, while a
+# single pass agrees to 1e-5.  Scaling the data by 1e3 (lambda 0.5) moves
+# |D x|^2 well above the floor, and in 3D the oracle's FP32 and FP64 runs
+# then agree to 3e-6: there the GPU is held to 1e-4.  In 2D every setting
+# tried (N 16-24, 512-2048 samples, lambda 2e-3-5, SNR 30-60 dB) keeps a
+# 1e-2-8e-2 FP32/FP64 spread: there the GPU is held to that spread.
+TV_SCALED = (1e3, 0.5)
 
 
 @pytest.mark.parametrize("Nz", [1, 4])
 def test_toeplitz_recon_tv_passes_match_oracle(ctx, tmp_path, Nz):
-    """-tv -tv_num 3 on the Toeplitz driver, 2D and 3D: three passes of 5
-    iterations, each after the first with W = tv_weight of the current image,
-    against the oracle's restatement of recon.cu:2011-2133 with FP32
-    vectors.  The TV passes move the image well beyond that tolerance."""
+    """-tv -tv_num 3 on the Toeplitz driver, 2D and 3D, unit-scale phantom:
+    three passes of 5 iterations, each after the first with W = tv_weight of
+    the current image, against the oracle's restatement of
+    recon.cu:2011-2133 with FP32 vectors.  One pass is held to 1e-4; three
+    passes to the oracle's own FP32-vs-FP64 spread (in 2D that spread,
+    8.5e-2, is a third of what the TV passes change: there the later passes
+    are not fixed by the arithmetic), and the GPU's distance is under a
+    tenth of the TV passes' change of the image."""
     lam = 2e-3
     p, b, q = _recon_dir(ctx, tmp_path, Nz, 2, 3, seed=41 + Nz)
     x = _run_recon(tmp_path, p, b, q, lam, 5, 3, True)
     ref, hists = _oracle_recon(p, b, q, lam, 5, 3, True)
+    ref64, _ = _oracle_recon(p, b, q, lam, 5, 3, True, vec=np.complex128)
     one = _run_recon(tmp_path, p, b, q, lam, 5, 3, False)
     ref1, _ = _oracle_recon(p, b, q, lam, 5, 1, False)
     e_tv, e_one, moved = rel_l2(x, ref), rel_l2(one, ref1), rel_l2(ref, ref1)
-    print(f"Nz={Nz}: 3 TV passes vs FP32-storage oracle {e_tv:.2e}; one pass {e_one:.2e}; "
-          f"TV moves the image by {moved:.2e}; pass residuals "
-          f"{[round(h[-1], 6) for h in hists]}")
+    spread = rel_l2(ref, ref64)
+    print(f"Nz={Nz}: 3 TV passes vs FP32-storage oracle {e_tv:.2e} (oracle FP32 vs FP64 "
+          f"{spread:.2e}); one pass {e_one:.2e}; TV moves the image by {moved:.2e}")
     assert len(hists) == 3
-    assert e_tv < 1e-4 and e_one < 1e-4
-    assert moved > 100 * e_tv
+    assert e_one < 1e-4
+    assert e_tv < max(1e-4, 2 * spread)
+    assert moved > 10 * e_tv
+
+
+def test_toeplitz_recon_tv_passes_match_oracle_well_conditioned(ctx, tmp_path):
+    """The same three TV passes in 3D on data scaled by 1e3 (lambda 0.5),
+    where the oracle's FP32 and FP64 runs agree to a few 1e-6: GPU against
+    the FP32-storage oracle within 1e-4."""
+    scale, lam = TV_SCALED
+    p, b, q = _recon_dir(ctx, tmp_path, 4, 2, 3, seed=45, scale=scale)
+    x = _run_recon(tmp_path, p, b, q, lam, 5, 3, True)
+    ref, hists = _oracle_recon(p, b, q, lam, 5, 3, True)
+    ref64, _ = _oracle_recon(p, b, q, lam, 5, 3, True, vec=np.complex128)
+    ref1, _ = _oracle_recon(p, b, q, lam, 5, 1, False)
+    e, spread, moved = rel_l2(x, ref), rel_l2(ref, ref64), rel_l2(ref, ref1)
+    print(f"3D scaled: 3 TV passes vs FP32-storage oracle {e:.2e} (oracle FP32 vs FP64 "
+          f"{spread:.2e}); TV moves the image by {moved:.2e}")
+    assert len(hists) == 3
+    assert e < 1e-4 and moved > 1e-2
 
 
 def test_toeplitz_recon_restarts_match_oracle(ctx, tmp_path):
     """numRestarts = 2 with 2 TV passes (argv[4], reachable through the C ABI):
-    restart 1 carries beta and the best residual (recon.cu:2043-2048)."""
-    lam = 2e-3
-    p, b, q = _recon_dir(ctx, tmp_path, 1, 2, 3, seed=47)
+    restart 1 carries beta and the best residual (recon.cu:2043-2048); 3D
+    scaled data as above, within 1e-4 of the FP32-storage oracle."""
+    scale, lam = TV_SCALED
+    p, b, q = _recon_dir(ctx, tmp_path, 4, 2, 3, seed=47, scale=scale)
     x = _run_recon(tmp_path, p, b, q, lam, 4, 2, True, restarts=2)
     ref, hists = _oracle_recon(p, b, q, lam, 4, 2, True, restarts=2)
-    e = rel_l2(x, ref)
-    print(f"2 restarts x 2 TV passes vs FP32-storage oracle: {e:.2e}")
+    ref64, _ = _oracle_recon(p, b, q, lam, 4, 2, True, restarts=2, vec=np.complex128)
+    one, _ = _oracle_recon(p, b, q, lam, 4, 2, True, restarts=1)
+    e, spread = rel_l2(x, ref), rel_l2(ref, ref64)
+    print(f"2 restarts x 2 TV passes vs FP32-storage oracle: {e:.2e} (oracle FP32 vs FP64 "
+          f"{spread:.2e}; the second restart moves the image by {rel_l2(ref, one):.2e})")
     assert len(hists) == 4
     assert e < 1e-4
 
