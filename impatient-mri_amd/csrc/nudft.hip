@@ -39,6 +39,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <mutex>
 #include <vector>
 
 namespace imp {
@@ -1197,6 +1198,23 @@ int adjoint_impl(imp_ctx *ctx, Scratch &S, bool allow_stack, int num_k, int num_
     return run_job(ctx, J, S);
 }
 
+// Bytes adjoint_impl carves from a fixed region without the stacked form,
+// with one chunk of partials (more chunks are used when they fit).
+size_t adjoint_fixed_bytes(int num_k, int num_x, int ncoils, bool d3) {
+    auto u = Scratch::unit;
+    size_t b = u(spos_bytes(num_k, d3)) + u((size_t)num_x * sizeof(float4)) +
+               u((size_t)num_k * ncoils * sizeof(float2));
+    if (ncoils >= MFMA_MIN_RHS) {
+        const int tiles = ceil_div(ncoils, MF_COLS);
+        const size_t nb = (size_t)(ceil_div(num_k, MF_K) + MF_BPAD) * tiles * 64;
+        b += u(256) + u(nb * sizeof(BPair)) + u((size_t)tiles * 32 * num_x * sizeof(float));
+    } else {
+        const int G = ncoils <= 1 ? 1 : 2;
+        b += u((size_t)num_k * 2 * G * sizeof(float)) + u((size_t)num_x * sizeof(float2));
+    }
+    return b;
+}
+
 }  // namespace
 
 int nudft_adjoint(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *kx,
@@ -1213,6 +1231,65 @@ int nudft_adjoint(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *k
                         phi_i, d_r, d_i, s_r, s_i, out_r, out_i);
 }
 
+// ---- the allocation-free F^H d ------------------------------------------------
+// Every buffer of imp_nudft_adjoint_noalloc lives in this fixed device array
+// (one copy per GPU, held until the process ends).  Host code never passes
+// `scratch` itself to a kernel: it uses the device address hipGetSymbolAddress
+// returns for the current device.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mx;
+void *g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+// scratch's address on ctx's device, looked up once per device with that
+// device current
+int noalloc_scratch(imp_ctx *ctx, void **out) {
+    std::lock_guard<std::mutex> lk(g_scratch_mx);
+    if (ctx->device < 0 || ctx->device >= NOALLOC_MAX_DEVICES)
+        return fail(IMP_EINVAL, "imp_nudft_adjoint_noalloc: device %d out of range", ctx->device);
+    if (!g_scratch_ptr[ctx->device]) {
+        int cur = -1;
+        IMP_HIP(hipGetDevice(&cur));
+        if (cur != ctx->device) IMP_HIP(hipSetDevice(ctx->device));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != ctx->device) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p)
+            return fail(IMP_EHIP, "imp_nudft_adjoint_noalloc: hipGetSymbolAddress: %s",
+                        hipGetErrorString(e));
+        g_scratch_ptr[ctx->device] = p;
+    }
+    *out = g_scratch_ptr[ctx->device];
+    return IMP_OK;
+}
+}  // namespace
+
+int nudft_adjoint_noalloc(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *kx,
+                          const float *ky, const float *kz, const float *t, const float *x,
+                          const float *y, const float *z, const float *fm,
+                          const float *phi_r, const float *phi_i, const float *d_r,
+                          const float *d_i, const float *s_r, const float *s_i,
+                          float *out_r, float *out_i) {
+    if (!adjoint_args_ok(ctx, num_k, num_x, ncoils, kx, ky, t, x, y, phi_r, phi_i, d_r, d_i,
+                         s_r, s_i, out_r, out_i))
+        return fail(IMP_EINVAL, "imp_nudft_adjoint_noalloc: bad arguments");
+    if (ncoils > 2 * MF_COLS)
+        return fail(IMP_EUNSUPPORTED, "imp_nudft_adjoint_noalloc: %d coils (at most 32)", ncoils);
+    const size_t need = adjoint_fixed_bytes(num_k, num_x, ncoils, kz && z);
+    if (need > NOALLOC_SCRATCH_BYTES)
+        return fail(IMP_EUNSUPPORTED,
+                    "imp_nudft_adjoint_noalloc: %d samples x %d voxels x %d coils need %zu bytes "
+                    "of scratch; the fixed region holds %zu (use imp_nudft_adjoint)",
+                    num_k, num_x, ncoils, need, NOALLOC_SCRATCH_BYTES);
+    void *base = nullptr;
+    IMP_TRY(noalloc_scratch(ctx, &base));
+    Scratch S{ctx, (char *)base, NOALLOC_SCRATCH_BYTES, 0};
+    return adjoint_impl(ctx, S, false, num_k, num_x, ncoils, kx, ky, kz, t, x, y, z, fm, phi_r,
+                        phi_i, d_r, d_i, s_r, s_i, out_r, out_i);
+}
 
 // F^H d of several images (different sizes, trajectories, coil maps) with
 // one persistent MFMA launch over all of them.  Every job needs 3..32 coils
@@ -2278,4 +2355,15 @@ int toeplitz_q(imp_ctx *ctx, int num_k, const float *kx, const float *ky,
 extern "C" int imp_nudft_adjoint_batch(imp_ctx *ctx, int njobs, const imp_nudft_job *jobs,
                                        int chunk_scale) {
     return imp::nudft_adjoint_batch(ctx, njobs, jobs, chunk_scale);
+}
+
+extern "C" int imp_nudft_adjoint_noalloc(imp_ctx *ctx, int num_k, int num_x, int ncoils,
+                                         const float *kx, const float *ky, const float *kz,
+                                         const float *t, const float *x, const float *y,
+                                         const float *z, const float *fm, const float *phi_r,
+                                         const float *phi_i, const float *d_r, const float *d_i,
+                                         const float *s_r, const float *s_i, float *out_r,
+                                         float *out_i) {
+    return imp::nudft_adjoint_noalloc(ctx, num_k, num_x, ncoils, kx, ky, kz, t, x, y, z, fm,
+                                      phi_r, phi_i, d_r, d_i, s_r, s_i, out_r, out_i);
 }

@@ -86,6 +86,12 @@ int nudft_adjoint(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *k
                   const float *phi_r, const float *phi_i, const float *d_r,
                   const float *d_i, const float *s_r, const float *s_i,
                   float *out_r, float *out_i);
+int nudft_adjoint_noalloc(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *kx,
+                          const float *ky, const float *kz, const float *t, const float *x,
+                          const float *y, const float *z, const float *fm,
+                          const float *phi_r, const float *phi_i, const float *d_r,
+                          const float *d_i, const float *s_r, const float *s_i,
+                          float *out_r, float *out_i);
 int nudft_forward(imp_ctx *ctx, int num_k, int num_x, int ncoils, const float *kx,
                   const float *ky, const float *kz, const float *t, const float *x,
                   const float *y, const float *z, const float *fm,

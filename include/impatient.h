@@ -104,6 +104,26 @@ int imp_nudft_adjoint(imp_ctx *ctx, int num_k, int num_x, int ncoils,
                       const float *s_r, const float *s_i,
                       float *out_r, float *out_i);
 
+/* imp_nudft_adjoint without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and result as imp_nudft_adjoint (stacked 3D trajectories are
+ * summed directly, without the separable form).  Every buffer it needs comes
+ * from one fixed 64 MiB device array per GPU, held until the process ends;
+ * a call that needs more returns IMP_EUNSUPPORTED before launching anything
+ * (about 2D 256^2 voxels with 8 coils and 32768 samples fits).  All calls on
+ * one GPU share that array: calls that overlap on one GPU (two streams, two
+ * contexts, two threads) race and must be serialised by the caller. */
+int imp_nudft_adjoint_noalloc(imp_ctx *ctx, int num_k, int num_x, int ncoils,
+                              const float *kx, const float *ky, const float *kz,
+                              const float *t,
+                              const float *x, const float *y, const float *z,
+                              const float *fm,
+                              const float *phi_r, const float *phi_i,
+                              const float *d_r, const float *d_i,
+                              const float *s_r, const float *s_i,
+                              float *out_r, float *out_i);
+
 /* Replaces ftGpu + parallelFtGpu (kernels/ftGpu.cu:198-604,
  * kernels/parImagingGpu.cu:92-117):
  *   d_c(k) = sum_x S_c(x) rho(x) e^{-i(2pi k.x + fm(x) t_k)}
