@@ -17,9 +17,9 @@
 //      u_i = (i - n/2)/n.
 //
 // This build's structure:
-//   - samples are sorted by grid cell (a counting sort made deterministic by
-//     ordering each cell's samples by index), so each grid point gathers
-//     from contiguous sample ranges with no atomics on the grid;
+//   - samples are sorted by grid cell (a stable radix sort, so each cell's
+//     samples stay in index order), so each grid point gathers from
+//     contiguous sample ranges with no atomics on the grid;
 //   - one gather launch grids every coil of a segment at once (each sample's
 //     kernel weight serves all coils);
 //   - the DFT of step 2 runs per axis on lines of length G = R P (P the
@@ -27,6 +27,7 @@
 //     then the R-term combination evaluated only at the n kept outputs, with
 //     step 3 folded in; every pass writes its output with the transformed
 //     axis outermost, so after one pass per axis the layout is the image's.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -72,10 +73,13 @@ struct GridGeom {
     int d3;
 };
 
-// shifted positions s = os (k + n/2) (float, as the reference computes them)
-// and the grid cell ((cy Gx) + cx) Gz + cz of every sample
+// shifted positions s = os (k + n/2) (float, as the reference computes them),
+// and the sort key of every sample: its time bucket b (segment l weighs only
+// buckets l and l + 1; b = floor((t - t0)/tau) + 1 clamped to [0, nseg]) and
+// its grid cell ((cy Gx) + cx) Gz + cz, as b ncells + cell
 __global__ void k_grid_cells(int m, const float *kx, const float *ky, const float *kz,
-                             float kscale, GridGeom g, float4 *pos, int *cell, int *count) {
+                             const float *t, float kscale, GridGeom g, float t0, float tau,
+                             int nseg, float4 *pos, int *key, int *count) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= m) return;
     const float sx = g.os * (kscale * kx[j] + (float)g.n[0] / 2.0f);
@@ -85,9 +89,16 @@ __global__ void k_grid_cells(int m, const float *kx, const float *ky, const floa
     const int cy = min(g.G[1] - 1, max(0, (int)floorf(sy)));
     const int cz = g.d3 ? min(g.G[2] - 1, max(0, (int)floorf(sz))) : 0;
     const int c = ((cy * g.G[0]) + cx) * g.G[2] + cz;
+    int b = 0;
+    if (tau > 0.f) {
+        const double f = floor(((double)t[j] - (double)t0) / (double)tau);
+        b = (int)fmin(fmax(f + 1.0, 0.0), (double)nseg);
+    }
+    const long ncells = (long)g.G[0] * g.G[1] * g.G[2];
+    const int k = (int)(b * ncells + c);
     pos[j] = make_float4(sx, sy, sz, 0.f);
-    cell[j] = c;
-    atomicAdd(&count[c], 1);
+    key[j] = k;
+    atomicAdd(&count[k], 1);
 }
 
 // exclusive scan of n ints in three launches: per-block scans with block
@@ -149,27 +160,66 @@ __global__ void k_scan_add(int *out, int n, const int *block_sum, const int *tot
     if (i == n) out[n] = *total;
 }
 
-// scatter sample j to its cell's range (any order within a cell)
-__global__ void k_cell_scatter(int m, const int *cell, int *cursor, int *perm) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < m) perm[atomicAdd(&cursor[cell[j]], 1)] = j;
+// Stable sort of the samples by cell (the gather then sums each cell's
+// samples in index order, so every run gives the same bits): LSD radix sort
+// of (cell, sample index) pairs, 4-bit digits, one element per thread and
+// RS_BLOCK elements per workgroup.  Per pass:
+//   k_rs_count    digit histogram of each block, stored digit-major
+//                 hist[d * nblocks + block];
+//   (exclusive scan of hist: every (digit, block) gets its output offset;
+//   digits ascending, blocks in order)
+//   k_rs_scatter  each element goes to its (digit, block) offset plus the
+//                 number of earlier elements of its block with that digit
+//                 (wave ballots within a wave, per-wave counts in LDS
+//                 across waves), which keeps equal keys in input order.
+constexpr int RS_BLOCK = 256;
+constexpr int RS_WAVES = RS_BLOCK / 64;
+constexpr int RS_BINS = 16;
+
+__global__ __launch_bounds__(RS_BLOCK) void k_rs_count(const int *__restrict__ key, int m,
+                                                       int shift, int nblocks,
+                                                       int *__restrict__ hist) {
+    __shared__ int cnt[RS_BINS];
+    if (threadIdx.x < RS_BINS) cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int i = blockIdx.x * RS_BLOCK + threadIdx.x;
+    if (i < m) atomicAdd(&cnt[(key[i] >> shift) & (RS_BINS - 1)], 1);
+    __syncthreads();
+    if (threadIdx.x < RS_BINS) hist[threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
 }
 
-// order each cell's samples by sample index (cells hold a handful of
-// samples), so the gather sums in a fixed order on every run
-__global__ void k_cell_sort(int ncells, const int *start, int *perm) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncells) return;
-    const int b = start[c], e = start[c + 1];
-    for (int i = b + 1; i < e; ++i) {
-        const int v = perm[i];
-        int k = i - 1;
-        while (k >= b && perm[k] > v) {
-            perm[k + 1] = perm[k];
-            --k;
-        }
-        perm[k + 1] = v;
+__global__ __launch_bounds__(RS_BLOCK) void k_rs_scatter(const int *__restrict__ key_in,
+                                                         const int *__restrict__ val_in, int m,
+                                                         int shift, int nblocks,
+                                                         const int *__restrict__ offs,
+                                                         int *__restrict__ key_out,
+                                                         int *__restrict__ val_out) {
+    __shared__ int wcnt[RS_WAVES][RS_BINS];
+    const int i = blockIdx.x * RS_BLOCK + threadIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const bool ok = i < m;
+    const int k = ok ? key_in[i] : 0;
+    const int d = ok ? (k >> shift) & (RS_BINS - 1) : RS_BINS;   // RS_BINS: no element
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    int rank = 0;
+#pragma unroll
+    for (int bin = 0; bin < RS_BINS; ++bin) {
+        const unsigned long long mk = __ballot(d == bin);
+        if (d == bin) rank = __popcll(mk & lt);
+        if (lane == 0) wcnt[wave][bin] = __popcll(mk);
     }
+    __syncthreads();
+    if (!ok) return;
+    int before = 0;
+    for (int w = 0; w < wave; ++w) before += wcnt[w][d];
+    const int dst = offs[d * nblocks + blockIdx.x] + before + rank;
+    key_out[dst] = k;
+    val_out[dst] = val_in[i];
+}
+
+__global__ void k_iota(int m, int *v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) v[i] = i;
 }
 
 // segment weights of every sample: a_l(t) is nonzero for at most two
@@ -213,133 +263,258 @@ __global__ void k_grid_data(int m, int ncoils, const int *perm, const float *d_r
                                              : make_float2(0.f, 0.f);
 }
 
-// One thread per grid point: sum over the samples within the kernel window
-// of segment l (weight a_l), for C coils (C = 0: unit data, one plane).
-// Grid layout per coil: [gy][gx][gz], written at grid + c * plane.
-template <int C>
-__global__ __launch_bounds__(GT) void k_grid_gather(GridGeom g, const int *start,
-                                                    const float4 *pos, const float2 *w,
-                                                    const float2 *data, int l, float2 *grid,
-                                                    size_t plane) {
-    constexpr int NC = C > 0 ? C : 1;
-    const long id = (long)blockIdx.x * GT + threadIdx.x;
-    const long total = (long)g.G[0] * g.G[1] * g.G[2];
-    if (id >= total) return;
-    // fastest index along the innermost cell axis (z in 3D, x in 2D)
-    int gx, gy, gz;
+// The window of grid point id ([gy][gx][gz], the innermost cell axis
+// fastest): the cells whose samples can lie within W/2 of it.
+struct Window {
+    int gx, gy, gz, x0, x1, y0, y1, z0, z1;
+};
+
+__device__ __forceinline__ Window window_of(const GridGeom &g, long id) {
+    Window W;
     if (g.d3) {
-        gz = (int)(id % g.G[2]);
-        gx = (int)((id / g.G[2]) % g.G[0]);
-        gy = (int)(id / ((long)g.G[2] * g.G[0]));
+        W.gz = (int)(id % g.G[2]);
+        W.gx = (int)((id / g.G[2]) % g.G[0]);
+        W.gy = (int)(id / ((long)g.G[2] * g.G[0]));
     } else {
-        gz = 0;
-        gx = (int)(id % g.G[0]);
-        gy = (int)(id / g.G[0]);
+        W.gz = 0;
+        W.gx = (int)(id % g.G[0]);
+        W.gy = (int)(id / g.G[0]);
     }
     const int r_lo = (int)ceilf(g.half), r_hi = (int)floorf(g.half);
-    float acc_r[NC], acc_i[NC];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) acc_r[c] = acc_i[c] = 0.f;
-    const int y0 = max(0, gy - r_lo), y1 = min(g.G[1] - 1, gy + r_hi);
-    const int x0 = max(0, gx - r_lo), x1 = min(g.G[0] - 1, gx + r_hi);
-    const int z0 = g.d3 ? max(0, gz - r_lo) : 0, z1 = g.d3 ? min(g.G[2] - 1, gz + r_hi) : 0;
-    // outer cell axes: (y, x) in 3D with z contiguous; (y) in 2D with x contiguous
-    const int o1 = g.d3 ? x0 : 0, o1e = g.d3 ? x1 : 0;
-    for (int cy = y0; cy <= y1; ++cy) {
-        for (int cx = o1; cx <= o1e; ++cx) {
-            int b, e;
-            if (g.d3) {
-                const long row = ((long)cy * g.G[0] + cx) * g.G[2];
-                b = start[row + z0];
-                e = start[row + z1 + 1];
-            } else {
-                const long row = (long)cy * g.G[0];
-                b = start[row + x0];
-                e = start[row + x1 + 1];
-            }
-            for (int ks = b; ks < e; ++ks) {
-                const float4 p = pos[ks];
-                const float2 ww = w[ks];
-                const int l0 = __float_as_int(p.w);
-                const float a = l == l0 ? ww.x : (l == l0 + 1 ? ww.y : 0.f);
-                if (a == 0.f) continue;
-                float kw = kb_weight(p.x, gx, g.inv_os, g.beta);
-                if (kw == 0.f) continue;
-                kw *= kb_weight(p.y, gy, g.inv_os, g.beta);
-                if (g.d3) kw *= kb_weight(p.z, gz, g.inv_os, g.beta);
-                const float s = kw * a;
-                if (C == 0) {
-                    acc_r[0] += s;
+    W.y0 = max(0, W.gy - r_lo);
+    W.y1 = min(g.G[1] - 1, W.gy + r_hi);
+    W.x0 = max(0, W.gx - r_lo);
+    W.x1 = min(g.G[0] - 1, W.gx + r_hi);
+    W.z0 = g.d3 ? max(0, W.gz - r_lo) : 0;
+    W.z1 = g.d3 ? min(g.G[2] - 1, W.gz + r_hi) : 0;
+    return W;
+}
+
+// f(b, e) for each contiguous sample range [b, e) of the window: per time
+// bucket l and l + 1 of segment l, per cell row (y, x) in 3D with z
+// contiguous, per row y in 2D with x contiguous
+template <class F>
+__device__ __forceinline__ void for_ranges(const GridGeom &g, const int *start, int nbuck, int l,
+                                           const Window &W, F f) {
+    const long total = (long)g.G[0] * g.G[1] * g.G[2];
+    const int o1 = g.d3 ? W.x0 : 0, o1e = g.d3 ? W.x1 : 0;
+    const int b1 = min(l + 1, nbuck - 1);
+    for (int bk = l; bk <= b1; ++bk) {
+        const long bbase = (long)bk * total;
+        for (int cy = W.y0; cy <= W.y1; ++cy) {
+            for (int cx = o1; cx <= o1e; ++cx) {
+                if (g.d3) {
+                    const long row = bbase + ((long)cy * g.G[0] + cx) * g.G[2];
+                    f(start[row + W.z0], start[row + W.z1 + 1]);
                 } else {
-#pragma unroll
-                    for (int c = 0; c < NC; ++c) {
-                        const float2 v = data[(size_t)ks * NC + c];
-                        acc_r[c] = fmaf(s, v.x, acc_r[c]);
-                        acc_i[c] = fmaf(s, v.y, acc_i[c]);
-                    }
+                    const long row = bbase + (long)cy * g.G[0];
+                    f(start[row + W.x0], start[row + W.x1 + 1]);
                 }
             }
         }
     }
+}
+
+// samples b + off, b + off + stride, ... < e of segment l into the point's sums
+template <int C>
+__device__ __forceinline__ void accumulate(const GridGeom &g, const Window &W,
+                                           const float4 *__restrict__ pos,
+                                           const float2 *__restrict__ w,
+                                           const float2 *__restrict__ data, int l, int b, int e,
+                                           int off, int stride, float *acc_r, float *acc_i) {
+    constexpr int NC = C > 0 ? C : 1;
+    for (int ks = b + off; ks < e; ks += stride) {
+        const float4 p = pos[ks];
+        const float2 ww = w[ks];
+        const int l0 = __float_as_int(p.w);
+        const float a = l == l0 ? ww.x : (l == l0 + 1 ? ww.y : 0.f);
+        if (a == 0.f) continue;
+        float kw = kb_weight(p.x, W.gx, g.inv_os, g.beta);
+        if (kw == 0.f) continue;
+        kw *= kb_weight(p.y, W.gy, g.inv_os, g.beta);
+        if (g.d3) kw *= kb_weight(p.z, W.gz, g.inv_os, g.beta);
+        const float s = kw * a;
+        if (C == 0) {
+            acc_r[0] += s;
+        } else {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                const float2 v = data[(size_t)ks * NC + c];
+                acc_r[c] = fmaf(s, v.x, acc_r[c]);
+                acc_i[c] = fmaf(s, v.y, acc_i[c]);
+            }
+        }
+    }
+}
+
+// a point whose window holds more samples than this is summed by a whole
+// wave (k_grid_gather_heavy): near the centre of a variable-density
+// trajectory a window holds thousands of samples, and one lane per point
+// would leave its wave waiting on that lane
+constexpr int GATHER_HEAVY = 64;
+
+// One thread per grid point: sum over the samples within the kernel window
+// of segment l (weight a_l), for C coils (C = 0: unit data, one plane).
+// Grid layout per coil: [gy][gx][gz], written at grid + c * plane.  Points
+// with more than GATHER_HEAVY samples are listed in heavy[] (in any order)
+// and left to k_grid_gather_heavy.
+template <int C>
+__global__ __launch_bounds__(GT) void k_grid_gather(GridGeom g, const int *start, int nbuck,
+                                                    const float4 *pos, const float2 *w,
+                                                    const float2 *data, int l, float2 *grid,
+                                                    size_t plane, int *heavy, int *nheavy) {
+    constexpr int NC = C > 0 ? C : 1;
+    const long id = (long)blockIdx.x * GT + threadIdx.x;
+    const long total = (long)g.G[0] * g.G[1] * g.G[2];
+    if (id >= total) return;
+    const Window W = window_of(g, id);
+    int count = 0;
+    for_ranges(g, start, nbuck, l, W, [&](int b, int e) { count += e - b; });
+    if (count > GATHER_HEAVY) {
+        heavy[atomicAdd(nheavy, 1)] = (int)id;
+        return;
+    }
+    float acc_r[NC], acc_i[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) acc_r[c] = acc_i[c] = 0.f;
+    for_ranges(g, start, nbuck, l, W, [&](int b, int e) {
+        accumulate<C>(g, W, pos, w, data, l, b, e, 0, 1, acc_r, acc_i);
+    });
 #pragma unroll
     for (int c = 0; c < NC; ++c) grid[c * plane + id] = make_float2(acc_r[c], acc_i[c]);
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// The listed points, one wave each: lane k sums the window's samples
+// k, k + 64, ... of every range, then a fixed butterfly adds the lanes, so
+// the result does not depend on which wave takes the point.  Waves stride
+// over the list; every wave reads the same count and ends.
+template <int C>
+__global__ __launch_bounds__(GT) void k_grid_gather_heavy(GridGeom g, const int *start, int nbuck,
+                                                          const float4 *pos, const float2 *w,
+                                                          const float2 *data, int l,
+                                                          float2 *grid, size_t plane,
+                                                          const int *heavy,
+                                                          const int *nheavy) {
+    constexpr int NC = C > 0 ? C : 1;
+    const int lane = threadIdx.x & 63;
+    const int wave = (int)((blockIdx.x * GT + threadIdx.x) >> 6);
+    const int nwaves = (int)(gridDim.x * (GT / 64));
+    const int n = *nheavy;
+    for (int it = wave; it < n; it += nwaves) {
+        const long id = heavy[it];
+        const Window W = window_of(g, id);
+        float acc_r[NC], acc_i[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) acc_r[c] = acc_i[c] = 0.f;
+        for_ranges(g, start, nbuck, l, W, [&](int b, int e) {
+            accumulate<C>(g, W, pos, w, data, l, b, e, lane, 64, acc_r, acc_i);
+        });
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            acc_r[c] = wave_sum(acc_r[c]);
+            acc_i[c] = wave_sum(acc_i[c]);
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) grid[c * plane + id] = make_float2(acc_r[c], acc_i[c]);
+        }
+    }
 }
 
 // Cropped, shifted inverse DFT along the contiguous axis of every line:
 //   out[b][i][a] = dp[i] sum_{g<G} in[b][a][g] e^{+i 2pi g (i - n/2) / G}
 // (dp carries (-1)^(i - n/2) of the shift and 1 / (os C(u_i))), for lines
 // a < A of batch item b.  G = R P: R radix-2 FFTs of length P in LDS over
-// the subsequences g = R q + j, then the R-term combination at each kept i.
-// tw[k] = e^{+i 2pi k / G}.  LPW lines per workgroup.
+// the subsequences g = R q + j, then the R-term combination at each kept i,
+//   out(nn) = sum_j Y_j[nn mod P] e^{+i 2pi j nn / G},  nn = i - n/2,
+// with the twiddle index j nn mod G stepped by additions.  tw[k] =
+// e^{+i 2pi k / G}.  2^logLPW lines per workgroup; every index is 32-bit
+// and every division by a power of two is a shift (q = g / R by a float
+// reciprocal and one correction step).
 __global__ __launch_bounds__(GT) void k_line_dft(const float2 *__restrict__ in,
                                                  float2 *__restrict__ out, int G, int R, int P,
-                                                 int logP, int n, long A, long nlines, int LPW,
-                                                 const float2 *__restrict__ tw,
+                                                 int logP, int n, int A, int nlines, int logLPW,
+                                                 float rcpR, const float2 *__restrict__ tw,
                                                  const float *__restrict__ dp) {
     extern __shared__ float2 lds[];
-    const long line0 = (long)blockIdx.x * LPW;
-    const int nl = (int)min((long)LPW, nlines - line0);
-    // load with bit reversal inside each length-P subsequence
-    for (int e = threadIdx.x; e < nl * G; e += GT) {
-        const int li = e / G, g = e % G;
-        const int j = g % R, q = g / R;
-        const int qr = logP ? (int)(__brev((unsigned)q) >> (32 - logP)) : 0;
-        lds[li * G + j * P + qr] = in[(line0 + li) * (long)G + g];
+    const int LPW = 1 << logLPW;
+    const int line0 = blockIdx.x * LPW;
+    const int nl = min(LPW, nlines - line0);
+    // load: each line in natural order (coalesced), scattered to the
+    // bit-reversed position of its subsequence
+    for (int li = 0; li < nl; ++li) {
+        const float2 *__restrict__ src = in + (size_t)(line0 + li) * G;
+        float2 *dst = lds + li * G;
+        for (int g = threadIdx.x; g < G; g += GT) {
+            int q = (int)((float)g * rcpR);
+            int j = g - q * R;
+            if (j < 0) {
+                --q;
+                j += R;
+            } else if (j >= R) {
+                ++q;
+                j -= R;
+            }
+            const int qr = logP ? (int)(__brev((unsigned)q) >> (32 - logP)) : 0;
+            dst[j * P + qr] = src[g];
+        }
     }
     __syncthreads();
-    // radix-2 stages: butterfly span h = 1, 2, ..., P/2
-    for (int h = 1; h < P; h <<= 1) {
-        const int nb = nl * R * (P / 2);
-        for (int bi = threadIdx.x; bi < nb; bi += GT) {
-            const int k = bi % h;
-            const int grp = (bi / h) % (P / (2 * h));
-            const int sub = bi / (P / 2);       // (line, j)
-            const int base = sub * P + grp * 2 * h + k;
-            const float2 wv = tw[(long)k * (G / (2 * h))];
-            const float2 u = lds[base], v = lds[base + h];
-            const float2 tv = make_float2(v.x * wv.x - v.y * wv.y, v.x * wv.y + v.y * wv.x);
-            lds[base] = make_float2(u.x + tv.x, u.y + tv.y);
-            lds[base + h] = make_float2(u.x - tv.x, u.y - tv.y);
+    // radix-2 stages: butterfly span h = 2^lh = 1, 2, ..., P/2
+    if (logP > 0) {
+        const int nb = (nl * R) << (logP - 1);
+        for (int lh = 0; lh < logP; ++lh) {
+            const int h = 1 << lh;
+            const int tstride = G >> (lh + 1);                 // G / (2h), exact
+            const int gmask = (P >> (lh + 1)) - 1;             // P / (2h) - 1
+            for (int bi = threadIdx.x; bi < nb; bi += GT) {
+                const int k = bi & (h - 1);
+                const int grp = (bi >> lh) & gmask;
+                const int sub = bi >> (logP - 1);              // (line, j)
+                const int base = sub * P + grp * 2 * h + k;
+                const float2 wv = tw[k * tstride];
+                const float2 u = lds[base], v = lds[base + h];
+                const float2 tv = make_float2(v.x * wv.x - v.y * wv.y, v.x * wv.y + v.y * wv.x);
+                lds[base] = make_float2(u.x + tv.x, u.y + tv.y);
+                lds[base + h] = make_float2(u.x - tv.x, u.y - tv.y);
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
-    // combination at the kept outputs, written with the axis outermost
-    for (int e = threadIdx.x; e < nl * n; e += GT) {
-        const int li = e % nl, i = e / nl;
+    // combination at the kept outputs, written with the axis outermost:
+    // consecutive threads take consecutive lines of one output index
+    const int b0 = line0 / A, a0 = line0 - b0 * A;
+    for (int e = threadIdx.x; e < (n << logLPW); e += GT) {
+        const int li = e & (LPW - 1), i = e >> logLPW;
+        if (li >= nl) continue;
         const int nn = i - n / 2;
-        const int m = ((nn % P) + P) % P;
+        const int m = nn & (P - 1);                            // nn mod P (P a power of two)
+        int step = nn % G;
+        if (step < 0) step += G;
+        const float2 *y = lds + li * G + m;
         float sr = 0.f, si = 0.f;
+        int kk = 0;
         for (int j = 0; j < R; ++j) {
-            const float2 y = lds[li * G + j * P + m];
-            const long kk = (((long)j * nn) % G + G) % G;
+            const float2 yv = y[j * P];
             const float2 wv = tw[kk];
-            sr += y.x * wv.x - y.y * wv.y;
-            si += y.x * wv.y + y.y * wv.x;
+            sr += yv.x * wv.x - yv.y * wv.y;
+            si += yv.x * wv.y + yv.y * wv.x;
+            kk += step;
+            if (kk >= G) kk -= G;
         }
-        const long line = line0 + li;
-        const long b = line / A, a = line % A;
+        int a = a0 + li, b = b0;
+        while (a >= A) {
+            a -= A;
+            ++b;
+        }
         const float f = dp[i];
-        out[(b * n + i) * A + a] = make_float2(sr * f, si * f);
+        out[((size_t)b * n + i) * A + a] = make_float2(sr * f, si * f);
     }
 }
 
@@ -463,17 +638,38 @@ struct Carve {
 // Sorted samples of one trajectory on one grid: cell ranges, positions and
 // segment weights.
 struct Binned {
-    int *start;       // [ncells + 1]
+    int nbuck;        // time buckets (segments + 1)
+    int *start;       // [nbuck * ncells + 1]
     int *perm;        // [m] sample index of sorted position
     float4 *pos;      // [m] sorted
     float2 *w;        // [m] sorted segment weights
 };
 
-size_t binned_bytes(int m, long ncells) {
+// the radix sort's histogram of m elements, and the block sums of an
+// exclusive scan of n ints
+long rs_blocks(int m) { return (m + RS_BLOCK - 1) / RS_BLOCK; }
+long scan_blocks(long n) { return (n + GT * 4 - 1) / (GT * 4); }
+
+// keys: time bucket x grid cell; nseg + 1 buckets
+long key_range(long ncells, int nseg) { return ncells * (nseg + 1); }
+
+size_t binned_bytes(int m, long ncells, int nseg) {
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    const long nb = (ncells + GT * 4 - 1) / (GT * 4);
-    return al((ncells + 1) * 4) * 2 + al(nb * 4) + al(16) + al((size_t)m * 4) * 2 +
-           al((size_t)m * 16) * 2 + al((size_t)m * 8);
+    const long nk = key_range(ncells, nseg);
+    const long nh = (long)RS_BINS * rs_blocks(m);
+    const long nb = std::max(scan_blocks(nk), scan_blocks(nh));
+    return al((nk + 1) * 4) * 2 + al(nb * 4) + al(16) + al((size_t)m * 4) * 4 +
+           al((size_t)(nh + 1) * 4) * 2 + al((size_t)m * 16) * 2 + al((size_t)m * 8);
+}
+
+// exclusive scan of n ints in three launches (out has n + 1 entries: the
+// total last); bsum holds scan_blocks(n) ints
+void scan_ints(hipStream_t st, const int *in, int *out, long n, int *bsum, int *total) {
+    const long nb = scan_blocks(n);
+    hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(GT), 0, st, in, (int)n, out, bsum);
+    hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(GT), 0, st, bsum, (int)nb, total);
+    hipLaunchKernelGGL(k_scan_add, dim3(ceil_div(n + 1, 256)), dim3(256), 0, st, out, (int)n, bsum,
+                       total);
 }
 
 int bin_samples(imp_ctx *ctx, Carve &cv, const GridGeom &g, int m, const float *kx,
@@ -481,32 +677,42 @@ int bin_samples(imp_ctx *ctx, Carve &cv, const GridGeom &g, int m, const float *
                 float tau, float t0, Binned &B) {
     const hipStream_t st = ctx->stream;
     const long ncells = (long)g.G[0] * g.G[1] * g.G[2];
-    const long nb = (ncells + GT * 4 - 1) / (GT * 4);
-    int *count = cv.take<int>(ncells + 1);
-    B.start = cv.take<int>(ncells + 1);
+    const long nk = key_range(ncells, nseg);
+    if (nk >= (1L << 30))
+        return fail(IMP_EUNSUPPORTED, "gridding: %ld time buckets x grid cells (at most 2^30)", nk);
+    const long nrb = rs_blocks(m), nh = (long)RS_BINS * nrb;
+    const long nb = std::max(scan_blocks(nk), scan_blocks(nh));
+    int *count = cv.take<int>(nk + 1);
+    B.start = cv.take<int>(nk + 1);
     int *bsum = cv.take<int>(nb);
     int *total = cv.take<int>(4);
-    int *cell = cv.take<int>(m);
-    B.perm = cv.take<int>(m);
+    int *key[2] = {cv.take<int>(m), cv.take<int>(m)};
+    int *val[2] = {cv.take<int>(m), cv.take<int>(m)};
+    int *hist = cv.take<int>(nh + 1);
+    int *offs = cv.take<int>(nh + 1);
     float4 *pos_raw = cv.take<float4>(m);
     B.pos = cv.take<float4>(m);
     B.w = cv.take<float2>(m);
-    hipLaunchKernelGGL(k_zero_i, dim3(ceil_div(ncells + 1, 256)), dim3(256), 0, st, count,
-                       ncells + 1);
-    hipLaunchKernelGGL(k_grid_cells, dim3(ceil_div(m, 256)), dim3(256), 0, st, m, kx, ky, kz,
-                       kscale, g, pos_raw, cell, count);
-    hipLaunchKernelGGL(k_scan_blocks, dim3((unsigned)nb), dim3(GT), 0, st, count, (int)ncells,
-                       B.start, bsum);
-    hipLaunchKernelGGL(k_scan_totals, dim3(1), dim3(GT), 0, st, bsum, (int)nb, total);
-    hipLaunchKernelGGL(k_scan_add, dim3(ceil_div(ncells + 1, 256)), dim3(256), 0, st, B.start,
-                       (int)ncells, bsum, total);
+    hipLaunchKernelGGL(k_zero_i, dim3(ceil_div(nk + 1, 256)), dim3(256), 0, st, count, nk + 1);
+    hipLaunchKernelGGL(k_grid_cells, dim3(ceil_div(m, 256)), dim3(256), 0, st, m, kx, ky, kz, t,
+                       kscale, g, t0, tau, nseg, pos_raw, key[0], count);
+    scan_ints(st, count, B.start, nk, bsum, total);
+    hipLaunchKernelGGL(k_iota, dim3(ceil_div(m, 256)), dim3(256), 0, st, m, val[0]);
     IMP_CHECK_LAUNCH();
-    // cursor = copy of start (reuses count)
-    IMP_HIP(hipMemcpyAsync(count, B.start, ncells * sizeof(int), hipMemcpyDeviceToDevice, st));
-    hipLaunchKernelGGL(k_cell_scatter, dim3(ceil_div(m, 256)), dim3(256), 0, st, m, cell, count,
-                       B.perm);
-    hipLaunchKernelGGL(k_cell_sort, dim3(ceil_div(ncells, 256)), dim3(256), 0, st, (int)ncells,
-                       B.start, B.perm);
+    // keys are < nk: sort on the bits that can be set
+    B.nbuck = nseg + 1;
+    int bits = 1;
+    while (bits < 31 && (1L << bits) < nk) ++bits;
+    int cur = 0;
+    for (int shift = 0; shift < bits; shift += 4, cur ^= 1) {
+        hipLaunchKernelGGL(k_rs_count, dim3((unsigned)nrb), dim3(RS_BLOCK), 0, st, key[cur], m,
+                           shift, (int)nrb, hist);
+        scan_ints(st, hist, offs, nh, bsum, total);
+        hipLaunchKernelGGL(k_rs_scatter, dim3((unsigned)nrb), dim3(RS_BLOCK), 0, st, key[cur],
+                           val[cur], m, shift, (int)nrb, offs, key[cur ^ 1], val[cur ^ 1]);
+        IMP_CHECK_LAUNCH();
+    }
+    B.perm = val[cur];
     hipLaunchKernelGGL(k_grid_samples, dim3(ceil_div(m, 256)), dim3(256), 0, st, m, B.perm,
                        pos_raw, t, nseg, tau, t0, B.pos, B.w);
     IMP_CHECK_LAUNCH();
@@ -515,7 +721,7 @@ int bin_samples(imp_ctx *ctx, Carve &cv, const GridGeom &g, int m, const float *
 
 // line-transform schedule of one axis
 struct AxisPlan {
-    int G, R, P, logP, n, LPW;
+    int G, R, P, logP, n, logLPW;
     float2 *tw;
     float *dp;
 };
@@ -535,7 +741,9 @@ int plan_axis(imp_ctx *ctx, Carve &cv, const GridGeom &g, int a, AxisPlan &ap) {
     ap.P = P;
     ap.logP = logP;
     ap.R = ap.G / P;
-    ap.LPW = std::max(1, std::min(16, 2048 / ap.G));
+    // lines per workgroup: a power of two, at most 16 and 64 KB of LDS
+    ap.logLPW = 0;
+    while (ap.logLPW < 4 && (2L << ap.logLPW) * ap.G * (long)sizeof(float2) <= 65536) ++ap.logLPW;
     ap.tw = cv.take<float2>(ap.G);
     ap.dp = cv.take<float>(ap.n);
     hipLaunchKernelGGL(k_twiddle_g, dim3(ceil_div(ap.G, 256)), dim3(256), 0, ctx->stream, ap.G,
@@ -548,10 +756,14 @@ int plan_axis(imp_ctx *ctx, Carve &cv, const GridGeom &g, int a, AxisPlan &ap) {
 
 int run_axis(imp_ctx *ctx, const AxisPlan &ap, const float2 *in, float2 *out, long nlines,
              long A) {
-    const unsigned wgs = (unsigned)((nlines + ap.LPW - 1) / ap.LPW);
-    const size_t lds = (size_t)ap.LPW * ap.G * sizeof(float2);
+    if (nlines > INT32_MAX / 2 || A > INT32_MAX / 2 || (long)ap.n * A > INT32_MAX)
+        return fail(IMP_EUNSUPPORTED, "gridding: %ld lines of %ld too many", nlines, A);
+    const int lpw = 1 << ap.logLPW;
+    const unsigned wgs = (unsigned)((nlines + lpw - 1) / lpw);
+    const size_t lds = (size_t)lpw * ap.G * sizeof(float2);
     hipLaunchKernelGGL(k_line_dft, dim3(wgs), dim3(GT), lds, ctx->stream, in, out, ap.G, ap.R,
-                       ap.P, ap.logP, ap.n, A, nlines, ap.LPW, ap.tw, ap.dp);
+                       ap.P, ap.logP, ap.n, (int)A, (int)nlines, ap.logLPW,
+                       1.0f / (float)ap.R, ap.tw, ap.dp);
     IMP_CHECK_LAUNCH();
     return IMP_OK;
 }
@@ -598,12 +810,19 @@ void host_t_ends(imp_ctx *ctx, const float *t, int m, float *tt, int *st) {
         *st = fail(IMP_EHIP, "gridding: %s", hipGetErrorString(hipGetLastError()));
 }
 
+// gather of segment l: light points one per lane, then the listed heavy
+// points one per wave (heavy holds ncells ints, nheavy one)
 template <int C>
 void launch_gather(imp_ctx *ctx, const GridGeom &g, const Binned &B, const float2 *data, int l,
-                   float2 *grid, size_t plane) {
+                   float2 *grid, size_t plane, int *heavy, int *nheavy) {
     const long total = (long)g.G[0] * g.G[1] * g.G[2];
+    hipLaunchKernelGGL(k_zero_i, dim3(1), dim3(256), 0, ctx->stream, nheavy, 1L);
     hipLaunchKernelGGL(k_grid_gather<C>, dim3((unsigned)ceil_div(total, GT)), dim3(GT), 0,
-                       ctx->stream, g, B.start, B.pos, B.w, data, l, grid, plane);
+                       ctx->stream, g, B.start, B.nbuck, B.pos, B.w, data, l, grid, plane, heavy,
+                       nheavy);
+    hipLaunchKernelGGL(k_grid_gather_heavy<C>, dim3(4 * (unsigned)ctx->num_cus), dim3(GT), 0,
+                       ctx->stream, g, B.start, B.nbuck, B.pos, B.w, data, l, grid, plane, heavy,
+                       nheavy);
 }
 
 template <int C>
@@ -633,7 +852,8 @@ int gridding_q(imp_ctx *ctx, int num_k, const float *kx, const float *ky, const 
     const long ncells = (long)g.G[0] * g.G[1] * g.G[2];
     const long nq = (long)n[0] * n[1] * n[2];
     const size_t gbytes = (size_t)nseg * ncells * sizeof(float2);
-    size_t bytes = binned_bytes(num_k, ncells) + 2 * (((gbytes + 255) & ~(size_t)255)) + 4096 +
+    size_t bytes = binned_bytes(num_k, ncells, nseg) + 2 * (((gbytes + 255) & ~(size_t)255)) + 4096 +
+                   (((size_t)ncells * 4 + 255) & ~(size_t)255) + 256 +
                    axis_bytes(g.G[0], n[0]) + axis_bytes(g.G[1], n[1]) + axis_bytes(g.G[2], n[2]);
     void *ws = nullptr;
     IMP_TRY(workspace(ctx, WS_GRID, bytes, &ws));
@@ -643,11 +863,13 @@ int gridding_q(imp_ctx *ctx, int num_k, const float *kx, const float *ky, const 
     IMP_TRY(bin_samples(ctx, cv, g, num_k, kx, ky, kz, 2.0f, t, nseg, tau, tt[0], B));
     float2 *bufA = cv.take<float2>((size_t)nseg * ncells);
     float2 *bufB = cv.take<float2>((size_t)nseg * ncells);
+    int *heavy = cv.take<int>(ncells);
+    int *nheavy = cv.take<int>(1);
     AxisPlan ax[3];
     for (int a = 0; a < (g.d3 ? 3 : 2); ++a) IMP_TRY(plan_axis(ctx, cv, g, a, ax[a]));
     const int rec = prof_begin(ctx, "k_grid_gather");
     for (int l = 0; l < nseg; ++l)
-        launch_gather<0>(ctx, g, B, nullptr, l, bufA + (size_t)l * ncells, 0);
+        launch_gather<0>(ctx, g, B, nullptr, l, bufA + (size_t)l * ncells, 0, heavy, nheavy);
     prof_end(ctx, rec);
     IMP_CHECK_LAUNCH();
     float2 *res = nullptr;
@@ -684,7 +906,8 @@ int gridding_fhd(imp_ctx *ctx, int num_k, int ncoils, const float *kx, const flo
                  : ncoils <= 16 ? 16 : 32;
     const int groups = ceil_div(ncoils, CP);
     const size_t gbytes = (size_t)CP * ncells * sizeof(float2);
-    size_t bytes = binned_bytes(num_k, ncells) + 2 * ((gbytes + 255) & ~(size_t)255) +
+    size_t bytes = binned_bytes(num_k, ncells, nseg) + 2 * ((gbytes + 255) & ~(size_t)255) +
+                   (((size_t)ncells * 4 + 255) & ~(size_t)255) + 256 +
                    (((size_t)num_k * CP * 8 + 255) & ~(size_t)255) + nv * 16 + 8192 +
                    axis_bytes(g.G[0], n[0]) + axis_bytes(g.G[1], n[1]) + axis_bytes(g.G[2], n[2]);
     void *ws = nullptr;
@@ -694,6 +917,8 @@ int gridding_fhd(imp_ctx *ctx, int num_k, int ncoils, const float *kx, const flo
     IMP_TRY(bin_samples(ctx, cv, g, num_k, kx, ky, kz, 1.0f, t, nseg, tau, tt[0], B));
     float2 *bufA = cv.take<float2>((size_t)CP * ncells);
     float2 *bufB = cv.take<float2>((size_t)CP * ncells);
+    int *heavy = cv.take<int>(ncells);
+    int *nheavy = cv.take<int>(1);
     float2 *data = cv.take<float2>((size_t)num_k * CP);
     double2 *acc = cv.take<double2>(nv);
     AxisPlan ax[3];
@@ -714,12 +939,12 @@ int gridding_fhd(imp_ctx *ctx, int num_k, int ncoils, const float *kx, const flo
         for (int l = 0; l < nseg; ++l) {
             const int rec = prof_begin(ctx, "k_grid_gather");
             switch (CP) {
-                case 1: launch_gather<1>(ctx, g, B, data, l, bufA, ncells); break;
-                case 2: launch_gather<2>(ctx, g, B, data, l, bufA, ncells); break;
-                case 4: launch_gather<4>(ctx, g, B, data, l, bufA, ncells); break;
-                case 8: launch_gather<8>(ctx, g, B, data, l, bufA, ncells); break;
-                case 16: launch_gather<16>(ctx, g, B, data, l, bufA, ncells); break;
-                default: launch_gather<32>(ctx, g, B, data, l, bufA, ncells); break;
+                case 1: launch_gather<1>(ctx, g, B, data, l, bufA, ncells, heavy, nheavy); break;
+                case 2: launch_gather<2>(ctx, g, B, data, l, bufA, ncells, heavy, nheavy); break;
+                case 4: launch_gather<4>(ctx, g, B, data, l, bufA, ncells, heavy, nheavy); break;
+                case 8: launch_gather<8>(ctx, g, B, data, l, bufA, ncells, heavy, nheavy); break;
+                case 16: launch_gather<16>(ctx, g, B, data, l, bufA, ncells, heavy, nheavy); break;
+                default: launch_gather<32>(ctx, g, B, data, l, bufA, ncells, heavy, nheavy); break;
             }
             prof_end(ctx, rec);
             IMP_CHECK_LAUNCH();
