@@ -46,7 +46,7 @@ MFMA_FLOPS_PER_PAIR = 24.0
 # rocprofv3 --pmc FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0), the factors a
 # 256 MiB device copy calibrates (profiles/traffic_512x512x1_32coil_pmc.md);
 # re-measure when the kernel changes
-NUDFT_TRAFFIC_BYTES = 2.0 * 3436252.2 * 1024 + 1.0 * 242554.8 * 1024
+NUDFT_TRAFFIC_BYTES = 2.0 * 3422184.6 * 1024 + 1.0 * 972259.7 * 1024
 NUDFT_TRAFFIC_SOURCE = ("constant from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of "
                         "bench.py --pmc-mode (profiles/traffic_512x512x1_32coil_pmc.md), "
                         "not measured in this run")
@@ -132,6 +132,45 @@ def host_cpu():
     except AttributeError:
         avail = os.cpu_count() or 1
     return {"cpu_model": model, "nproc": avail}
+
+
+def cpu_vector_ops(prob, fd_gpu_ms):
+    """The CG's vector stages at the image size, on the host (the C
+    restatements of DHWD2dCpu, dotProductCpu and addCpu, one thread as in the
+    reference) beside the GPU's kernels from the bench's own kernel timer:
+    seconds per call and algorithmic GB/s (stencil: p read, s written; dot:
+    two vectors read; axpy: x read, y read and written; complex64 = 8 B)."""
+    from oracle import oracle as _orc  # noqa: F401  (oracle package: checker/baseline only)
+    so = REPO / "oracle" / "liboracle_cpu.so"
+    if not so.exists():
+        return None
+    lib = C.CDLL(str(so))
+    lib.dot_re_cpu.restype = C.c_float
+    n = prob.nvox
+    rng = np.random.default_rng(3)
+    a = [np.ascontiguousarray(rng.standard_normal(n), np.float32) for _ in range(4)]
+    o = [np.zeros(n, np.float32) for _ in range(2)]
+    P = lambda v: v.ctypes.data_as(C.POINTER(C.c_float))
+    calls = {"fd_reg": lambda: lib.fd_reg_cpu(P(o[0]), P(o[1]), P(a[0]), P(a[1]), prob.Nx,
+                                              prob.Ny, C.c_float(1e-3)),
+             "dot": lambda: lib.dot_re_cpu(P(a[0]), P(a[1]), P(a[2]), P(a[3]), n),
+             "axpy": lambda: lib.axpy_cpu(P(o[0]), P(o[1]), C.c_float(0.5), P(a[0]), P(a[1]), n)}
+    nbytes = {"fd_reg": 16 * n, "dot": 16 * n, "axpy": 24 * n}
+    out = {}
+    for name, fn in calls.items():
+        fn()
+        reps = 20
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        dt = (time.perf_counter() - t0) / reps
+        out[name] = {"cpu_s": round(dt, 6), "cpu_gbs": round(nbytes[name] / dt / 1e9, 2)}
+    if fd_gpu_ms:
+        out["fd_reg"]["gpu_kernel"] = "k_fd_reg"
+        out["fd_reg"]["gpu_s"] = round(fd_gpu_ms * 1e-3, 7)
+    out["note"] = "one image of the 512^2 workload per call"
+    out["threads"] = 1
+    return out
 
 
 def cpu_bruteforce_cfg0(ctx):
@@ -278,6 +317,7 @@ def batch_leg(ctx, log):
         "first_pass_s": round(first["total_s"], 4), "total_s": round(res["total_s"], 4),
         "slices_per_s": round(len(recs) / res["total_s"], 2),
         "fhd_batch_s": round(res["fhd_s"], 4), "q_s": round(res["q_s"], 4),
+        "solve_s": round(res["solve_s"], 4), "streams": res["streams"],
         "converged": int(sum(r["converged"] for r in recs)),
         "max_true_rel_residual": max(r["true_rel_residual"] for r in recs),
         "model": MODEL.name if params is not None else None,
@@ -564,6 +604,8 @@ def main():
                             "algorithmic_bytes": nbytes,
                             "gbs": round(nbytes / (avg * 1e-3) / 1e9, 1) if avg > 0 else None}
             log(f"{name}: {n} launches, {avg:.4f} ms avg, {passes[name]['gbs']} GB/s")
+        fd_tot, fd_n = kernel_ms("k_fd_reg")
+        fd_gpu_ms = fd_tot / fd_n if fd_n else None
         call("imp_profile_enable", ctx.h, C.c_int(0))
         if args.pmc_mode:
             # calibration kernel for the byte counters: a 256 MiB device copy
@@ -637,6 +679,9 @@ def main():
             bf = cpu_bruteforce_cfg0(ctx)
             if bf is not None:
                 result["cpu_baseline"]["bruteforce_small_case"] = bf
+            vo = cpu_vector_ops(prob, fd_gpu_ms if not args.no_recon else None)
+            if vo is not None:
+                result["cpu_baseline"]["vector_ops_512x512"] = vo
                 log(f"brute force, small case: CPU {bf['cpu_s']:.3f} s, GPU {bf['gpu_s']:.4f} s")
     print(json.dumps(result), flush=True)
     ctx.close()

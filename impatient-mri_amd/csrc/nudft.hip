@@ -64,6 +64,12 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 // doubles per step of MF_K stream positions: x, y, w (+ z in 3D)
 __host__ __device__ constexpr int pos_stride(bool d3) { return (d3 ? 4 : 3) * MF_K; }
 
+// Stream positions are read through the constant address space: they are
+// never written while a kernel runs, and the compiler then loads each
+// step's (wave-uniform) positions into scalar registers even where the
+// pointer comes from a job table instead of a restrict-qualified argument.
+typedef const __attribute__((address_space(4))) double *cdouble_ptr;
+
 struct alignas(16) BPair {
     half8 hi;
     half8 lo;
@@ -248,7 +254,7 @@ __device__ __forceinline__ void mfma_body(const float4 *__restrict__ lanes, int 
     constexpr int NPOS = pos_stride(D3);
     double pos[NPOS];
     {
-        const double *B = spos + (size_t)min(s0, max(n_steps - 1, 0)) * NPOS;
+        const cdouble_ptr B = (cdouble_ptr)spos + (size_t)min(s0, max(n_steps - 1, 0)) * NPOS;
 #pragma unroll
         for (int j = 0; j < NPOS; ++j) pos[j] = B[j];
     }
@@ -271,7 +277,7 @@ __device__ __forceinline__ void mfma_body(const float4 *__restrict__ lanes, int 
             sv[j] = __builtin_amdgcn_sinf(r);
         }
         {
-            const double *B = spos + (size_t)min(next, s1 - 1) * NPOS;
+            const cdouble_ptr B = (cdouble_ptr)spos + (size_t)min(next, s1 - 1) * NPOS;
 #pragma unroll
             for (int j = 0; j < NPOS; ++j) pos[j] = B[j];
         }
@@ -412,6 +418,14 @@ void k_nudft_mfma(const float4 *__restrict__ lanes, int n_lane, const double *__
                                one, blockIdx.x, blockIdx.y, lds);
 }
 
+// A wave-uniform pointer moved into scalar registers.
+__device__ __forceinline__ const void *uniform_ptr(const void *p) {
+    const unsigned long long v = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return (const void *)(((unsigned long long)hi << 32) | lo);
+}
+
 // One job of a batched launch (images of different sizes in one grid).
 struct MfmaJob {
     const float4 *lanes;
@@ -433,11 +447,21 @@ void k_nudft_mfma_batch(const MfmaJob *__restrict__ jobs, int njobs, float one) 
     const int g = blockIdx.x;
     int j = 0;
     while (j + 1 < njobs && jobs[j + 1].wg_begin <= g) ++j;
-    const MfmaJob J = jobs[j];
-    const int rem = g - J.wg_begin;
-    mfma_body<TILES, D3, false>(J.lanes, J.n_lane, J.spos, J.bpack, J.n_steps,
-                                J.steps_per_chunk, J.partial, one, rem % J.wgs_x, rem / J.wgs_x,
-                                lds);
+    // the job's fields are the same for every lane: held in scalar
+    // registers (loaded per lane, they would occupy vector registers and the
+    // step loop would spill)
+    const MfmaJob &Jm = jobs[j];
+    const float4 *lanes = (const float4 *)uniform_ptr(Jm.lanes);
+    const double *spos = (const double *)uniform_ptr(Jm.spos);
+    const BPair *bpack = (const BPair *)uniform_ptr(Jm.bpack);
+    float *partial = (float *)uniform_ptr(Jm.partial);
+    const int n_lane = __builtin_amdgcn_readfirstlane(Jm.n_lane);
+    const int n_steps = __builtin_amdgcn_readfirstlane(Jm.n_steps);
+    const int spc = __builtin_amdgcn_readfirstlane(Jm.steps_per_chunk);
+    const int wgs_x = __builtin_amdgcn_readfirstlane(Jm.wgs_x);
+    const int rem = g - __builtin_amdgcn_readfirstlane(Jm.wg_begin);
+    mfma_body<TILES, D3, true>(lanes, n_lane, spos, bpack, n_steps, spc, partial, one,
+                               rem % wgs_x, rem / wgs_x, lds);
 }
 
 // ---- packing ---------------------------------------------------------------

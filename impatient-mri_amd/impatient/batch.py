@@ -10,7 +10,8 @@ each, spiral trajectories scaled per matrix size, field maps within
   3. per slice: the learned warm start (network of the size class, trained
      weights when a model is given) with the zero-start fallback, then
      circulant-preconditioned CG to the tolerance with one HIP graph per
-     iteration (IMP_CG_GRAPH).
+     iteration (IMP_CG_GRAPH); the slices run on several contexts (streams)
+     from as many host threads, each keeping one plan per size class.
 
 The schedule knobs (the NUDFT batch's workgroups per CU, and the pass-E
 coils per workgroup of each size class) are picked by the Autotuner: it
@@ -108,7 +109,7 @@ def _plan(ctx, p, qr, qi):
 
 
 def reconstruct(ctx: Context, slices, tol=1e-5, max_iter=500, model=None, tuner=None, log=print,
-                keep_images=False):
+                keep_images=False, streams=3):
     """Batched reconstruction; returns per-slice records and totals
     (keep_images: each record also holds its F^H d "b" and image "x" as
     host arrays, copied after the slice's timing ends)."""
@@ -135,56 +136,105 @@ def reconstruct(ctx: Context, slices, tol=1e-5, max_iter=500, model=None, tuner=
                                s.prob.ntime_segs)
             qr = ctx.to_device(np.ascontiguousarray(q.real.reshape(-1), np.float32))
             qi = ctx.to_device(np.ascontiguousarray(q.imag.reshape(-1), np.float32))
-            net = None
-            if model is not None:
-                net = ops.CNN(ctx, N, N, seed=1)
-                net.set_params(model)
-            classes[N] = dict(qr=qr, qi=qi, net=net)
+            classes[N] = dict(qr=qr, qi=qi)
     ctx.synchronize()
     t_q = time.perf_counter() - t0
-    # 3. per slice: plan, start, PCG (graph-captured iterations)
-    recs = []
-    for s, b in zip(slices, bs):
-        p = s.prob
-        N = p.Nx
-        cl = classes[N]
-        t0 = time.perf_counter()
-        h, keep = _plan(ctx, p, cl["qr"], cl["qi"])
-        br = ctx.to_device(np.ascontiguousarray(b.real, np.float32))
-        bi = ctx.to_device(np.ascontiguousarray(b.imag, np.float32))
+    # 3. per slice: learned start and graph-captured PCG.  Slices run on
+    # `streams` contexts (one HIP stream each) from as many host threads, so
+    # one slice's small kernels and per-iteration read-back overlap another's
+    # work.  Each worker keeps one plan per size class (the trajectory, Q_l
+    # and coils are shared within a class; imp_toep_set_fieldmap switches
+    # the field map) and preallocated vectors: no device allocation or free
+    # (which would synchronise the device) per slice.
+    from concurrent.futures import ThreadPoolExecutor
+    first = {}
+    for i, s in enumerate(slices):
+        first.setdefault(s.prob.Nx, i)
+    # pass-E schedule of each size class, tuned on the main context
+    cpg_of = {}
+    for N, i in first.items():
+        p = slices[i].prob
+        h, keep = _plan(ctx, p, classes[N]["qr"], classes[N]["qi"])
+        br, bi = ctx.alloc((p.nvox,)), ctx.alloc((p.nvox,))
         xr, xi = ctx.alloc((p.nvox,)), ctx.alloc((p.nvox,))
-        # pass-E schedule of this size class
         cpg0 = C.c_int()
         call("imp_toep_get_schedule", h, C.byref(cpg0))
         cands = sorted({c for c in (cpg0.value, 2 * cpg0.value, p.ncoils) if c <= p.ncoils})
-        cpg = tuner.pick(("pass_e", N), cands,
-                         lambda c: (call("imp_toep_set_schedule", h, C.c_int(c)),
-                                    call("imp_toep_apply", h, ptr(br), ptr(bi), ptr(xr), ptr(xi))),
-                         "toep_pass_e")
-        call("imp_toep_set_schedule", h, C.c_int(cpg))
-        flags = IMP_CG_BEST_ITERATE | IMP_CG_TRUE_RESIDUAL | IMP_CG_PRECOND | IMP_CG_GRAPH
-        if cl["net"] is not None:
-            zr, zi = ctx.alloc((p.nvox,)), ctx.alloc((p.nvox,))
-            call("imp_toep_precond_apply", h, ptr(br), ptr(bi), ptr(zr), ptr(zi))
-            cl["net"].forward_device(zr, zi, xr, xi)
+        cpg_of[N] = tuner.pick(("pass_e", N), cands,
+                               lambda c: (call("imp_toep_set_schedule", h, C.c_int(c)),
+                                          call("imp_toep_apply", h, ptr(br), ptr(bi), ptr(xr),
+                                               ptr(xi))),
+                               "toep_pass_e")
+        call("imp_toep_destroy", h)
+    nw = max(1, min(streams, len(slices)))
+    workers = [ctx] + [Context(ctx.device) for _ in range(nw - 1)]
+    wstate = [dict() for _ in workers]
+
+    def state(w, N, p):
+        st_ = wstate[w]
+        if N not in st_:
+            c = workers[w]
+            h, keep = _plan(c, p, classes[N]["qr"], classes[N]["qi"])
+            call("imp_toep_set_schedule", h, C.c_int(cpg_of[N]))
+            v = {k: c.alloc((p.nvox,)) for k in ("br", "bi", "xr", "xi", "zr", "zi")}
+            net = None
+            if model is not None:
+                net = ops.CNN(c, N, N, seed=1)
+                net.set_params(model)
+            st_[N] = dict(h=h, keep=keep, fm=c.alloc((p.nvox,)), net=net, **v)
+        return st_[N]
+
+    flags0 = IMP_CG_BEST_ITERATE | IMP_CG_TRUE_RESIDUAL | IMP_CG_PRECOND | IMP_CG_GRAPH
+
+    def solve(i):
+        w = i % nw
+        s, b = slices[i], bs[i]
+        p = s.prob
+        N = p.Nx
+        t0 = time.perf_counter()
+        S = state(w, N, p)
+        h = S["h"]
+        S["fm"].upload(np.ascontiguousarray(p.fm, np.float32))
+        call("imp_toep_set_fieldmap", h, ptr(S["fm"]))
+        S["br"].upload(np.ascontiguousarray(b.real, np.float32))
+        S["bi"].upload(np.ascontiguousarray(b.imag, np.float32))
+        flags = flags0
+        if S["net"] is not None:
+            call("imp_toep_precond_apply", h, ptr(S["br"]), ptr(S["bi"]), ptr(S["zr"]),
+                 ptr(S["zi"]))
+            S["net"].forward_device(S["zr"], S["zi"], S["xr"], S["xi"])
             flags |= IMP_CG_USE_X0 | IMP_CG_FALLBACK_ZERO
         st = CGStats()
-        call("imp_toep_cg", h, ptr(br), ptr(bi), ptr(xr), ptr(xi), C.c_int(max_iter),
-             C.c_double(tol), C.c_int(flags), C.byref(st))
+        call("imp_toep_cg", h, ptr(S["br"]), ptr(S["bi"]), ptr(S["xr"]), ptr(S["xi"]),
+             C.c_int(max_iter), C.c_double(tol), C.c_int(flags), C.byref(st))
         t_slice = time.perf_counter() - t0
-        call("imp_toep_destroy", h)
         r = st.as_dict()
-        r.update(N=N, slice_s=t_slice, cpg=cpg)
+        r.update(N=N, slice_s=t_slice, cpg=cpg_of[N], stream=w)
         if keep_images:
-            r.update(b=np.asarray(b), x=xr.get() + 1j * xi.get())
-        recs.append(r)
+            r.update(b=np.asarray(b), x=S["xr"].get() + 1j * S["xi"].get())
+        return r
+
+    # slices of one worker run in order on its stream; workers in parallel
+    def run_worker(w):
+        return [(i, solve(i)) for i in range(w, len(slices), nw)]
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=nw) as pool:
+        parts = list(pool.map(run_worker, range(nw)))
+    t_solve = time.perf_counter() - t0
+    recs = [r for _, r in sorted((x for part in parts for x in part), key=lambda x: x[0])]
+    for w, st_ in enumerate(wstate):
+        for S in st_.values():
+            call("imp_toep_destroy", S["h"])
+            if S["net"] is not None:
+                S["net"].close()
+    for c in workers[1:]:
+        c.close()
     t_total = time.perf_counter() - t_all
-    for cl in classes.values():
-        if cl["net"] is not None:
-            cl["net"].close()
     log(f"batch: {len(slices)} slices in {t_total:.3f} s ({len(slices) / t_total:.1f} slices/s); "
-        f"F^H d {t_fhd * 1e3:.1f} ms (one launch, chunk scale {scale}), Q {t_q * 1e3:.1f} ms")
-    return dict(slices=recs, total_s=t_total, fhd_s=t_fhd, q_s=t_q, nudft_chunk_scale=scale,
+        f"F^H d {t_fhd * 1e3:.1f} ms (one launch, chunk scale {scale}), Q {t_q * 1e3:.1f} ms, "
+        f"solves {t_solve * 1e3:.1f} ms on {nw} streams")
+    return dict(slices=recs, total_s=t_total, fhd_s=t_fhd, q_s=t_q, solve_s=t_solve,
+                streams=nw, nudft_chunk_scale=scale,
                 tuner=dict((str(k), v) for k, v in tuner.trials.items()))
 
 
@@ -195,6 +245,8 @@ def main(argv=None):
     ap.add_argument("--slices", type=int, default=48)
     ap.add_argument("--model", default=None, help="trained warm-start model (.npz)")
     ap.add_argument("--tol", type=float, default=1e-5)
+    ap.add_argument("--streams", type=int, default=3,
+                    help="contexts (HIP streams) the per-slice solves run on")
     ap.add_argument("--repeat", type=int, default=2,
                     help="passes over the batch with one autotuner; the first includes its trials")
     args = ap.parse_args(argv)
@@ -211,7 +263,8 @@ def main(argv=None):
         from .learn import load_model
         params, _, _ = load_model(args.model)
     tuner = Autotuner(ctx)
-    passes = [reconstruct(ctx, sl, tol=args.tol, model=params, tuner=tuner, log=log)
+    passes = [reconstruct(ctx, sl, tol=args.tol, model=params, tuner=tuner, log=log,
+                          streams=args.streams)
               for _ in range(max(1, args.repeat))]
     res = passes[-1]
     recs = res["slices"]
@@ -224,6 +277,7 @@ def main(argv=None):
         "total_s": round(res["total_s"], 4),
         "slices_per_s": round(len(recs) / res["total_s"], 2),
         "fhd_batch_s": round(res["fhd_s"], 4), "q_s": round(res["q_s"], 4),
+        "solve_s": round(res["solve_s"], 4), "streams": res["streams"],
         "converged": int(sum(r["converged"] for r in recs)),
         "max_true_rel_residual": max(r["true_rel_residual"] for r in recs),
         "per_size": {N: {"count": len(v),

@@ -169,6 +169,25 @@ static float dot_cpu(const float *a_r, const float *a_i, const float *b_r, const
     return s;
 }
 
+/* Exported for the timed CPU baseline of the vector stages (one thread, as
+ * the reference's DHWD2dCpu / dotProductCpu / addCpu run): the stencil, the
+ * real dot product and y += a x (addCpu.cpp) on split re/im arrays. */
+void fd_reg_cpu(float *s_r, float *s_i, const float *p_r, const float *p_i, int nx, int ny,
+                float lam) {
+    fd_cpu(s_r, s_i, p_r, p_i, nx, ny, lam);
+}
+
+float dot_re_cpu(const float *a_r, const float *a_i, const float *b_r, const float *b_i, int n) {
+    return dot_cpu(a_r, a_i, b_r, b_i, n);
+}
+
+void axpy_cpu(float *y_r, float *y_i, float a, const float *x_r, const float *x_i, int n) {
+    for (int i = 0; i < n; i++) {
+        y_r[i] += a * x_r[i];
+        y_i[i] += a * x_i[i];
+    }
+}
+
 /* bruteForceCpu's CG (bruteForce/bruteForceCpu.cpp:121-534, the loop at
  * :280-420) with the finite-difference regulariser: cg_num - 1 iterations
  * from x (in/out), grad = F^H (y - A f) - lam D^T D x, d = grad + gamma d,
