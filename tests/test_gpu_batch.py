@@ -94,3 +94,21 @@ def test_config4_48_slices_at_size(ctx):
         e = rel_l2(recs[i]["b"][idx], ref)
         print(f"N={N}: batched F^H d at 16 voxels vs FP64 {e:.2e}")
         assert e < 2e-6
+
+
+def test_batch_streams_give_identical_slices(ctx):
+    """The per-slice solves run on several streams from several host
+    threads; each slice's arithmetic does not depend on which stream runs it
+    or what runs beside it, so one stream and three give the same bits."""
+    from impatient import batch
+    sl = batch.make_slices(6, seed=91, ctx=ctx)
+    tuner = batch.Autotuner(ctx)   # one set of schedule choices for both runs
+    one = batch.reconstruct(ctx, sl, tol=1e-5, keep_images=True, streams=1, tuner=tuner,
+                            log=lambda m: None)
+    three = batch.reconstruct(ctx, sl, tol=1e-5, keep_images=True, streams=3, tuner=tuner,
+                              log=lambda m: None)
+    assert one["streams"] == 1 and three["streams"] == 3
+    for a, b in zip(one["slices"], three["slices"]):
+        assert a["iterations"] == b["iterations"]
+        assert np.array_equal(a["x"], b["x"])
+        assert np.array_equal(a["b"], b["b"])
