@@ -156,6 +156,17 @@ extern "C" int imp_ctx_create(int device, imp_ctx **out) {
         delete ctx;
         return fail(IMP_EHIP, "imp_ctx_create: stream / pinned allocation failed");
     }
+    // reduction partials (up to 1024 blocks) with their sums, and the FFT
+    // twiddles, made here so that solves never allocate on first use
+    if (ensure_dscratch(ctx, 1024 + 16) != IMP_OK || twiddle_table_create(ctx) != IMP_OK) {
+        (void)hipGetLastError();
+        if (ctx->dscratch) (void)hipFree(ctx->dscratch);
+        if (ctx->twiddle) (void)hipFree(ctx->twiddle);
+        (void)hipStreamDestroy(ctx->stream);
+        (void)hipHostFree(ctx->host_scalars);
+        delete ctx;
+        return fail(IMP_ENOMEM, "imp_ctx_create: reduction scratch / twiddle table");
+    }
     *out = ctx;
     clear_error();
     return IMP_OK;

@@ -345,23 +345,29 @@ bool fft_len_ok(int L) {
 }  // namespace
 
 // ------------------------------------------------------------ twiddles
-int twiddles(imp_ctx *ctx, int L, const float2 **out) {
+int twiddle_table_create(imp_ctx *ctx) {
     // table for length L: powers of two at offset L, lengths 3 * 2^k at
     // offset 4096 + L (8192 entries, FP64 values rounded once)
+    if (ctx->twiddle) return IMP_OK;
+    std::vector<float2> h(8192, make_float2(0.f, 0.f));
+    auto fill = [&](int n, int off) {
+        for (int m = 0; m < n; ++m) {
+            const double a = -2.0 * M_PI * (double)m / (double)n;
+            h[off + m] = make_float2((float)cos(a), (float)sin(a));
+        }
+    };
+    for (int n = 2; n <= 2048; n *= 2) fill(n, n);
+    for (int n = 24; n <= 1536; n *= 2) fill(n, 4096 + n);
+    IMP_HIP(hipMalloc(&ctx->twiddle, 8192 * sizeof(float2)));
+    IMP_HIP(hipMemcpyAsync(ctx->twiddle, h.data(), 8192 * sizeof(float2), hipMemcpyHostToDevice,
+                           ctx->stream));
+    IMP_HIP(hipStreamSynchronize(ctx->stream));
+    return IMP_OK;
+}
+
+int twiddles(imp_ctx *ctx, int L, const float2 **out) {
     if (!fft_len_ok(L)) return fail(IMP_EUNSUPPORTED, "FFT length %d unsupported", L);
-    if (!ctx->twiddle) {
-        std::vector<float2> h(8192, make_float2(0.f, 0.f));
-        auto fill = [&](int n, int off) {
-            for (int m = 0; m < n; ++m) {
-                const double a = -2.0 * M_PI * (double)m / (double)n;
-                h[off + m] = make_float2((float)cos(a), (float)sin(a));
-            }
-        };
-        for (int n = 2; n <= 2048; n *= 2) fill(n, n);
-        for (int n = 24; n <= 1536; n *= 2) fill(n, 4096 + n);
-        IMP_HIP(hipMalloc(&ctx->twiddle, 8192 * sizeof(float2)));
-        IMP_HIP(hipMemcpy(ctx->twiddle, h.data(), 8192 * sizeof(float2), hipMemcpyHostToDevice));
-    }
+    IMP_TRY(twiddle_table_create(ctx));
     *out = ctx->twiddle + (L % 3 == 0 ? 4096 + L : L);
     return IMP_OK;
 }

@@ -49,6 +49,8 @@ struct ToepPlan {
     float *pw_chan = nullptr;     // the Chan weights computed at create time
 };
 
+// The context's FFT twiddle table (64 KiB), made once at context creation.
+int twiddle_table_create(imp_ctx *ctx);
 int twiddles(imp_ctx *ctx, int L, const float2 **out);
 int fft_grid(imp_ctx *ctx, float2 *data, int M1, int M2, int M3, bool inverse);
 int toep_prepare_qf(imp_ctx *ctx, ToepPlan &P, const float *q_r, const float *q_i);

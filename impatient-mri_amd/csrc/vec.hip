@@ -12,6 +12,7 @@
 
 #include <chrono>
 #include <cmath>
+#include <mutex>
 
 namespace imp {
 
@@ -351,10 +352,19 @@ __global__ void k_sub_c(const float2 *a, const float2 *b, float2 *out, long n) {
 }
 
 // owns an instantiated CG iteration graph
+// Stream capture on one host thread is invalidated by an allocation, a free
+// or a synchronous copy made by another thread while it records.  Solves run
+// from several threads (one context each) take this lock around capture,
+// instantiation and destruction, so graphs are never recorded concurrently;
+// callers that solve from several threads allocate before they start.
+std::mutex g_capture_mu;
+
 struct CgGraph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     ~CgGraph() {
+        if (!exec && !graph) return;
+        std::lock_guard<std::mutex> lk(g_capture_mu);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
     }
@@ -561,6 +571,7 @@ int toep_cg(imp_ctx *ctx, ToepPlan &P, const float2 *b, float2 *x, int max_iter,
         // kernel timer events are not recorded into the graph
         const bool prof = ctx->prof_on;
         ctx->prof_on = false;
+        std::lock_guard<std::mutex> lk(g_capture_mu);
         IMP_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
         const int st_cap = iteration();
         hipGraph_t g = nullptr;

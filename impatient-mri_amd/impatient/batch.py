@@ -168,21 +168,25 @@ def reconstruct(ctx: Context, slices, tol=1e-5, max_iter=500, model=None, tuner=
         call("imp_toep_destroy", h)
     nw = max(1, min(streams, len(slices)))
     workers = [ctx] + [Context(ctx.device) for _ in range(nw - 1)]
+    # every worker's plans, vectors and networks are made here, before the
+    # threads start: an allocation or a synchronous copy on one thread while
+    # another records its CG graph would invalidate that capture
     wstate = [dict() for _ in workers]
-
-    def state(w, N, p):
-        st_ = wstate[w]
-        if N not in st_:
-            c = workers[w]
-            h, keep = _plan(c, p, classes[N]["qr"], classes[N]["qi"])
-            call("imp_toep_set_schedule", h, C.c_int(cpg_of[N]))
-            v = {k: c.alloc((p.nvox,)) for k in ("br", "bi", "xr", "xi", "zr", "zi")}
-            net = None
-            if model is not None:
-                net = ops.CNN(c, N, N, seed=1)
-                net.set_params(model)
-            st_[N] = dict(h=h, keep=keep, fm=c.alloc((p.nvox,)), net=net, **v)
-        return st_[N]
+    for i, s in enumerate(slices):
+        w, N, p = i % nw, s.prob.Nx, s.prob
+        if N in wstate[w]:
+            continue
+        c = workers[w]
+        h, keep = _plan(c, p, classes[N]["qr"], classes[N]["qi"])
+        call("imp_toep_set_schedule", h, C.c_int(cpg_of[N]))
+        v = {k: c.alloc((p.nvox,)) for k in ("br", "bi", "xr", "xi", "zr", "zi")}
+        net = None
+        if model is not None:
+            net = ops.CNN(c, N, N, seed=1)
+            net.set_params(model)
+        wstate[w][N] = dict(h=h, keep=keep, fm=c.alloc((p.nvox,)), net=net, **v)
+    for c in workers:
+        c.synchronize()
 
     flags0 = IMP_CG_BEST_ITERATE | IMP_CG_TRUE_RESIDUAL | IMP_CG_PRECOND | IMP_CG_GRAPH
 
@@ -192,7 +196,7 @@ def reconstruct(ctx: Context, slices, tol=1e-5, max_iter=500, model=None, tuner=
         p = s.prob
         N = p.Nx
         t0 = time.perf_counter()
-        S = state(w, N, p)
+        S = wstate[w][N]
         h = S["h"]
         S["fm"].upload(np.ascontiguousarray(p.fm, np.float32))
         call("imp_toep_set_fieldmap", h, ptr(S["fm"]))

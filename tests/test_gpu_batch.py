@@ -42,10 +42,13 @@ def test_config4_48_slices_at_size(ctx):
     model with the zero-start fallback, graph-captured PCG to 1e-5.
 
     Every slice's true residual (recomputed on the GPU after the solve) is
-    below 1e-5, and so is its residual under the FP64 operator (the oracle's
-    Toeplitz operator in complex128 on the GPU's FP64 FFT, from the same Q_l);
-    the batched F^H d of the first slice of each size class matches the FP64
-    oracle at 16 sampled voxels."""
+    below 1e-5.  Its residual under the FP64 operator (the oracle's Toeplitz
+    operator in complex128 on the GPU's FP64 FFT, from the same Q_l) is within
+    5e-7 of it: the FP32 apply is about 2.4e-7 (relative) from the FP64 one,
+    which moves a residual near 1e-5 by at most that much, so an image that
+    stopped just under 1e-5 may read 1.00x e-5 under FP64.  The batched F^H d
+    of the first slice of each size class matches the FP64 oracle at 16
+    sampled voxels."""
     from pathlib import Path
     import torch
     from impatient import batch, learn, ops
@@ -76,7 +79,8 @@ def test_config4_48_slices_at_size(ctx):
         res64 = (torch.linalg.norm(A(r["x"].astype(np.complex128)) - b) /
                  torch.linalg.norm(b)).item()
         worst[N] = max(worst.get(N, 0.0), res64)
-        assert res64 < 1e-5, (N, res64)
+        assert abs(res64 - r["true_rel_residual"]) < 5e-7, (N, res64, r["true_rel_residual"])
+        assert res64 < 1e-5 + 5e-7, (N, res64)
     for N in sizes:
         v = [r for r in recs if r["N"] == N]
         print(f"N={N}: {len(v)} slices, mean {np.mean([r['iterations'] for r in v]):.1f} it, "
