@@ -6,7 +6,8 @@ ill-conditioned systems: on the single-coil 16x16 problem an FP32-storage
 run of the FP64 oracle drifts 2.7e-2 from the FP64 iterate after 7
 iterations.  Parity is therefore asserted against the oracle with FP32
 vector storage (vec_dtype=complex64, FP64 scalars, as the GPU does), and the
-FP64 distance is bounded per case.
+FP64 distance is bounded by the FP32-storage run's own distance plus twice
+the pin (triangle inequality).
 """
 import numpy as np
 import pytest
@@ -34,19 +35,23 @@ def _run(ctx, J, S, ncoils, lam, cg_num=8):
     return got, ref32, ref64, cost
 
 
-@pytest.mark.parametrize("J,S,ncoils,lam,tol32,tol64",
-                         [(16, 512, 4, 1.0, 5e-3, 5e-3), (16, 512, 4, 0.1, 5e-3, 5e-3),
-                          (4, 128, 3, 1e-2, 5e-3, 5e-3), (4, 128, 1, 1e-2, 2e-2, 6e-2),
-                          (4, 128, 1, 0.0, 2e-2, 6e-2)])
-def test_bruteforce_matches_oracle(ctx, J, S, ncoils, lam, tol32, tol64):
+# tol32 about 3x the measured distance to the FP32-storage run (7.1e-4,
+# 7.1e-4, 3.4e-4, 1.7e-3, 5.5e-4); the single-coil cost differs by 1.0e-2 at
+# lambda = 1e-2 (the ill-conditioned directions barely change the cost)
+@pytest.mark.parametrize("J,S,ncoils,lam,tol32",
+                         [(16, 512, 4, 1.0, 2e-3), (16, 512, 4, 0.1, 2e-3),
+                          (4, 128, 3, 1e-2, 1e-3), (4, 128, 1, 1e-2, 5e-3),
+                          (4, 128, 1, 0.0, 2e-3)])
+def test_bruteforce_matches_oracle(ctx, J, S, ncoils, lam, tol32):
     got, ref32, ref64, cost = _run(ctx, J, S, ncoils, lam)
     e32, e64 = rel_l2(got, ref32), rel_l2(got, ref64)
     cg, c32 = cost(got.astype(np.complex128)), cost(ref32.astype(np.complex128))
     print(f"bruteforce C={ncoils} M={J * S} lam={lam}: vs FP32-storage {e32:.2e}, "
-          f"vs FP64 {e64:.2e}, cost {cg:.6e} vs {c32:.6e}")
+          f"vs FP64 {e64:.2e} (FP32-storage vs FP64 {rel_l2(ref32, ref64):.2e}), "
+          f"cost {cg:.6e} vs {c32:.6e}")
     assert e32 < tol32
-    assert e64 < tol64
-    assert abs(cg - c32) / c32 < 2e-2
+    assert e64 <= rel_l2(ref32, ref64) + 2 * e32
+    assert abs(cg - c32) / c32 < (2e-2 if ncoils == 1 else 5e-3)
 
 
 def test_bruteforce_tv_passes_match_oracle(ctx):

@@ -113,6 +113,8 @@ def test_computeQ_and_recon_file_level(data_dir):
     rg, rr = res(x.astype(np.complex128)), res(x64)
     print(f"recon 64x64x1 10 it: image vs FP32-storage CG {e32:.2e}, vs FP64 CG {e64:.2e}, "
           f"true residual {rg:.4e} vs {rr:.4e}")
-    assert e32 < 5e-3
-    assert e64 < 2e-2
-    assert abs(rg - rr) < 0.5 * rr
+    # measured: 4.9e-5 from the FP32-vector run, 4.0e-5 from the FP64 run,
+    # true residuals 1.352e-3 and 1.373e-3
+    assert e32 < 5e-4
+    assert e64 <= rel_l2(x32, x64) + 2 * e32
+    assert abs(rg - rr) < 0.05 * rr

@@ -106,9 +106,13 @@ def test_tv_weight_matches_oracle(ctx):
 
 
 def test_cg_matches_oracle_iterates(ctx):
-    """10 CG iterations against the FP64 CG of the same operator
-    (toeplitz_recon semantics) on a well-conditioned system (b = F^H y,
-    4 coils, 8192 samples, lambda = 1), best and last iterate."""
+    """10 CG iterations against the oracle CG of the same FP64 operator
+    (toeplitz_recon semantics; b = F^H y, 4 coils, 8192 samples, lambda = 1),
+    best and last iterate.  The reference CG keeps its vectors in FP32, and
+    an FP32-vector run of the oracle already ends 3.1e-2 from the all-FP64
+    one here, so the GPU is pinned to the FP32-vector oracle run (measured
+    9.2e-5) and its distance to the FP64 run is bounded by that run's own
+    distance plus twice the pin (triangle inequality, with margin)."""
     from impatient import ops, synth, IMP_CG_BEST_ITERATE
     from oracle import oracle as orc
     p = synth.tiny(N=16, ncoils=4, ntime_segs=3, J=16, S=512)
@@ -126,10 +130,11 @@ def test_cg_matches_oracle_iterates(ctx):
         e32, e64 = rel_l2(x, x32), rel_l2(x, x64)
         print(f"CG 10 iters best={best}: vs FP32-storage CG {e32:.2e}, vs FP64 CG {e64:.2e}, "
               f"res {st['rel_residual']:.3e} vs {hist32[-1]:.3e} / {hist64[-1]:.3e}")
+        spread = rel_l2(x32, x64)
         assert st["iterations"] == 10
-        assert abs(st["rel_residual"] - hist32[-1]) / hist32[-1] < 2e-2
-        assert e32 < 5e-3
-        assert e64 < 5e-2
+        assert abs(st["rel_residual"] - hist32[-1]) / hist32[-1] < 5e-3
+        assert e32 < 1e-3
+        assert e64 <= spread + 2 * e32
 
 
 def test_cg_converges_to_dense_solution(ctx):
@@ -189,6 +194,5 @@ def test_cg_continues_through_negative_curvature_like_the_reference(ctx):
     print(f"indefinite CG 4 iters: vs FP32-storage CG {e32:.2e}, vs FP64 CG {e64:.2e}, "
           f"res {st['rel_residual']:.3e} vs {h32[-1]:.3e} / {h64[-1]:.3e}")
     assert st["iterations"] == 4
-    assert abs(st["rel_residual"] - h32[-1]) / h32[-1] < 2e-2
-    assert e32 < 5e-3
-    assert e64 < 5e-2
+    assert abs(st["rel_residual"] - h32[-1]) / h32[-1] < 1e-4
+    assert e32 < 1e-5 and e64 < 1e-5
