@@ -46,7 +46,7 @@ MFMA_FLOPS_PER_PAIR = 24.0
 # rocprofv3 --pmc FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0), the factors a
 # 256 MiB device copy calibrates (profiles/traffic_512x512x1_32coil_pmc.md);
 # re-measure when the kernel changes
-NUDFT_TRAFFIC_BYTES = 2.0 * 3422184.6 * 1024 + 1.0 * 972259.7 * 1024
+NUDFT_TRAFFIC_BYTES = 2.0 * 3414246.9 * 1024 + 1.0 * 957937.5 * 1024
 NUDFT_TRAFFIC_SOURCE = ("constant from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of "
                         "bench.py --pmc-mode (profiles/traffic_512x512x1_32coil_pmc.md), "
                         "not measured in this run")
