@@ -233,3 +233,32 @@ def test_read_mtx_rejects_unsupported_and_bad_files(tmp_path):
     with pytest.raises(_lib.ImpError) as e:
         ops.read_mtx(tmp_path / "missing.rmtx")
     assert e.value.status == -3
+
+
+def test_preconditioner_network_weights_stay_positive_and_bounded():
+    """The preconditioner network's input and output maps (host side): the
+    input is log w_chan, mean removed, with DC moved to the image centre;
+    an output equal to its input gives back Chan's weights exactly; any
+    output gives positive weights within exp(+-PC_CLIP) of Chan's, so the
+    preconditioner M = F^H diag(1/w) F stays Hermitian positive definite."""
+    from impatient import learn
+    rng = np.random.default_rng(4)
+    chan = np.exp(rng.uniform(-4, 2, size=(32, 48)))
+    z, mu = learn.precond_input(chan)
+    assert abs(np.log(chan).mean() - mu) < 1e-12
+    # DC (index 0, 0 in natural order) sits at the centre after the shift
+    assert abs(z[16, 24].real - (np.log(chan[0, 0]) - mu)) < 1e-5
+    assert np.allclose(learn.precond_from_output(z, z, chan), chan, rtol=1e-6)
+    big = z + 50.0 * rng.standard_normal(z.shape).astype(np.float32)
+    w = learn.precond_from_output(big, z, chan).astype(np.float64)
+    r = np.log(w / chan)
+    assert np.all(w > 0) and np.all(np.isfinite(w))
+    assert r.max() <= learn.PC_CLIP + 1e-5 and r.min() >= -learn.PC_CLIP - 1e-5
+    assert r.max() > learn.PC_CLIP - 1e-3 and r.min() < -learn.PC_CLIP + 1e-3
+    # a change at one shifted position moves the weight of the matching
+    # natural-order frequency only
+    x = z.copy()
+    x[16 + 3, 24 - 5] += 1.0
+    w1 = learn.precond_from_output(x, z, chan).astype(np.float64)
+    moved = np.argwhere(np.abs(w1 / chan - 1) > 1e-6)
+    assert moved.tolist() == [[3, -5 % 48]]
