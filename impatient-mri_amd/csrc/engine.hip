@@ -411,14 +411,17 @@ extern "C" int imp_toep_create(imp_ctx *ctx, int N1, int N2, int N3, int ncoils,
     P.tau = tau;
     P.t0 = t0;
     P.lambda = lambda;
-    // coil groups for pass E: enough workgroups to fill the GPU (pass E
-    // holds 256 / (threads per line) image lines per workgroup)
+    // coil groups for pass E: enough workgroups to fill the GPU, 16 waves
+    // per CU (pass E holds row_threads / (threads per line) image lines per
+    // workgroup)
     {
         const int Ml = N3 > 1 ? P.M3 : P.M2;
-        const int nl = 256 / fftc::threads_per_line(Ml);
+        const int rt = fftc::row_threads(Ml);
+        const int nl = rt / fftc::threads_per_line(Ml);
         const long wgs_per_group = (long)ceil_div((long)N1 * (N3 > 1 ? N2 : 1), nl);
+        const long target = (1024L / rt) * ctx->num_cus;
         int groups = 1;
-        while (groups < ncoils && wgs_per_group * groups < 4L * ctx->num_cus) groups *= 2;
+        while (groups < ncoils && wgs_per_group * groups < target) groups *= 2;
         if (groups > ncoils) groups = ncoils;
         P.coils_per_group = ceil_div(ncoils, groups);
         P.coils_per_group_min = P.coils_per_group;

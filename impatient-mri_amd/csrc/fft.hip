@@ -79,15 +79,14 @@ __device__ __forceinline__ void run_fft(int tl, const float2 *__restrict__ tw, F
 // ---------------------------------------------------------------- rows
 // Contiguous lines: T = L/16 consecutive threads per line, so each global
 // load or store of a stage covers T consecutive elements.
-constexpr int ROW_THREADS = 512;   // 8 lines of 1024 points per workgroup
 template <int L>
-constexpr int row_lines() { return ROW_THREADS / fftc::threads_per_line(L); }
+constexpr int row_lines() { return fftc::row_threads(L) / fftc::threads_per_line(L); }
 template <int L>
 constexpr int row_lds() { return fftc::num_stages(L) > 1 ? row_lines<L>() * fftc::line_lds(L) : 1; }
 
 // In-place FFT of `nlines` contiguous lines of length L.
 template <int L, bool INV>
-__global__ __launch_bounds__(ROW_THREADS) void k_fft_rows(float2 *data, long nlines,
+__global__ __launch_bounds__(fftc::row_threads(L)) void k_fft_rows(float2 *data, long nlines,
                                                          const float2 *__restrict__ tw) {
     constexpr int T = fftc::threads_per_line(L);
     __shared__ float2 buf[row_lds<L>()];
@@ -203,7 +202,7 @@ __global__ __launch_bounds__((col_threads<L, MODE>())) void k_fft_cols(
 // contiguous), v = S_c * e^{-i 2pi f tau_l} * p, zero-padded to L, forward
 // FFT, written to G1 line ((c*N1 + y)*A + x)*L with y = o / N2o, x = o % N2o.
 template <int L>
-__global__ __launch_bounds__(ROW_THREADS) void k_toep_pass_a(
+__global__ __launch_bounds__(fftc::row_threads(L)) void k_toep_pass_a(
     const float2 *__restrict__ p, const float2 *__restrict__ sens,
     const float *__restrict__ f, float tau_l, int nimg_lines, int N1, int N2o, int A,
     int ncoils, float2 *__restrict__ g1, const float2 *__restrict__ tw) {
@@ -249,7 +248,7 @@ __global__ __launch_bounds__(ROW_THREADS) void k_toep_pass_a(
 // lines, keep the first nlast elements, sum conj(S_c) * z_c over the group
 // in registers, write partial[group][voxel].
 template <int L>
-__global__ __launch_bounds__(ROW_THREADS) void k_toep_pass_e(
+__global__ __launch_bounds__(fftc::row_threads(L)) void k_toep_pass_e(
     const float2 *__restrict__ g1, const float2 *__restrict__ sens, int nimg_lines, int N1,
     int N2o, int A, int ncoils, int coils_per_group, float2 *__restrict__ partial,
     const float2 *__restrict__ tw) {
@@ -412,12 +411,13 @@ int fft_grid(imp_ctx *ctx, float2 *data, int M1, int M2, int M3, bool inverse) {
     const long nl = (long)M1 * (d3 ? M2 : 1);
     IMP_L_SWITCH(Ml, {
         const int NL = row_lines<L_>();
+        const dim3 block(fftc::row_threads(L_));
         if (inverse)
-            hipLaunchKernelGGL((k_fft_rows<L_, true>), dim3(ceil_div(nl, NL)), dim3(ROW_THREADS),
-                               0, ctx->stream, data, nl, tw);
+            hipLaunchKernelGGL((k_fft_rows<L_, true>), dim3(ceil_div(nl, NL)), block, 0,
+                               ctx->stream, data, nl, tw);
         else
-            hipLaunchKernelGGL((k_fft_rows<L_, false>), dim3(ceil_div(nl, NL)), dim3(ROW_THREADS),
-                               0, ctx->stream, data, nl, tw);
+            hipLaunchKernelGGL((k_fft_rows<L_, false>), dim3(ceil_div(nl, NL)), block, 0,
+                               ctx->stream, data, nl, tw);
     });
     IMP_CHECK_LAUNCH();
     if (d3) {  // x axis: batch y, lines z
@@ -490,7 +490,7 @@ int toep_apply_normal(imp_ctx *ctx, ToepPlan &P, const float2 *p, float2 *out) {
         IMP_L_SWITCH(Ml, {
             const long nlines = (long)C * nimg_lines;
             hipLaunchKernelGGL((k_toep_pass_a<L_>), dim3(ceil_div(nlines, row_lines<L_>())),
-                               dim3(ROW_THREADS), 0, ctx->stream, p, P.sens, P.fcyc, tau_l,
+                               dim3(fftc::row_threads(L_)), 0, ctx->stream, p, P.sens, P.fcyc, tau_l,
                                nimg_lines, P.N1, N2o, A, C, P.g1, tw_last);
         });
         prof_end(ctx, rec);
@@ -536,8 +536,9 @@ int toep_apply_normal(imp_ctx *ctx, ToepPlan &P, const float2 *p, float2 *out) {
         rec = prof_begin(ctx, "toep_pass_e");
         IMP_L_SWITCH(Ml, {
             dim3 grid(ceil_div(nimg_lines, row_lines<L_>()), ngroups);
-            hipLaunchKernelGGL((k_toep_pass_e<L_>), grid, dim3(ROW_THREADS), 0, ctx->stream, P.g1,
-                               P.sens, nimg_lines, P.N1, N2o, A, C, cpg, P.partial, tw_last);
+            hipLaunchKernelGGL((k_toep_pass_e<L_>), grid, dim3(fftc::row_threads(L_)), 0,
+                               ctx->stream, P.g1, P.sens, nimg_lines, P.N1, N2o, A, C, cpg,
+                               P.partial, tw_last);
         });
         prof_end(ctx, rec);
         IMP_CHECK_LAUNCH();

@@ -201,6 +201,10 @@ __host__ __device__ constexpr int vals_per_thread(int L) {
     return has3(L) ? (L < 12 ? L : 12) : (L < 16 ? L : 16);
 }
 __host__ __device__ constexpr int threads_per_line(int L) { return L / vals_per_thread(L); }
+// Threads per workgroup of the row-transform passes: 512 (8 lines of 1024
+// points, whose LDS twiddle table then still leaves two workgroups per CU)
+// from 512-point lines up, 256 below.
+__host__ __device__ constexpr int row_threads(int L) { return L >= 512 ? 512 : 256; }
 __host__ __device__ constexpr int num_stages(int L) {
     return has3(L) ? (L <= 12 ? 1 : 1 + (ilog2c(L / 12) + 1) / 2)
                    : (L <= 16 ? 1 : (ilog2c(L) + 3) / 4);
