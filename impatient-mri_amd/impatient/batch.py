@@ -132,10 +132,8 @@ def reconstruct(ctx: Context, slices, tol=1e-5, max_iter=500, model=None, tuner=
     for s in slices:
         N = s.prob.Nx
         if N not in classes:
-            q = ops.toeplitz_q(ctx, s.prob.kx, s.prob.ky, None, s.prob.t, N, N, 1,
-                               s.prob.ntime_segs)
-            qr = ctx.to_device(np.ascontiguousarray(q.real.reshape(-1), np.float32))
-            qi = ctx.to_device(np.ascontiguousarray(q.imag.reshape(-1), np.float32))
+            qr, qi = ops.toeplitz_q_device(ctx, s.prob.kx, s.prob.ky, s.prob.t, N, N,
+                                           s.prob.ntime_segs)
             classes[N] = dict(qr=qr, qi=qi)
     ctx.synchronize()
     t_q = time.perf_counter() - t0
@@ -150,9 +148,13 @@ def reconstruct(ctx: Context, slices, tol=1e-5, max_iter=500, model=None, tuner=
     first = {}
     for i, s in enumerate(slices):
         first.setdefault(s.prob.Nx, i)
-    # pass-E schedule of each size class, tuned on the main context
+    # pass-E schedule of each size class, tuned on the main context (once:
+    # the tuner keeps its choice for later batches)
     cpg_of = {}
     for N, i in first.items():
+        if ("pass_e", N) in tuner.choice:
+            cpg_of[N] = tuner.choice[("pass_e", N)]
+            continue
         p = slices[i].prob
         h, keep = _plan(ctx, p, classes[N]["qr"], classes[N]["qi"])
         br, bi = ctx.alloc((p.nvox,)), ctx.alloc((p.nvox,))

@@ -142,6 +142,19 @@ def nudft_forward(ctx: Context, kx, ky, kz, t, x, y, z, fm, rho, sens=None, ncoi
     return _cplx(dr, di, (ncoils, nk))
 
 
+def toeplitz_q_device(ctx: Context, kx, ky, t, Nx, Ny, ntime_segs):
+    """Q_l of a 2D trajectory on the standard 2x grid, left on the device:
+    returns the real and imaginary planes [(L+1) * 4 Nx Ny] as DeviceArrays
+    (what imp_toep_create takes)."""
+    a = [ctx.to_device(_f32(v)) for v in (kx, ky, t)]
+    nq = 4 * Nx * Ny
+    qr, qi = _pair(ctx, (ntime_segs + 1) * nq)
+    call("imp_toeplitz_q", ctx.h, C.c_int(len(kx)), ptr(a[0]), ptr(a[1]), None, ptr(a[2]),
+         None, None, C.c_int(Nx), C.c_int(Ny), C.c_int(1), C.c_int(nq), None, None, None,
+         C.c_int(ntime_segs), C.c_int(0), ptr(qr), ptr(qi))
+    return qr, qi
+
+
 def toeplitz_q(ctx: Context, kx, ky, kz, t, Nx, Ny, Nz, ntime_segs, phi=None, flags=0,
                grid=None):
     """Q_l on the standard 2x grid (or on `grid` = (gx, gy, gz)); returns [(L+1), G]."""
