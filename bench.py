@@ -589,11 +589,12 @@ def main():
         log(f"Toeplitz plan {t_plan:.3f} s, apply {t_apply:.2f} ms (timer off)")
         # HBM roofline of the FFT passes: algorithmic bytes per launch (one
         # segment l, all coils; complex64 = 8 B; the padded grid keeps the
-        # N1 x 2N2 rows that hold data)
+        # N1 x 2N2 rows that hold data; Qf_l of this standard-grid Q_l is
+        # real, 4 B per point of the 2x grid)
         half = 2 * nv                                   # N1 x 2 N2 points
         pass_bytes = {
             "toep_pass_a": 8 * nv + 4 * nv + 8 * ncoils * nv + 8 * ncoils * half,
-            "toep_pass_c": 8 * ncoils * half * 2 + 8 * 4 * nv,
+            "toep_pass_c": 8 * ncoils * half * 2 + 4 * 4 * nv,
             "toep_pass_e": 8 * ncoils * half + 8 * ncoils * nv,
         }
         passes = {}

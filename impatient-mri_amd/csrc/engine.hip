@@ -433,6 +433,7 @@ extern "C" int imp_toep_create(imp_ctx *ctx, int N1, int N2, int N3, int ncoils,
 #define ALLOC(field, bytes)                                   \
     if (st == IMP_OK && (st = toep_alloc(op, (bytes), &p)) == IMP_OK) field = (decltype(field))p;
     ALLOC(P.qf, (size_t)P.nseg * P.G * sizeof(float2));
+    ALLOC(P.qfr, (size_t)P.nseg * P.G * sizeof(float));
     ALLOC(P.g1, (size_t)ncoils * N1 * A * Ml * sizeof(float2));
     ALLOC(P.partial, (size_t)ngroups * P.nvox * sizeof(float2));
     ALLOC(P.fcyc, (size_t)P.nvox * sizeof(float));

@@ -20,6 +20,7 @@
 #include "toeplitz.h"
 
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 namespace imp {
@@ -134,10 +135,11 @@ constexpr int col_threads() { return col_lines<L, MODE>() * fftc::threads_per_li
 // batch stride 0), inverse.  HALF_IN: inputs i >= L/2 are zero (the
 // zero-padded axis of the 2x grid); HALF_OUT: only outputs i < L/2 are
 // stored (the cropped axis).  blockIdx.y selects the batch (bstride).
-template <int L, int MODE, bool HALF_IN, bool HALF_OUT>
+// RQ (MODE 2): the spectrum is real and read from qr instead of qf.
+template <int L, int MODE, bool HALF_IN, bool HALF_OUT, bool RQ = false>
 __global__ __launch_bounds__((col_threads<L, MODE>())) void k_fft_cols(
     float2 *data, long bstride, long es, int nlines, const float2 *__restrict__ qf,
-    const float2 *__restrict__ tw) {
+    const float2 *__restrict__ tw, const float *__restrict__ qr) {
     constexpr int P = col_lines<L, MODE>();
     constexpr int E = fftc::vals_per_thread(L);
     __shared__ float2 buf[fftc::num_stages(L) > 1 ? fftc::line_lds(L) * P : 1];
@@ -177,18 +179,27 @@ __global__ __launch_bounds__((col_threads<L, MODE>())) void k_fft_cols(
         run_fft<L, true>(tl, tw, gld, [] {}, gst, lld, lst);
     } else {
         float2 regs[E];
-        const float2 *q = qf + (ok ? j : 0) + (long)tl * es;
+        const long qoff = (ok ? j : 0) + (long)tl * es;
         // Qf_l of this thread's elements, fetched before the forward
         // transform so that their latency hides under it (108 -> 93 us per
         // launch at 1024-point columns)
-        float2 qv[E];
+        using QV = typename std::conditional<RQ, float, float2>::type;
+        QV qv[E];
 #pragma unroll
-        for (int m = 0; m < E; ++m) qv[m] = q[m * step];
+        for (int m = 0; m < E; ++m) {
+            if constexpr (RQ)
+                qv[m] = qr[qoff + m * step];
+            else
+                qv[m] = qf[qoff + m * step];
+        }
         run_fft<L, false>(
             tl, tw, gld, [] {},
             [&](int b, int r, int, float2 v) {
                 const int m = fftc::last_reg<L>(b, r);
-                regs[m] = ok ? cmul(v, qv[m]) : v;
+                if constexpr (RQ)
+                    regs[m] = ok ? make_float2(v.x * qv[m], v.y * qv[m]) : v;
+                else
+                    regs[m] = ok ? cmul(v, qv[m]) : v;
             },
             lld, lst);
         run_fft<L, true>(
@@ -327,6 +338,11 @@ __global__ void k_toep_pass_f(const float2 *__restrict__ partial, int ngroups, l
 }
 
 // ifftshift of Q_l (split, standard grid) into interleaved Qf: out[m] = Q[(m + M/2) mod M]
+// Q_l in DFT order (index m holds offset m, or m - M past M/2).  Offsets
+// of -N on an axis (index M/2) never enter the cropped product (two image
+// points differ by at most N - 1 along it), so they are stored as zero:
+// Q_l is conjugate-symmetric on all other offsets, and the edge values
+// would be the only source of an imaginary part in its spectrum.
 __global__ void k_ifftshift_q(const float *__restrict__ qr, const float *__restrict__ qi,
                               int M1, int M2, int M3, float2 *__restrict__ out) {
     const long n = (long)M1 * M2 * M3;
@@ -335,11 +351,38 @@ __global__ void k_ifftshift_q(const float *__restrict__ qr, const float *__restr
     const int m3 = (int)(m % M3);
     const int m2 = (int)((m / M3) % M2);
     const int m1 = (int)(m / ((long)M3 * M2));
+    if (m1 == M1 / 2 || m2 == M2 / 2 || (M3 > 1 && m3 == M3 / 2)) {
+        out[m] = make_float2(0.f, 0.f);
+        return;
+    }
     const int s1 = (m1 + M1 / 2) % M1;
     const int s2 = (m2 + M2 / 2) % M2;
     const int s3 = M3 > 1 ? (m3 + M3 / 2) % M3 : 0;
     const long src = ((long)s1 * M2 + s2) * M3 + s3;
     out[m] = make_float2(qr[src], qi[src]);
+}
+
+// Sets *asym when Q_l (DFT order) is not exactly conjugate-symmetric:
+// q[m] != conj(q[-m]) for some m (indices mod M per axis).
+__global__ void k_q_conj_check(const float2 *__restrict__ q, int M1, int M2, int M3,
+                               unsigned int *asym) {
+    const long n = (long)M1 * M2 * M3;
+    const long m = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    const int m3 = (int)(m % M3);
+    const int m2 = (int)((m / M3) % M2);
+    const int m1 = (int)(m / ((long)M3 * M2));
+    const long pm = ((long)((M1 - m1) % M1) * M2 + (M2 - m2) % M2) * M3 + (M3 - m3) % M3;
+    const float2 a = q[m], b = q[pm];
+    if (a.x != b.x || a.y != -b.y) atomicOr(asym, 1u);
+}
+
+__global__ void k_zero_flag(unsigned int *p) { *p = 0u; }
+
+__global__ void k_qf_real(const float2 *__restrict__ qf, long n, float *__restrict__ qr) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long)gridDim.x * blockDim.x)
+        qr[i] = qf[i].x;
 }
 
 // ---------------------------------------------------------------- dispatch
@@ -427,11 +470,11 @@ int fft_grid(imp_ctx *ctx, float2 *data, int M1, int M2, int M3, bool inverse) {
             if (inverse)
                 hipLaunchKernelGGL((k_fft_cols<L_, 1, false, false>), grid,
                                    dim3(col_threads<L_, 1>()), 0, ctx->stream, data, (long)M2 * M3,
-                                   (long)M3, M3, nullptr, tw);
+                                   (long)M3, M3, nullptr, tw, nullptr);
             else
                 hipLaunchKernelGGL((k_fft_cols<L_, 0, false, false>), grid,
                                    dim3(col_threads<L_, 0>()), 0, ctx->stream, data, (long)M2 * M3,
-                                   (long)M3, M3, nullptr, tw);
+                                   (long)M3, M3, nullptr, tw, nullptr);
         });
         IMP_CHECK_LAUNCH();
     }
@@ -442,10 +485,10 @@ int fft_grid(imp_ctx *ctx, float2 *data, int M1, int M2, int M3, bool inverse) {
         dim3 grid(ceil_div(nlines, col_lines<L_, 0>()), 1);
         if (inverse)
             hipLaunchKernelGGL((k_fft_cols<L_, 1, false, false>), grid, dim3(col_threads<L_, 1>()),
-                               0, ctx->stream, data, 0L, (long)nlines, nlines, nullptr, tw);
+                               0, ctx->stream, data, 0L, (long)nlines, nlines, nullptr, tw, nullptr);
         else
             hipLaunchKernelGGL((k_fft_cols<L_, 0, false, false>), grid, dim3(col_threads<L_, 0>()),
-                               0, ctx->stream, data, 0L, (long)nlines, nlines, nullptr, tw);
+                               0, ctx->stream, data, 0L, (long)nlines, nlines, nullptr, tw, nullptr);
     });
     IMP_CHECK_LAUNCH();
     return IMP_OK;
@@ -454,12 +497,32 @@ int fft_grid(imp_ctx *ctx, float2 *data, int M1, int M2, int M3, bool inverse) {
 // Qf_l = FFT(ifftshift(Q_l)) for all l, into plan-owned HBM.
 int toep_prepare_qf(imp_ctx *ctx, ToepPlan &P, const float *q_r, const float *q_i) {
     const long G = P.G;
+    // With the edges zeroed, a Q_l that is exactly conjugate-symmetric
+    // (Q_l(-r) = conj(Q_l(r)) bit for bit, as the standard-grid kernels
+    // write it) has a real spectrum: its imaginary part is FFT rounding, and
+    // pass C multiplies by the real plane (half the bytes per element).
+    // Any other Q_l (read from files, computed point by point on explicit
+    // grids) keeps the complex spectrum as given.
+    IMP_TRY(ensure_dscratch(ctx, 1));
+    unsigned int *asym = (unsigned int *)ctx->dscratch;
+    hipLaunchKernelGGL(k_zero_flag, dim3(1), dim3(1), 0, ctx->stream, asym);
     for (int l = 0; l < P.nseg; ++l) {
         float2 *dst = P.qf + (long)l * G;
         hipLaunchKernelGGL(k_ifftshift_q, dim3(ceil_div(G, 256)), dim3(256), 0, ctx->stream,
                            q_r + (long)l * G, q_i + (long)l * G, P.M1, P.M2, P.M3, dst);
+        hipLaunchKernelGGL(k_q_conj_check, dim3(ceil_div(G, 256)), dim3(256), 0, ctx->stream,
+                           dst, P.M1, P.M2, P.M3, asym);
         IMP_CHECK_LAUNCH();
         IMP_TRY(fft_grid(ctx, dst, P.M1, P.M2, P.M3, false));
+    }
+    unsigned int h = 1u;
+    IMP_HIP(hipMemcpyAsync(&h, asym, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    IMP_HIP(hipStreamSynchronize(ctx->stream));
+    P.qf_real = h == 0u && P.qfr;
+    if (P.qf_real) {
+        hipLaunchKernelGGL(k_qf_real, dim3(1024), dim3(256), 0, ctx->stream, P.qf,
+                           (long)P.nseg * G, P.qfr);
+        IMP_CHECK_LAUNCH();
     }
     return IMP_OK;
 }
@@ -502,7 +565,7 @@ int toep_apply_normal(imp_ctx *ctx, ToepPlan &P, const float2 *p, float2 *out) {
                 dim3 grid(ceil_div(P.M3, col_lines<L_, 0>()), C * P.N1);
                 hipLaunchKernelGGL((k_fft_cols<L_, 0, true, false>), grid, dim3(col_threads<L_, 0>()),
                                    0, ctx->stream, P.g1, (long)P.M2 * P.M3, (long)P.M3, P.M3,
-                                   nullptr, tw_x);
+                                   nullptr, tw_x, nullptr);
             });
             prof_end(ctx, rec);
             IMP_CHECK_LAUNCH();
@@ -512,10 +575,17 @@ int toep_apply_normal(imp_ctx *ctx, ToepPlan &P, const float2 *p, float2 *out) {
             const int nl = A * Ml;
             const float2 *qf = P.qf + (long)l * P.G;
             rec = prof_begin(ctx, "toep_pass_c");
+            const float *qr = P.qf_real ? P.qfr + (long)l * P.G : nullptr;
             IMP_L_SWITCH(P.M1, {
                 dim3 grid(ceil_div(nl, col_lines<L_, 2>()), C);
-                hipLaunchKernelGGL((k_fft_cols<L_, 2, true, true>), grid, dim3(col_threads<L_, 2>()),
-                                   0, ctx->stream, P.g1, (long)P.N1 * nl, (long)nl, nl, qf, tw_y);
+                if (qr)
+                    hipLaunchKernelGGL((k_fft_cols<L_, 2, true, true, true>), grid,
+                                       dim3(col_threads<L_, 2>()), 0, ctx->stream, P.g1,
+                                       (long)P.N1 * nl, (long)nl, nl, nullptr, tw_y, qr);
+                else
+                    hipLaunchKernelGGL((k_fft_cols<L_, 2, true, true>), grid,
+                                       dim3(col_threads<L_, 2>()), 0, ctx->stream, P.g1,
+                                       (long)P.N1 * nl, (long)nl, nl, qf, tw_y, nullptr);
             });
             prof_end(ctx, rec);
             IMP_CHECK_LAUNCH();
@@ -527,7 +597,7 @@ int toep_apply_normal(imp_ctx *ctx, ToepPlan &P, const float2 *p, float2 *out) {
                 dim3 grid(ceil_div(P.M3, col_lines<L_, 1>()), C * P.N1);
                 hipLaunchKernelGGL((k_fft_cols<L_, 1, false, true>), grid, dim3(col_threads<L_, 1>()),
                                    0, ctx->stream, P.g1, (long)P.M2 * P.M3, (long)P.M3, P.M3,
-                                   nullptr, tw_x);
+                                   nullptr, tw_x, nullptr);
             });
             prof_end(ctx, rec);
             IMP_CHECK_LAUNCH();

@@ -2276,6 +2276,43 @@ __global__ void k_q_outer(int nseg, long n2, int M3, const float *q2r, const flo
     }
 }
 
+// Standard grid, y outer, x, z inner (y = iy - Ny, x = ix - Nx, z = iz - Nz):
+// Q(-r) = conj(Q(r)) exactly.  The mirror-filled paths compute the row
+// y = 0 point by point, so its pairs (x, 0, z), (-x, 0, -z) agree only to
+// rounding; here every point after the origin in (y, x, z) order whose
+// mirror is on the grid takes the conjugate of that mirror, and Q(0) is
+// real.  (Points with y = -Ny, x = -Nx or z = -Nz have no mirror.)
+__global__ void k_q_conj_sym(int nseg, int Nx, int Ny, int Nz, long nq, float *q_r,
+                             float *q_i) {
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nq) return;
+    const int M2 = 2 * Nx, M3 = Nz > 1 ? 2 * Nz : 1;
+    const int iz = (int)(g % M3);
+    const int ix = (int)((g / M3) % M2);
+    const int iy = (int)(g / ((long)M3 * M2));
+    if (iy == 0 || ix == 0 || (M3 > 1 && iz == 0)) return;
+    const int y = iy - Ny, x = ix - Nx, z = M3 > 1 ? iz - Nz : 0;
+    if (y == 0 && x == 0 && z == 0) {
+        for (int l = 0; l < nseg; ++l) q_i[(size_t)l * nq + g] = 0.f;
+        return;
+    }
+    const bool upper = y > 0 || (y == 0 && (x > 0 || (x == 0 && z > 0)));
+    if (!upper) return;
+    const long m = ((long)(2 * Ny - iy) * M2 + (2 * Nx - ix)) * M3 + (M3 > 1 ? 2 * Nz - iz : 0);
+    for (int l = 0; l < nseg; ++l) {
+        q_r[(size_t)l * nq + g] = q_r[(size_t)l * nq + m];
+        q_i[(size_t)l * nq + g] = -q_i[(size_t)l * nq + m];
+    }
+}
+
+int q_conj_sym(imp_ctx *ctx, int nseg, int Nx, int Ny, int Nz, float *q_r, float *q_i) {
+    const long nq = 4L * Nx * Ny * (Nz > 1 ? 2L * Nz : 1L);
+    hipLaunchKernelGGL(k_q_conj_sym, dim3(ceil_div(nq, 256)), dim3(256), 0, ctx->stream, nseg,
+                       Nx, Ny, Nz, nq, q_r, q_i);
+    IMP_CHECK_LAUNCH();
+    return IMP_OK;
+}
+
 }  // namespace
 
 int toeplitz_q(imp_ctx *ctx, int num_k, const float *kx, const float *ky,
@@ -2326,7 +2363,7 @@ int toeplitz_q(imp_ctx *ctx, int num_k, const float *kx, const float *ky,
                                (const float *)q2i, (const double2 *)Z, q_r, q_i);
             prof_end(ctx, rec);
             IMP_CHECK_LAUNCH();
-            return IMP_OK;
+            return q_conj_sym(ctx, nseg, Nx, Ny, Nz, q_r, q_i);
         }
     }
     // tau and t0 as the reference computes them, in FP32 on the host
@@ -2351,6 +2388,7 @@ int toeplitz_q(imp_ctx *ctx, int num_k, const float *kx, const float *ky,
         const int rc = toeplitz_q_sep(ctx, num_k, kx, ky, d3 ? kz : nullptr, t, spos,
                                       (const float2 *)W, d3, Nx, Ny, Nz, nseg, tau, t0,
                                       mode == Out::PerRhsCumulative, q_r, q_i);
+        if (rc == IMP_OK) return q_conj_sym(ctx, nseg, Nx, Ny, Nz, q_r, q_i);
         if (rc != QS_FALLBACK) return rc;
     }
     const int M2 = 2 * Nx, M3 = Nz > 1 ? 2 * Nz : 1;
@@ -2382,7 +2420,7 @@ int toeplitz_q(imp_ctx *ctx, int num_k, const float *kx, const float *ky,
                        Nz, n1, ne_row, n_comp, nseg, (const float *)h_r, (const float *)h_i, q_r,
                        q_i);
     IMP_CHECK_LAUNCH();
-    return IMP_OK;
+    return q_conj_sym(ctx, nseg, Nx, Ny, Nz, q_r, q_i);
 }
 
 }  // namespace imp

@@ -31,7 +31,9 @@ struct ToepPlan {
     int coils_per_group_min = 8;  // the grouping the partial buffer was sized for
     float tau = 0.f, t0 = 0.f;
     float lambda = 0.f;
-    float2 *qf = nullptr;         // [nseg][G]  FFT(ifftshift(Q_l))
+    float2 *qf = nullptr;         // [nseg][G]  FFT(ifftshift(Q_l)), offsets -N zeroed
+    float *qfr = nullptr;         // [nseg][G]  Re(qf), used when qf_real
+    bool qf_real = false;         // qf is real to rounding (conjugate-symmetric Q_l)
     float2 *g1 = nullptr;         // [ncoils][N1][A][Ml] pass buffer
     float2 *partial = nullptr;    // [ngroups][nvox]
     float2 *sens = nullptr;       // [ncoils][nvox] or null (S = 1)

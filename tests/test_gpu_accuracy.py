@@ -245,10 +245,12 @@ def test_config3_images_against_fp64(ctx):
       * converged images (learned start, Chan PCG to 1e-5 and 1e-6) against
         a reference solution: FP64 PCG on the correction of the GPU's 1e-6
         image.
-    Asserted: the GPU reaches 1e-5 in the FP64 run's iteration count (+-2);
-    its distance from the FP64 run is within 2.5x that of the all-FP32
-    restatement (FP32 rounding, not a kernel error); in the band it is below
-    2.5e-4 and, like the FP64 run's own 1e-5 image, equally far from the
+    Asserted: the GPU reaches 1e-5 in the all-FP32 restatement's iteration
+    count (+-2), and the FP64 run no later; against the FP64 run's iterate
+    after as many iterations, the GPU's distance is within 2.5x that of the
+    all-FP32 restatement, overall and in the band (FP32 rounding, not a
+    kernel error); the GPU is within 1e-3 of that restatement; the GPU's
+    1e-5 image is, like the FP64 run's own 1e-5 image, equally far from the
     reference solution."""
     import ctypes as C
     from pathlib import Path
@@ -291,32 +293,40 @@ def test_config3_images_against_fp64(ctx):
     A32 = ft.ToeplitzOperator(q, tau, t0, p.shape, p.fm, p.sens, p.lam, complex64=True)
     M = ft.circulant(w)
     x64, it64, _ = ft.pcg(A64, M, b64, 200, tol=1e-5)
-    x_v32, _, _ = ft.pcg(A64, M, b64, 200, tol=1e-5, store=np.complex64)
-    x_a32, _, _ = ft.pcg(A32, M, b64, 200, tol=1e-5)
-    x_32, _, _ = ft.pcg(A32, M, b64, 200, tol=1e-5, store=np.complex64)
+    # the FP64 run for as many iterations as the GPU's (compared iterate
+    # for iterate: with the exactly conjugate-symmetric Q_l the FP64 run
+    # crosses 1e-5 a few iterations earlier than any run with an FP32 part)
+    itg = s5["iterations"]
+    x64n, _, _ = ft.pcg(A64, M, b64, itg)
+    x_v32, itv32, _ = ft.pcg(A64, M, b64, 200, tol=1e-5, store=np.complex64)
+    x_a32, ita32, _ = ft.pcg(A32, M, b64, 200, tol=1e-5)
+    x_32, it32, _ = ft.pcg(A32, M, b64, 200, tol=1e-5, store=np.complex64)
     # reference solution: FP64 PCG on the correction of the GPU's 1e-6 image
     xt64 = xt.astype(np.complex128)
     r0 = b64 - A64(xt64).cpu().numpy()
     delta, itc, _ = ft.pcg(A64, M, r0, 150, tol=1e-10 * np.linalg.norm(b64) / np.linalg.norm(r0))
     xref = xt64 + delta
     res_ref = np.linalg.norm(b64 - A64(xref).cpu().numpy()) / np.linalg.norm(b64)
-    e = dict(gpu_vs_fp64=rel_l2(xc5, x64), fp32_vectors_vs_fp64=rel_l2(x_v32, x64),
-             fp32_operator_vs_fp64=rel_l2(x_a32, x64), all_fp32_vs_fp64=rel_l2(x_32, x64),
-             gpu_vs_all_fp32=rel_l2(xc5, x_32),
+    e = dict(gpu_vs_fp64=rel_l2(xc5, x64n), fp32_vectors_vs_fp64=rel_l2(x_v32, x64n),
+             fp32_operator_vs_fp64=rel_l2(x_a32, x64n), all_fp32_vs_fp64=rel_l2(x_32, x64n),
+             gpu_vs_all_fp32=rel_l2(xc5, x_32), fp64_1e5_vs_fp64_same_count=rel_l2(x64, x64n),
              learned_true_res=sl["true_rel_residual"], chan_1e5_true_res=s5["true_rel_residual"],
              chan_1e6_true_res=stt["true_rel_residual"], reference_residual=res_ref)
-    band, corners = _band_split(xc5, x64, N)
+    band, corners = _band_split(xc5, x64n, N)
     e["gpu_vs_fp64_band"], e["gpu_vs_fp64_corners"] = band, corners
+    e["all_fp32_vs_fp64_band"], e["all_fp32_vs_fp64_corners"] = _band_split(x_32, x64n, N)
     for name, v in (("learned_1e5", xl), ("chan_1e5", xc5), ("fp64_chan_1e5", x64),
                     ("chan_1e6", xt)):
         e[name + "_ref_band"], e[name + "_ref_corners"] = _band_split(v, xref, N)
     _report(f"cfg3 512x512 32 coils (GPU Chan PCG {s5['iterations']} iterations to 1e-5, FP64 "
-            f"{it64}; reference: {itc} iterations on the correction; "
+            f"{it64}, FP32 vectors {itv32}, FP32 operator {ita32}, both {it32}; reference: {itc} "
+            f"iterations on the correction; "
             f"{time.perf_counter() - t_start:.0f} s)", **e)
     assert res_ref < 1e-6
-    assert abs(s5["iterations"] - it64) <= 2
+    assert abs(itg - it32) <= 2 and it64 <= itg + 2
     assert e["gpu_vs_fp64"] < 2.5 * e["all_fp32_vs_fp64"]
-    assert band < 2.5e-4
+    assert band < 2.5 * e["all_fp32_vs_fp64_band"]
+    assert e["gpu_vs_all_fp32"] < 1e-3
     assert e["chan_1e5_ref_band"] <= 1.1 * e["fp64_chan_1e5_ref_band"] + 1e-4
     assert e["chan_1e6_ref_band"] < e["chan_1e5_ref_band"]
     assert max(sl["true_rel_residual"], s5["true_rel_residual"]) < 1e-5

@@ -298,9 +298,19 @@ def test_cli_toeplitz_gridding(data_dir):
     A = lambda v: orc.toeplitz_apply(v.reshape(n, n), qe, tau, t0, fm_eng, None, 0.0).reshape(-1)
     b_eng = b.reshape(n, n).T.reshape(-1).astype(np.complex128)
     x32, _, _ = orc.toeplitz_cg(A, b_eng, 10, vec_dtype=np.complex64)
-    ex = rel_l2(x.reshape(n, n).T.reshape(-1), x32)
+    x64, _, _ = orc.toeplitz_cg(A, b_eng, 10)
+    xe = x.reshape(n, n).T.reshape(-1)
+    ex, spread = rel_l2(xe, x32), rel_l2(x32, x64)
+    res = lambda v: np.linalg.norm(A(v) - b_eng) / np.linalg.norm(b_eng)
+    rg, r32 = res(xe.astype(np.complex128)), res(x32.astype(np.complex128))
     print(f"CLI gridding: Q vs FP64 gridding {eq:.2e}, F^H d {eb:.2e}, image vs FP32-storage "
-          f"oracle CG {ex:.2e}")
+          f"oracle CG {ex:.2e} (oracle FP32 vs FP64 vectors {spread:.2e}), true residual "
+          f"{rg:.4e} vs {r32:.4e}")
     # FP32 sample positions s = os (2k + n/2) on a 160-point grid (as the
     # reference computes them) against the FP64 restatement: ~1e-5
-    assert eq < 2e-5 and eb < 2e-5 and ex < 5e-3
+    assert eq < 2e-5 and eb < 2e-5
+    # lambda = 0 and 10 iterations: the oracle's own FP32- and FP64-vector
+    # runs differ by 1.0e-2 here, so the GPU's FP32 run is held to that
+    # spread, and its true residual to the oracle's within 5%
+    assert ex < max(5e-3, 2 * spread)
+    assert abs(rg - r32) < 0.05 * r32

@@ -192,8 +192,9 @@ def test_q_stacked_trajectory_path(ctx, N, Nz, L, phi_mode):
 def test_q_nonsquare_grid_and_hermitian_fill(ctx, Nx, Ny, Nz):
     """The standard grid is evaluated on half the points and mirrored:
     check every point against the FP64 oracle on non-square grids (mirror
-    edges x = -Nx, z = -Nz; Ny = 1 has no mirrored row) and that the mirror
-    pairs are exact conjugates."""
+    edges x = -Nx, z = -Nz; Ny = 1 has no mirrored row) and that every
+    point with a mirror on the grid, the row y = 0 included, is the exact
+    conjugate of it (Q(0) real)."""
     from impatient import ops
     from oracle import oracle as orc
     rng = np.random.default_rng(5)
@@ -208,15 +209,16 @@ def test_q_nonsquare_grid_and_hermitian_fill(ctx, Nx, Ny, Nz):
     err = rel_l2(got, ref)
     print(f"Q {Nx}x{Ny}x{Nz}: rel_l2={err:.2e}")
     assert err < OP_TOL
-    # rows y > 0 with a mirror on the grid are filled from rows y < 0
+    # every point whose mirror is on the grid (y > -Ny, x > -Nx, z > -Nz)
     M3 = 2 * Nz if Nz > 1 else 1
     g = got.reshape(L + 1, 2 * Ny, 2 * Nx, M3)
     z0 = 1 if M3 > 1 else 0
-    for iy in range(Ny + 1, 2 * Ny):
+    for iy in range(1, 2 * Ny):
         for ix in range(1, 2 * Nx):
             for iz in range(z0, M3):
                 mz = 2 * Nz - iz if M3 > 1 else 0
                 assert np.array_equal(g[:, iy, ix, iz], np.conj(g[:, 2 * Ny - iy, 2 * Nx - ix, mz]))
+    assert np.all(g[:, Ny, Nx, Nz if M3 > 1 else 0].imag == 0)
 
 
 def test_q_cumulative_compat(ctx):

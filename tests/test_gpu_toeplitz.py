@@ -35,6 +35,35 @@ def test_apply_matches_oracle(ctx, N, Nz, ncoils, L, lam):
     assert err < 2e-6
 
 
+@pytest.mark.parametrize("N,Nz", [(16, 1), (8, 4)])
+def test_apply_real_and_complex_spectrum_paths(ctx, N, Nz):
+    """A Q_l that is exactly conjugate-symmetric (the GPU's standard-grid
+    Q_l) has a real spectrum once the offsets -N, which never reach the
+    cropped product, are zeroed: the apply then multiplies by Re(Qf_l).  Any
+    other Q_l keeps the complex spectrum.  Both match the oracle's apply on
+    the same Q_l: the GPU's own Q_l, and that Q_l plus a 10% perturbation
+    that is not conjugate-symmetric (with the real path taken by mistake it
+    would be ~1e-1 off)."""
+    from impatient import ops
+    from oracle import oracle as orc
+    p, _, tau, t0 = _setup(N, Nz, 2, 3)
+    q = ops.toeplitz_q(ctx, p.kx, p.ky, p.kz, p.t, p.Nx, p.Ny, p.Nz, 3)
+    rng = np.random.default_rng(9)
+    pert = rng.standard_normal(q.shape) + 1j * rng.standard_normal(q.shape)
+    q_asym = (q + 0.1 * np.abs(q).max() * pert).astype(np.complex64)
+    shape = p.shape if Nz > 1 else p.shape[:2]
+    v = (rng.standard_normal(p.nvox) + 1j * rng.standard_normal(p.nvox)).astype(np.complex64)
+    for name, qq in (("conjugate-symmetric", q), ("perturbed", q_asym)):
+        op = ops.Toeplitz(ctx, p.shape, qq, tau, t0, p.fm, p.sens, 1e-2)
+        got = op.apply(v)
+        op.close()
+        ref = orc.toeplitz_apply(v.reshape(shape), qq.astype(np.complex128), tau, t0, p.fm,
+                                 p.sens, 1e-2).reshape(-1)
+        err = rel_l2(got, ref)
+        print(f"N={N} Nz={Nz} {name} Q_l: apply vs oracle {err:.2e}")
+        assert err < 2e-6
+
+
 def test_apply_is_normal_operator_without_fieldmap(ctx):
     """With w = 0 and L = 1 (partition of unity), the Toeplitz apply is
     exactly F^H F: compare with forward then adjoint on the GPU."""
