@@ -13,7 +13,8 @@ once (complex64 = 8 bytes) on the pruned 2x grid:
             the G1 lines along the last axis (8C x 2 per voxel)
     pass B  (3D) reads the N2 kept x values, writes 2N2            (16C, 32C)
     pass C  reads and writes the N1 kept y values of every (x, z)
-            line of the 2x grid, plus Qf_l (8 x 2^d per voxel)
+            line of the 2x grid, plus Qf_l (real for the conjugate-symmetric
+            Q_l of the standard grid: 4 x 2^d per voxel)
     pass D  (3D) reads 2N2 x values, keeps N2                      (32C, 16C)
     pass E  reads the G1 lines and S, keeps N along the last axis
 
@@ -46,11 +47,11 @@ def pass_bytes(nv, ncoils, d3):
     if not d3:
         half = 2 * nv                              # N1 x 2 N2 points
         return {"toep_pass_a": 8 * nv + 4 * nv + 8 * c * nv + 8 * c * half,
-                "toep_pass_c": 8 * c * half * 2 + 8 * 4 * nv,
+                "toep_pass_c": 8 * c * half * 2 + 4 * 4 * nv,
                 "toep_pass_e": 8 * c * half + 8 * c * nv}
     return {"toep_pass_a": 8 * nv + 4 * nv + 8 * c * nv + 16 * c * nv,
             "toep_pass_b": 16 * c * nv + 32 * c * nv,
-            "toep_pass_c": 2 * 32 * c * nv + 8 * 8 * nv,
+            "toep_pass_c": 2 * 32 * c * nv + 4 * 8 * nv,
             "toep_pass_d": 32 * c * nv + 16 * c * nv,
             "toep_pass_e": 16 * c * nv + 8 * c * nv}
 
