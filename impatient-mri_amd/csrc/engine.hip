@@ -267,6 +267,27 @@ extern "C" int imp_device_free(imp_ctx *ctx, void *dptr) {
     return IMP_OK;
 }
 
+extern "C" int imp_host_alloc(imp_ctx *ctx, size_t bytes, void **hptr) {
+    if (!ctx || !hptr) return fail(IMP_EINVAL, "imp_host_alloc: NULL argument");
+    IMP_HIP(hipSetDevice(ctx->device));
+    if (hipHostMalloc(hptr, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        *hptr = nullptr;
+        return fail(IMP_ENOMEM, "imp_host_alloc: %zu bytes", bytes);
+    }
+    return IMP_OK;
+}
+
+extern "C" int imp_host_free(imp_ctx *ctx, void *hptr) {
+    if (!ctx) return fail(IMP_EINVAL, "imp_host_free: NULL ctx");
+    if (hptr) {
+        // a copy from it may still be queued on the stream
+        IMP_HIP(hipStreamSynchronize(ctx->stream));
+        IMP_HIP(hipHostFree(hptr));
+    }
+    return IMP_OK;
+}
+
 extern "C" int imp_copy_h2d(imp_ctx *ctx, void *dst, const void *src, size_t bytes) {
     if (!ctx) return fail(IMP_EINVAL, "imp_copy_h2d: NULL ctx");
     IMP_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));

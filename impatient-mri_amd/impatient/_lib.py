@@ -115,8 +115,23 @@ class Context:
 
     def close(self):
         if getattr(self, "h", None):
+            st = getattr(self, "_staging", None)
+            if st is not None:
+                st.free()
+                self._staging = None
             load().imp_ctx_destroy(self.h)
             self.h = None
+
+    def staging(self, n: int) -> np.ndarray:
+        """A page-locked float32 host array of at least n values, kept by
+        the context and grown on demand (uploads from it run at the link's
+        DMA rate).  Its contents are valid until the next call."""
+        st = getattr(self, "_staging", None)
+        if st is None or st.size < n:
+            if st is not None:
+                st.free()
+            self._staging = st = HostStaging(self, max(n, 1))
+        return st.array[:n]
 
     def __del__(self):
         try:
@@ -184,6 +199,24 @@ class DeviceArray:
                 self.free()
         except Exception:
             pass
+
+
+class HostStaging:
+    """Page-locked host buffer (imp_host_alloc / imp_host_free) of float32."""
+
+    def __init__(self, ctx: Context, size: int):
+        self.ctx = ctx
+        self.size = int(size)
+        p = C.c_void_p()
+        call("imp_host_alloc", ctx.h, C.c_size_t(4 * self.size), C.byref(p))
+        self.ptr = p.value
+        self.array = np.ctypeslib.as_array((C.c_float * self.size).from_address(self.ptr))
+
+    def free(self):
+        if self.ptr:
+            self.array = None
+            load().imp_host_free(self.ctx.h, C.c_void_p(self.ptr))
+            self.ptr = None
 
 
 class DeviceView:

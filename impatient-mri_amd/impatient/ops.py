@@ -83,8 +83,9 @@ def nudft_adjoint_batch(ctx: Context, jobs, chunk_scale=0):
     dicts with kx, ky, t, x, y, fm, d [C, M], sens [C, N] (numpy).  Returns
     the list of images."""
     from ._lib import NudftJob
-    # Every input goes into one host buffer and one device buffer (one
-    # upload, no allocation per array); fields start on 256-byte boundaries.
+    # Every input goes into the context's page-locked staging buffer and one
+    # device buffer (one upload at the DMA rate, no allocation per array);
+    # fields start on 256-byte boundaries.
     def padded(n):
         return -(-n // 64) * 64
 
@@ -106,7 +107,7 @@ def nudft_adjoint_batch(ctx: Context, jobs, chunk_scale=0):
             n_in += padded(v.size)
         layout.append((nk, nx, ncoils, fields, n_out))
         n_out += 2 * padded(nx)
-    host = np.empty(max(n_in, 1), np.float32)
+    host = ctx.staging(max(n_in, 1))
     _fill(host, [f for _, _, _, fields, _ in layout for f in fields.values()])
     dev_in = ctx.to_device(host)
     dev_out = ctx.alloc((max(n_out, 1),))

@@ -80,6 +80,11 @@ int imp_device_free(imp_ctx *ctx, void *dptr);
 int imp_copy_h2d(imp_ctx *ctx, void *dst, const void *src, size_t bytes);
 int imp_copy_d2h(imp_ctx *ctx, void *dst, const void *src, size_t bytes);
 int imp_memset_d(imp_ctx *ctx, void *dst, int value, size_t bytes);
+/* Page-locked host memory (hipHostMalloc) for staging uploads: copies from
+ * it run at the link's DMA rate.  No counterpart in the reference; for
+ * callers that upload large inputs repeatedly (the batched F^H d). */
+int imp_host_alloc(imp_ctx *ctx, size_t bytes, void **hptr);
+int imp_host_free(imp_ctx *ctx, void *hptr);
 
 /* ------------------------------------------------------------- operators
  * Direct, field-corrected NUDFT with SENSE (device pointers).
