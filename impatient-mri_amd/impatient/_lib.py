@@ -119,6 +119,10 @@ class Context:
             if st is not None:
                 st.free()
                 self._staging = None
+            dst = getattr(self, "_dstaging", None)
+            if dst is not None:
+                dst.free()
+                self._dstaging = None
             load().imp_ctx_destroy(self.h)
             self.h = None
 
@@ -132,6 +136,18 @@ class Context:
                 st.free()
             self._staging = st = HostStaging(self, max(n, 1))
         return st.array[:n]
+
+    def device_staging(self, n: int) -> "DeviceArray":
+        """A float32 device buffer of at least n values, kept by the context
+        and grown on demand (the device side of staging(): no allocation or
+        free, which synchronises the device, per upload).  Its contents are
+        valid until the next call."""
+        d = getattr(self, "_dstaging", None)
+        if d is None or d.shape[0] < n:
+            if d is not None:
+                d.free()
+            self._dstaging = d = DeviceArray(self, (max(n, 1),))
+        return d
 
     def __del__(self):
         try:

@@ -83,9 +83,9 @@ def nudft_adjoint_batch(ctx: Context, jobs, chunk_scale=0):
     dicts with kx, ky, t, x, y, fm, d [C, M], sens [C, N] (numpy).  Returns
     the list of images."""
     from ._lib import NudftJob
-    # Every input goes into the context's page-locked staging buffer and one
-    # device buffer (one upload at the DMA rate, no allocation per array);
-    # fields start on 256-byte boundaries.
+    # Every input goes into the context's page-locked staging buffer and its
+    # device staging buffer (one upload at the DMA rate, no allocation per
+    # array or call); fields start on 256-byte boundaries.
     def padded(n):
         return -(-n // 64) * 64
 
@@ -109,7 +109,9 @@ def nudft_adjoint_batch(ctx: Context, jobs, chunk_scale=0):
         n_out += 2 * padded(nx)
     host = ctx.staging(max(n_in, 1))
     _fill(host, [f for _, _, _, fields, _ in layout for f in fields.values()])
-    dev_in = ctx.to_device(host)
+    dev_in = ctx.device_staging(host.size)
+    call("imp_copy_h2d", ctx.h, C.c_void_p(dev_in.ptr), host.ctypes.data_as(C.c_void_p),
+         C.c_size_t(host.nbytes))
     dev_out = ctx.alloc((max(n_out, 1),))
     arr = (NudftJob * len(jobs))()
     for j, (nk, nx, ncoils, fields, o) in enumerate(layout):
